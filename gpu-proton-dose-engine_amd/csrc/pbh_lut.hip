@@ -1,0 +1,536 @@
+// pbh_lut.hip -- FP64 physics look-up tables on the device: CSDA range table,
+// per-layer pristine IDD and Fermi-Eyges lateral sigma.
+//
+// Restates physics.py:72-139 (stopping power, range table, interpolation),
+// :154-174 and :310-322 (range straggling), :185-286 (scattering power,
+// Fermi-Eyges moments) and :328-387 (pristine IDD), in FP64 throughout: the
+// moment sums cancel strongly (B = z^2 S0 - 2 z S1 + S2 loses ~8 digits), so
+// B is evaluated as the reference's explicit double sum, term by term.
+//
+// FP contraction is disabled in this file so that every product and sum rounds
+// as NumPy's does; transcendental functions (log, exp, pow, hypot) are the
+// device library's and may differ from the host's by an ulp.
+//
+// The IDD's range-spread convolution is a zero-padded real FFT in the
+// reference (nfft = 16384 >= 2 n, multiplier exp(-2 (pi f sigma)^2)). By
+// Poisson summation that product is exactly the linear convolution with the
+// sampled Gaussian h / (sqrt(2 pi) sigma) exp(-(m h)^2 / (2 sigma^2)) up to
+// terms below exp(-pi^2 sigma^2 / (2 h^2)) (zero in FP64 for sigma >> h), so
+// the kernel sums that convolution directly, only at the fine-grid points the
+// final interpolation reads.
+#pragma clang fp contract(off)
+
+#include "pbh_common.h"
+
+namespace pbh {
+
+// ---- constants (physics.py:42-65, :151) ----
+constexpr double MP = 938.27208816;
+constexpr double ME = 0.51099895000;
+constexpr double KBB = 0.307075;
+constexpr double ZA = 10.0 / 18.01528;
+constexpr double IMEV = 75.0 * 1e-6;
+constexpr double RHO = 1.0;
+constexpr double X0W = 36.08;
+constexpr double EHL = 13.6;
+constexpr double ELO = 0.5, EHI = 400.0;
+constexpr double PR = 1.77;
+constexpr double BOHR = KBB * ZA * RHO * ME;
+
+constexpr int N_TABLE = 3797;      // 800 + 899 + 899 + 1199 (shared segment ends removed)
+constexpr int LUT_THREADS = 256;
+
+// ---- kinematics and stopping power (physics.py:72-100) ----
+__device__ inline double beta2_of(double E, double* gamma_out)
+{
+    const double g = 1.0 + E / MP;
+    double b2 = 1.0 - 1.0 / (g * g);
+    b2 = b2 < 1e-14 ? 1e-14 : (b2 > 1.0 ? 1.0 : b2);
+    if (gamma_out) *gamma_out = g;
+    return b2;
+}
+
+__device__ inline double stopping_power(double E)
+{
+    double g;
+    const double b2 = beta2_of(E, &g);
+    const double r = ME / MP;
+    const double tmax = 2.0 * ME * b2 * (g * g) / (1.0 + 2.0 * g * r + r * r);
+    double arg = 2.0 * ME * b2 * (g * g) * tmax / (IMEV * IMEV);
+    if (arg < 1.0 + 1e-12) arg = 1.0 + 1e-12;
+    return KBB * ZA * RHO / b2 * (0.5 * log(arg) - b2);
+}
+
+// Local Highland scattering power with L = 1 (ln L = 0). physics.py:202-217
+__device__ inline double scattering_power(double E)
+{
+    double g;
+    const double b2 = beta2_of(E, &g);
+    const double beta = sqrt(b2);
+    double p2 = g * g - 1.0;
+    if (p2 < 0.0) p2 = 0.0;
+    const double pc = sqrt(p2) * MP;
+    double bp = beta * pc;
+    if (bp < 1e-9) bp = 1e-9;
+    const double q = EHL / bp;
+    const double base = q * q / X0W;
+    const double lnL = 0.0;
+    return base * (1.0 + 0.038 * lnL) * (1.076 + 0.038 * lnL);
+}
+
+// ---- NumPy linspace(start, stop, num)[i] ----
+__device__ inline double lin_at(double start, double stop, int num, int i)
+{
+    if (i == num - 1) return stop;
+    const double step = (stop - start) / (double)(num - 1);
+    return (double)i * step + start;
+}
+
+// Energy grid of the range table: index -> energy. physics.py:118-123
+__device__ inline double table_energy(int k)
+{
+    if (k < 800) return lin_at(ELO, 2.0, 800, k);
+    k -= 800;
+    if (k < 899) return lin_at(2.0, 20.0, 900, k + 1);
+    k -= 899;
+    if (k < 899) return lin_at(20.0, 60.0, 900, k + 1);
+    k -= 899;
+    return lin_at(60.0, EHI, 1200, k + 1);
+}
+
+// NumPy interp for sorted xp with the default end values (fp[0], fp[n-1]).
+// Finds j with xp[j] <= x < xp[j+1]; x == xp[j] returns fp[j] exactly.
+__device__ inline double interp_sorted(const double* xp, const double* fp, int n, double x)
+{
+    if (x > xp[n - 1]) return fp[n - 1];
+    if (x < xp[0]) return fp[0];
+    int lo = 0, hi = n - 1;   // invariant xp[lo] <= x, and x < xp[hi] or hi == n-1
+    if (x >= xp[n - 1]) return fp[n - 1];
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (xp[mid] <= x) lo = mid; else hi = mid;
+    }
+    if (xp[lo] == x) return fp[lo];
+    const double slope = (fp[lo + 1] - fp[lo]) / (xp[lo + 1] - xp[lo]);
+    return slope * (x - xp[lo]) + fp[lo];
+}
+
+__device__ inline double csda_range(const double* E, const double* R, double e)
+{
+    return interp_sorted(E, R, N_TABLE, e);
+}
+
+__device__ inline double energy_from_range(const double* E, const double* R, double r)
+{
+    if (r < 0.0) r = 0.0;
+    if (r > R[N_TABLE - 1]) r = R[N_TABLE - 1];
+    return interp_sorted(R, E, N_TABLE, r);
+}
+
+// Block-wide FP64 sum (LUT_THREADS threads).
+__device__ inline double block_sum(double v, double* red)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wv] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < LUT_THREADS / 64; ++i) t += red[i];
+        red[0] = t;
+    }
+    __syncthreads();
+    t = red[0];
+    __syncthreads();
+    return t;
+}
+
+__device__ inline double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// ---- range table: E grid, S, cumulative trapezoid of 1/S ----
+__global__ __launch_bounds__(LUT_THREADS) void range_table_kernel(double* E, double* R)
+{
+    __shared__ double seg[N_TABLE];
+    for (int k = threadIdx.x; k < N_TABLE; k += LUT_THREADS) E[k] = table_energy(k);
+    __syncthreads();
+    for (int k = threadIdx.x; k < N_TABLE - 1; k += LUT_THREADS) {
+        const double e0 = table_energy(k), e1 = table_energy(k + 1);
+        const double i0 = 1.0 / stopping_power(e0), i1 = 1.0 / stopping_power(e1);
+        seg[k] = 0.5 * (i1 + i0) * (e1 - e0);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {   // sequential, as np.cumsum
+        double acc = 0.0;
+        R[0] = 0.0;
+        for (int k = 0; k < N_TABLE - 1; ++k) { acc += seg[k]; R[k + 1] = acc; }
+    }
+}
+
+__global__ void interp_kernel(const double* xp, const double* fp, int nt, const double* x,
+                              double* out, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = interp_sorted(xp, fp, nt, x[i]);
+}
+
+// ---- per-layer scalars: R0, total range spread sigma_R ----
+struct LayerLutArgs {
+    const double* E;      // range table energies
+    const double* R;      // range table ranges
+    const double* energies;
+    int L, nz;
+    double z0, dz;
+    PbhBeamModel m;
+    int n_fine;           // IDD fine grid
+    int nf;               // Fermi-Eyges fine grid
+    double* R0;           // [L]
+    double* sR;           // [L]
+    double* dcsda;        // [L][n_fine]
+    double* gk;           // [L][n_fine]
+    double* T;            // [L][nf]
+    double* hf;           // [nf]
+    double* idd;          // [L][nz]
+    double* sigma;        // [L][nz]
+};
+
+__global__ __launch_bounds__(LUT_THREADS) void layer_scalars_kernel(LayerLutArgs a)
+{
+    __shared__ double red[LUT_THREADS / 64];
+    const int l = blockIdx.x;
+    const double E0 = a.energies[l];
+    const double R0 = csda_range(a.E, a.R, E0);
+    double s_strag = 0.0;
+    if (a.m.straggling == 0) {
+        // sigma^2 = trapezoid of BOHR / (beta^2 S^3) on linspace(0.5, E0, 4000). physics.py:168-174
+        const int n = 4000;
+        double part = 0.0;
+        const double last = lin_at(ELO, E0, n, n - 1), first = lin_at(ELO, E0, n, 0);
+        if (last > first) {
+            for (int i = threadIdx.x; i < n - 1; i += LUT_THREADS) {
+                const double e0 = lin_at(ELO, E0, n, i), e1 = lin_at(ELO, E0, n, i + 1);
+                double b0 = beta2_of(e0, nullptr), b1 = beta2_of(e1, nullptr);
+                b0 = b0 < 1e-12 ? 1e-12 : b0;
+                b1 = b1 < 1e-12 ? 1e-12 : b1;
+                const double S0 = stopping_power(e0), S1 = stopping_power(e1);
+                const double f0 = BOHR / (b0 * (S0 * S0 * S0));
+                const double f1 = BOHR / (b1 * (S1 * S1 * S1));
+                part += (e1 - e0) * (f1 + f0) / 2.0;
+            }
+        }
+        s_strag = sqrt(block_sum(part, red));
+    } else if (a.m.straggling == 1) {
+        s_strag = a.m.icru_coef * pow(R0, a.m.icru_exp);
+    }
+    if (threadIdx.x == 0) {
+        const double s_espr = PR * a.m.energy_spread * R0;
+        a.R0[l] = R0;
+        a.sR[l] = hypot(s_strag, s_espr);
+    }
+}
+
+// ---- IDD, step 1: energy-conserving CSDA binning and the Gaussian kernel ----
+__global__ __launch_bounds__(LUT_THREADS) void idd_prep_kernel(LayerLutArgs a)
+{
+    const int l = blockIdx.x;
+    const int n = a.n_fine;
+    const double R0 = a.R0[l], sR = a.sR[l];
+    const double top = R0 + 8.0 * sR + 0.2;
+    const double h = lin_at(0.0, top, n, 1) - lin_at(0.0, top, n, 0);
+    double* dc = a.dcsda + (size_t)l * n;
+    double* gk = a.gk + (size_t)l * n;
+    for (int i = threadIdx.x; i < n; i += LUT_THREADS) {
+        // bin edges zb = linspace(0, top, n + 1); Eb = E(R0 - zb) inside the range, else 0
+        const double zb0 = lin_at(0.0, top, n + 1, i), zb1 = lin_at(0.0, top, n + 1, i + 1);
+        const double eb0 = zb0 < R0 ? energy_from_range(a.E, a.R, (R0 - zb0) > 0.0 ? R0 - zb0 : 0.0) : 0.0;
+        const double eb1 = zb1 < R0 ? energy_from_range(a.E, a.R, (R0 - zb1) > 0.0 ? R0 - zb1 : 0.0) : 0.0;
+        const double zf = lin_at(0.0, top, n, i);
+        const double flu = exp(-zf / a.m.attenuation_length);
+        dc[i] = flu * (eb0 - eb1) / h;
+        double g;
+        if (sR > 0.0) {
+            const double t = (double)i * h;
+            g = h / (sqrt(2.0 * kPi) * sR) * exp(-(t * t) / (2.0 * sR * sR));
+        } else {
+            g = (i == 0) ? 1.0 : 0.0;
+        }
+        gk[i] = g;
+    }
+}
+
+// Convolved fine-grid dose at index j: sum_i dcsda[i] g[|j - i|] (one wave).
+__device__ inline double conv_at(const double* dc, const double* gk, int n, int j)
+{
+    double s = 0.0;
+    for (int i = (threadIdx.x & 63); i < n; i += 64) {
+        const int m = j > i ? j - i : i - j;
+        s += dc[i] * gk[m];
+    }
+    return wave_sum(s);
+}
+
+constexpr int Z_PER_BLOCK = 16;
+
+// ---- IDD, step 2: evaluate the convolution at the bracketing fine points and
+// interpolate onto z (left = right = 0). One wave per depth. ----
+__global__ __launch_bounds__(LUT_THREADS) void idd_eval_kernel(LayerLutArgs a)
+{
+    const int l = blockIdx.y;
+    const int n = a.n_fine;
+    const double R0 = a.R0[l], sR = a.sR[l];
+    const double top = R0 + 8.0 * sR + 0.2;
+    const double h = lin_at(0.0, top, n, 1) - lin_at(0.0, top, n, 0);
+    const double* dc = a.dcsda + (size_t)l * n;
+    const double* gk = a.gk + (size_t)l * n;
+    const int wv = threadIdx.x >> 6;
+    for (int q = wv; q < Z_PER_BLOCK; q += LUT_THREADS / 64) {
+        const int k = blockIdx.x * Z_PER_BLOCK + q;
+        if (k >= a.nz) break;
+        const double z = a.z0 + (double)k * a.dz;
+        double out;
+        if (z > top || z < 0.0) {
+            out = 0.0;
+        } else {
+            int j = (int)(z / h);
+            if (j > n - 1) j = n - 1;
+            while (j > 0 && lin_at(0.0, top, n, j) > z) --j;
+            while (j < n - 1 && lin_at(0.0, top, n, j + 1) <= z) ++j;
+            const double zj = lin_at(0.0, top, n, j);
+            if (j == n - 1 || zj == z) {
+                out = conv_at(dc, gk, n, j);
+            } else {
+                const double d0 = conv_at(dc, gk, n, j), d1 = conv_at(dc, gk, n, j + 1);
+                const double slope = (d1 - d0) / (lin_at(0.0, top, n, j + 1) - zj);
+                out = slope * (z - zj) + d0;
+            }
+        }
+        if ((threadIdx.x & 63) == 0) a.idd[(size_t)l * a.nz + k] = out;
+    }
+}
+
+// ---- Fermi-Eyges, step 1: scattering power and np.gradient weights on the
+// fine grid zf = linspace(z[0], z[-1], nf). physics.py:255-264 ----
+__global__ __launch_bounds__(LUT_THREADS) void fe_prep_kernel(LayerLutArgs a)
+{
+    const int l = blockIdx.x;
+    const int nf = a.nf;
+    const double zlo = a.z0, zhi = a.z0 + (double)(a.nz - 1) * a.dz;
+    const double R0 = a.R0[l];
+    for (int i = threadIdx.x; i < nf; i += LUT_THREADS) {
+        const double zf = lin_at(zlo, zhi, nf, i);
+        double t = 0.0;
+        if (zf < R0) {
+            const double rem = (R0 - zf) > 0.0 ? R0 - zf : 0.0;
+            double e = energy_from_range(a.E, a.R, rem);
+            if (e < 1e-3) e = 1e-3;
+            t = scattering_power(e);
+        }
+        a.T[(size_t)l * nf + i] = t;
+        if (l == 0) {
+            double hv;
+            if (i == 0) hv = lin_at(zlo, zhi, nf, 1) - lin_at(zlo, zhi, nf, 0);
+            else if (i == nf - 1) hv = lin_at(zlo, zhi, nf, nf - 1) - lin_at(zlo, zhi, nf, nf - 2);
+            else hv = (lin_at(zlo, zhi, nf, i + 1) - lin_at(zlo, zhi, nf, i - 1)) / 2.0;
+            a.hf[i] = hv;
+        }
+    }
+}
+
+// B at fine index j: sum over i with zf_j - zf_i > 0 of (dz^2 T_i) h_i (one wave).
+__device__ inline double bsum_at(const double* T, const double* hf, double zlo, double zhi, int nf, int j)
+{
+    const double zj = lin_at(zlo, zhi, nf, j);
+    double s = 0.0;
+    for (int i = (threadIdx.x & 63); i < nf; i += 64) {
+        const double d = zj - lin_at(zlo, zhi, nf, i);
+        if (d > 0.0) s += d * d * T[i] * hf[i];
+    }
+    return wave_sum(s);
+}
+
+// ---- Fermi-Eyges, step 2: B interpolated onto z, sigma = sqrt(var). ----
+__global__ __launch_bounds__(LUT_THREADS) void fe_eval_kernel(LayerLutArgs a)
+{
+    const int l = blockIdx.y;
+    const int nf = a.nf;
+    const double zlo = a.z0, zhi = a.z0 + (double)(a.nz - 1) * a.dz;
+    const double* T = a.T + (size_t)l * nf;
+    const int wv = threadIdx.x >> 6;
+    for (int q = wv; q < Z_PER_BLOCK; q += LUT_THREADS / 64) {
+        const int k = blockIdx.x * Z_PER_BLOCK + q;
+        if (k >= a.nz) break;
+        const double z = a.z0 + (double)k * a.dz;
+        double B;
+        if (z >= zhi) {
+            B = bsum_at(T, a.hf, zlo, zhi, nf, nf - 1);
+        } else if (z <= zlo) {
+            B = bsum_at(T, a.hf, zlo, zhi, nf, 0);
+        } else {
+            const double step = (zhi - zlo) / (double)(nf - 1);
+            int j = (int)((z - zlo) / step);
+            if (j > nf - 1) j = nf - 1;
+            while (j > 0 && lin_at(zlo, zhi, nf, j) > z) --j;
+            while (j < nf - 1 && lin_at(zlo, zhi, nf, j + 1) <= z) ++j;
+            const double zj = lin_at(zlo, zhi, nf, j);
+            if (j == nf - 1 || zj == z) {
+                B = bsum_at(T, a.hf, zlo, zhi, nf, j);
+            } else {
+                const double b0 = bsum_at(T, a.hf, zlo, zhi, nf, j);
+                const double b1 = bsum_at(T, a.hf, zlo, zhi, nf, j + 1);
+                const double slope = (b1 - b0) / (lin_at(zlo, zhi, nf, j + 1) - zj);
+                B = slope * (z - zj) + b0;
+            }
+        }
+        const double sx = a.m.sigma_x0, st = a.m.sigma_theta0, rho = a.m.rho0;
+        double var = sx * sx + 2.0 * rho * sx * st * z + st * st * (z * z) + B;
+        if (var < 0.0) var = 0.0;
+        if ((threadIdx.x & 63) == 0) a.sigma[(size_t)l * a.nz + k] = sqrt(var);
+    }
+}
+
+// Per-layer summary: R0, sigma_R, depth and value of the first IDD maximum (np.argmax).
+__global__ void layer_info_kernel(const double* idd, const double* R0, const double* sR,
+                                  int L, int nz, double z0, double dz, double* info)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= L) return;
+    const double* row = idd + (size_t)l * nz;
+    int best = 0;
+    for (int k = 1; k < nz; ++k)
+        if (row[k] > row[best]) best = k;
+    info[4 * l + 0] = R0[l];
+    info[4 * l + 1] = sR[l];
+    info[4 * l + 2] = z0 + (double)best * dz;
+    info[4 * l + 3] = row[best];
+}
+
+// Fine-grid size of the moment integrals (physics.py:256-258), evaluated on the
+// host exactly as the reference does: span = z[-1] - z[0].
+static int fe_fine_count(int nz, double z0, double dz, int n_sub, double max_step)
+{
+    const double zl = z0 + (double)(nz - 1) * dz;
+    const double span = zl - z0;
+    const double ms = max_step > 1e-6 ? max_step : 1e-6;
+    const int need = span > 0 ? (int)ceil(span / ms) + 1 : 2;
+    int nf = (nz - 1) * n_sub + 1;
+    if (need > nf) nf = need;
+    if (nf < 2) nf = 2;
+    return nf;
+}
+
+struct LutLayout {
+    size_t table, scal, dcsda, gk, T, hf, total;
+    int nf;
+};
+
+static LutLayout lut_layout(const PbhBeamModel* m, int nz, double z0, double dz, int L)
+{
+    LutLayout o;
+    o.nf = fe_fine_count(nz, z0, dz, m->fe_n_sub, m->fe_max_step);
+    size_t off = 0;
+    auto take = [&](size_t doubles) { size_t p = off; off += (doubles * 8 + 255) & ~(size_t)255; return p; };
+    o.table = take(2 * (size_t)N_TABLE);
+    o.scal = take(2 * (size_t)L);
+    o.dcsda = take((size_t)L * m->n_fine_idd);
+    o.gk = take((size_t)L * m->n_fine_idd);
+    o.T = take((size_t)L * o.nf);
+    o.hf = take((size_t)o.nf);
+    o.total = off;
+    return o;
+}
+
+}  // namespace pbh
+
+using namespace pbh;
+
+extern "C" int pbh_range_table_size(void) { return N_TABLE; }
+
+extern "C" int pbh_range_table(double* E, double* R, void* stream)
+{
+    if (!E || !R) return PBH_ERR_NULL_ARG;
+    hipLaunchKernelGGL(range_table_kernel, dim3(1), dim3(LUT_THREADS), 0, (hipStream_t)stream, E, R);
+    return launch_rc("range_table_kernel");
+}
+
+extern "C" int pbh_interp_table(const double* xp, const double* fp, int n_table,
+                                const double* x, double* out, int n, void* stream)
+{
+    if (!xp || !fp || !x || !out) return PBH_ERR_NULL_ARG;
+    if (n_table < 2 || n < 0) return PBH_ERR_BAD_DIM;
+    if (n == 0) return PBH_OK;
+    hipLaunchKernelGGL(interp_kernel, dim3(cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream,
+                       xp, fp, n_table, x, out, n);
+    return launch_rc("interp_kernel");
+}
+
+static int check_model(const PbhBeamModel* m)
+{
+    if (!m) return PBH_ERR_NULL_ARG;
+    if (m->n_fine_idd < 2 || m->n_fine_idd > (1 << 20)) return PBH_ERR_BAD_DIM;
+    if (m->fe_n_sub < 1 || !(m->fe_max_step > 0)) return PBH_ERR_BAD_DIM;
+    if (m->straggling < 0 || m->straggling > 2) return PBH_ERR_UNSUPPORTED;
+    if (!(m->attenuation_length > 0)) return PBH_ERR_BAD_DIM;
+    return PBH_OK;
+}
+
+extern "C" size_t pbh_lut_workspace_bytes(const PbhBeamModel* m, int nz, int n_layers)
+{
+    if (check_model(m) != PBH_OK || nz < 2 || n_layers < 1) return 0;
+    return lut_layout(m, nz, 0.0, 1.0, n_layers).total;
+}
+
+extern "C" int pbh_build_lut(const PbhBeamModel* m, int nz, double z0, double dz, int n_layers,
+                             const double* energies, double* idd, double* sigma, double* layer_info,
+                             void* workspace, size_t workspace_bytes, void* stream)
+{
+    int rc = check_model(m);
+    if (rc != PBH_OK) return rc;
+    if (!energies || !idd || !sigma || !workspace) return PBH_ERR_NULL_ARG;
+    if (nz < 2 || nz > PBH_MAX_DIM || n_layers < 1 || n_layers > (1 << 20)) return PBH_ERR_BAD_DIM;
+    if (!(dz > 0)) return PBH_ERR_BAD_DIM;
+    const LutLayout lay = lut_layout(m, nz, z0, dz, n_layers);
+    if (lay.nf > (1 << 22)) return PBH_ERR_UNSUPPORTED;
+    if (workspace_bytes < lay.total) return PBH_ERR_WORKSPACE;
+    hipStream_t s = (hipStream_t)stream;
+    char* ws = (char*)workspace;
+    LayerLutArgs a;
+    a.E = (double*)(ws + lay.table);
+    a.R = a.E + N_TABLE;
+    a.energies = energies;
+    a.L = n_layers; a.nz = nz; a.z0 = z0; a.dz = dz;
+    a.m = *m;
+    a.n_fine = m->n_fine_idd;
+    a.nf = lay.nf;
+    a.R0 = (double*)(ws + lay.scal);
+    a.sR = a.R0 + n_layers;
+    a.dcsda = (double*)(ws + lay.dcsda);
+    a.gk = (double*)(ws + lay.gk);
+    a.T = (double*)(ws + lay.T);
+    a.hf = (double*)(ws + lay.hf);
+    a.idd = idd; a.sigma = sigma;
+
+    hipLaunchKernelGGL(range_table_kernel, dim3(1), dim3(LUT_THREADS), 0, s, (double*)a.E, (double*)a.R);
+    if ((rc = launch_rc("range_table_kernel")) != PBH_OK) return rc;
+    hipLaunchKernelGGL(layer_scalars_kernel, dim3(n_layers), dim3(LUT_THREADS), 0, s, a);
+    if ((rc = launch_rc("layer_scalars_kernel")) != PBH_OK) return rc;
+    hipLaunchKernelGGL(idd_prep_kernel, dim3(n_layers), dim3(LUT_THREADS), 0, s, a);
+    if ((rc = launch_rc("idd_prep_kernel")) != PBH_OK) return rc;
+    const dim3 eval_grid(cdiv(nz, Z_PER_BLOCK), n_layers);
+    hipLaunchKernelGGL(idd_eval_kernel, eval_grid, dim3(LUT_THREADS), 0, s, a);
+    if ((rc = launch_rc("idd_eval_kernel")) != PBH_OK) return rc;
+    hipLaunchKernelGGL(fe_prep_kernel, dim3(n_layers), dim3(LUT_THREADS), 0, s, a);
+    if ((rc = launch_rc("fe_prep_kernel")) != PBH_OK) return rc;
+    hipLaunchKernelGGL(fe_eval_kernel, eval_grid, dim3(LUT_THREADS), 0, s, a);
+    if ((rc = launch_rc("fe_eval_kernel")) != PBH_OK) return rc;
+    if (layer_info) {
+        hipLaunchKernelGGL(layer_info_kernel, dim3(cdiv(n_layers, 64)), dim3(64), 0, s,
+                           (const double*)idd, (const double*)a.R0, (const double*)a.sR,
+                           n_layers, nz, z0, dz, layer_info);
+        if ((rc = launch_rc("layer_info_kernel")) != PBH_OK) return rc;
+    }
+    return PBH_OK;
+}

@@ -1,0 +1,8 @@
+"""pbdose -- per-layer-exact Fermi-Eyges pencil-beam dose on MI355X (gfx950).
+
+Same module names as the reference package (``physics``, ``model``,
+``engines``); the dose, look-up-table, adjoint and gamma arithmetic runs in the
+hand-written HIP kernels of ``libpbdose_hip.so``.
+"""
+__version__ = "0.1.0"
+__all__ = ["physics", "model", "engines"]

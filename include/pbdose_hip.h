@@ -1,0 +1,184 @@
+/*
+ * pbdose_hip.h -- C ABI of the MI355X (gfx950) per-layer-exact Fermi-Eyges
+ * pencil-beam dose library (libpbdose_hip.so).
+ *
+ * Model (one energy layer l, one depth slice z):
+ *     dose[z] += IDD_l(z) * Gy_l(z) * W_l * Gx_l(z)^T
+ *     G[k][i]  = exp(-(v_k - s_i)^2 / (2 s^2)) / (sqrt(2 pi) s),  s = max(sigma_l(z), 1e-9)
+ * summed over layers, sigma exact per layer.
+ *
+ * Conventions shared by every entry point
+ *   - all array arguments are DEVICE pointers, owned by the caller;
+ *   - dose layout (nz, ny, nx), x fastest; weights (L, nsy, nsx), i fastest;
+ *     lookup tables (L, nz); voxel centres o + k d; spot centres
+ *     x0 + (i - (n-1)/2) ds;
+ *   - `stream` is a hipStream_t (NULL = default stream); every call is
+ *     asynchronous on it and never synchronises the device;
+ *   - return 0 (PBH_OK) or a negative PBH_ERR_* code, readable through
+ *     pbh_strerror(); no entry point exits the process;
+ *   - no entry point allocates device memory: scratch space is a caller-owned
+ *     workspace whose size the matching *_workspace_bytes() query returns.
+ *
+ * Reference interfaces replaced (penglinliu52/gpu-proton-dose-engine):
+ *   engines.py:208-283  torch_separable(problem, per_layer=True)  -> pbh_forward
+ *   engines.py:172-202  separable_numpy (the FP64 model)          -> pbh_forward
+ *   physics.py:111-139  _range_table / csda_range / energy_from_range
+ *                                                   -> pbh_range_table, pbh_interp_table
+ *   physics.py:328-387  idd_pristine(normalize=False)
+ *   physics.py:221-286  fermi_eyges_moments(...)["sigma_x"]      -> pbh_build_lut
+ *   model.py:225-233    per-layer LUT loop of the field builders  -> pbh_build_lut
+ *   gamma.py:81-160     gamma_index (global)                     -> pbh_gamma_index
+ *   pencil_beam.cu:647-699 PB_ERR_* codes and pb_strerror         -> PBH_ERR_*, pbh_strerror
+ *   pencil_beam.cu:717-730 pb_problem_validate                    -> pbh_validate
+ *   pencil_beam.cu:907-919 pb_problem_bytes (sizing query)        -> *_workspace_bytes
+ * The weighted-quadratic adjoint (pbh_adjoint_w, pbh_adjoint_sigma) and the
+ * robust-scenario batch (pbh_batch_*) have no counterpart in the reference;
+ * they serve inverse planning on top of the forward model above.
+ */
+#ifndef PBDOSE_HIP_H
+#define PBDOSE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes (pencil_beam.cu:647-653) ---- */
+#define PBH_OK                 0
+#define PBH_ERR_NULL_ARG      -1
+#define PBH_ERR_BAD_DIM       -2
+#define PBH_ERR_ZGRID_MISMATCH -3
+#define PBH_ERR_HIP           -4
+#define PBH_ERR_UNSUPPORTED   -5
+#define PBH_ERR_NO_SMEM       -6
+#define PBH_ERR_WORKSPACE     -7
+
+/* ---- arithmetic forms of the dose contraction ---- */
+#define PBH_PREC_FP32        0  /* FP32 factors, FP32 MFMA, FP32 accumulation */
+#define PBH_PREC_BF16        1  /* BF16 factors on MFMA, FP32 accumulation */
+#define PBH_PREC_FP16        2  /* FP16 factors on MFMA, FP32 accumulation */
+#define PBH_PREC_BF16X3      3  /* split hi+lo BF16, three products, everywhere */
+#define PBH_PREC_FP16X3      4  /* split hi+lo FP16, three products, everywhere */
+#define PBH_PREC_BF16X3_PEAK 5  /* BF16; split products only within 1 cm of each layer's Bragg peak */
+#define PBH_PREC_FP16X3_PEAK 6  /* FP16; split products only within 1 cm of each layer's Bragg peak */
+
+/* Voxel grid (model.py:27-79). */
+typedef struct PbhGrid {
+    int32_t nx, ny, nz, _pad;
+    double dx, dy, dz;
+    double ox, oy, oz;
+} PbhGrid;
+
+/* Spot lattice (model.py:85-131); weights live in a separate device array. */
+typedef struct PbhLattice {
+    int32_t n_sx, n_sy, n_layers, _pad;
+    double dx_spot, dy_spot;
+    double x0, y0;
+} PbhLattice;
+
+/* Beam / IDD model parameters (physics.py:292-325, model.py:193-205). */
+typedef struct PbhBeamModel {
+    double sigma_x0;            /* cm, initial beam size            (0.30) */
+    double sigma_theta0;        /* rad, initial divergence          (0.0)  */
+    double rho0;                /* position-angle correlation       (0.0)  */
+    double energy_spread;       /* relative 1-sigma energy spread   (0.0033) */
+    double attenuation_length;  /* cm, primary fluence attenuation  (150.0) */
+    double fe_max_step;         /* cm, Fermi-Eyges fine-grid step    (0.02) */
+    int32_t n_fine_idd;         /* fine binning grid points          (6000) */
+    int32_t fe_n_sub;           /* Fermi-Eyges subdivision           (4)    */
+    int32_t straggling;         /* 0 = Bohr integral, 1 = ICRU49 fit, 2 = none */
+    int32_t _pad;
+    double icru_coef, icru_exp; /* ICRU49 fit sigma = coef * R0^exp (0.012, 0.935) */
+} PbhBeamModel;
+
+const char* pbh_strerror(int code);
+const char* pbh_version(void);
+
+/* Checks grid and lattice dimensions against the library limits
+ * (pencil_beam.cu:717-730). */
+int pbh_validate(const PbhGrid* g, const PbhLattice* lat);
+
+/* ---------------------------------------------------------------------------
+ * Physics look-up tables, FP64 on the device.
+ * ------------------------------------------------------------------------- */
+
+/* Number of points in the CSDA range table (3797). */
+int pbh_range_table_size(void);
+
+/* Builds the CSDA range table: E[n] (MeV) and R[n] (cm), n = pbh_range_table_size().
+ * physics.py:111-127 */
+int pbh_range_table(double* E, double* R, void* stream);
+
+/* out[i] = linear interpolation of (xp, fp) at x[i] after clamping x to
+ * [xp[0], xp[n-1]] (csda_range: xp = E, fp = R; energy_from_range: xp = R,
+ * fp = E). physics.py:130-139 */
+int pbh_interp_table(const double* xp, const double* fp, int n_table,
+                     const double* x, double* out, int n, void* stream);
+
+/* Workspace for pbh_build_lut with nz depths and n_layers layers. */
+size_t pbh_lut_workspace_bytes(const PbhBeamModel* m, int nz, int n_layers);
+
+/* Per-layer IDD (fluence-normalised, MeV/cm) and lateral sigma (cm) on the
+ * depths z_k = z0 + k dz, k < nz, for layer energies energies[L] (device).
+ * Writes idd[L*nz], sigma[L*nz] (FP64) and, if layer_info != NULL,
+ * layer_info[L][4] = {CSDA range R0, range spread sigma_R, depth of the IDD
+ * maximum on the grid (first maximum), IDD maximum}.
+ * physics.py:221-286, :328-387; model.py:225-233 */
+int pbh_build_lut(const PbhBeamModel* m, int nz, double z0, double dz,
+                  int n_layers, const double* energies,
+                  double* idd, double* sigma, double* layer_info,
+                  void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Forward dose.
+ * ------------------------------------------------------------------------- */
+size_t pbh_forward_workspace_bytes(const PbhGrid* g, const PbhLattice* lat, int precision);
+
+/* dose (nz, ny, nx) FP32 := sum_l IDD_l Gy_l W_l Gx_l^T for weights w
+ * (L, nsy, nsx) FP32 and FP64 tables idd, sigma (L, nz). The dose array is
+ * overwritten, not accumulated. engines.py:208-283 (per_layer=True) */
+int pbh_forward(const PbhGrid* g, const PbhLattice* lat,
+                const float* w, const double* idd, const double* sigma,
+                float* dose, int precision,
+                void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Adjoint of the forward map (inverse planning).
+ * ------------------------------------------------------------------------- */
+size_t pbh_adjoint_workspace_bytes(const PbhGrid* g, const PbhLattice* lat, int precision);
+
+/* grad_w (L, nsy, nsx) FP32 := sum_z IDD_l(z) Gy_l(z)^T R_z Gx_l(z) for a dose
+ * cotangent R (nz, ny, nx). Deterministic (no atomics).
+ * If grad_sigma != NULL it also writes grad_sigma (L, nz) FP64:
+ *     d<R, dose>/d sigma_l(z)  and, if grad_idd != NULL, grad_idd (L, nz) FP64:
+ *     d<R, dose>/d IDD_l(z). */
+int pbh_adjoint(const PbhGrid* g, const PbhLattice* lat,
+                const float* w, const double* idd, const double* sigma,
+                const float* R, float* grad_w, double* grad_idd, double* grad_sigma,
+                int precision, void* workspace, size_t workspace_bytes, void* stream);
+
+/* R = 2 wv (dose - presc) and J = sum wv (dose - presc)^2 (J written to
+ * j_out[0] as FP64). Fused residual of the weighted-quadratic objective. */
+size_t pbh_objective_workspace_bytes(const PbhGrid* g);
+int pbh_objective(const PbhGrid* g, const float* dose, const float* presc,
+                  const float* vox_w, float* R, double* j_out,
+                  void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Gamma index (global, integer-voxel DTA offsets). gamma.py:81-160
+ * ------------------------------------------------------------------------- */
+size_t pbh_gamma_workspace_bytes(const PbhGrid* g);
+/* counts[0] = voxels evaluated (ref >= thr * ref_max), counts[1] = voxels with
+ * gamma <= 1; gamma_out (nz, ny, nx) FP32 may be NULL. ref_max <= 0 means
+ * "use max(ref)". */
+int pbh_gamma_index(const PbhGrid* g, const float* ref, const float* eval,
+                    double dose_pct, double dta_mm, double threshold_pct,
+                    double ref_max, float* gamma_out, unsigned long long* counts,
+                    void* workspace, size_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PBDOSE_HIP_H */

@@ -1,0 +1,473 @@
+"""FP64 CPU oracle for the per-layer-exact pencil-beam dose path.
+
+TEST INFRASTRUCTURE ONLY -- see ``oracle/__init__.py``. The shipped package
+never imports this module; it is the checker the HIP path is compared against.
+
+Every function restates one piece of the reference's arithmetic and cites the
+reference file:line it follows. Where NumPy's own numerics decide the last bits
+(``linspace``, ``interp``, ``cumsum``, ``fft``) the oracle calls the same NumPy
+routine, because the reference does.
+
+Pinning (what fixes each function, beyond retyping the formula):
+
+==============================  ==============================================
+function                        pinned by
+==============================  ==============================================
+stopping_power                  NIST/reference table values (published 4-5
+                                digits), Bethe-Bloch closed form at 3 energies
+range_table / csda_range        reference golden ranges 200/250/300 MeV; the
+energy_from_range               16-digit clinical_layer_energies endpoints;
+                                round trip R(E(R)) = R; reference fixtures
+straggling_bohr / sigma_range   published sigma_R at 100/150/200/230 MeV;
+                                reference fixtures
+fe_moments                      constant-T closed forms (A0=Tz, A2=Tz^3/6,
+                                B=Tz^3/3); published sigma_x table at 200 MeV;
+                                reference fixtures
+idd_pristine                    FFT == direct Gaussian convolution; energy
+                                conservation of the CSDA binning; published
+                                peak IDD values; reference fixtures
+separable_dose                  brute-force triple loop (tiny problems),
+                                single-spot analytic Gaussian, linearity,
+                                mirror symmetry, published peak dose 635.2707
+relative_error / gamma_index    hand-computed cases; gamma(ref, ref) == 0
+objective_grad_w                dot-product test and central differences
+==============================  ==============================================
+"""
+from __future__ import annotations
+
+import math
+from functools import lru_cache
+
+import numpy as np
+
+# ---------------------------------------------------------------------------
+# Physical constants (physics.py:42-65). These are CODATA/PDG values and the
+# reference's model parameters; they must be the same numbers to reproduce it.
+# ---------------------------------------------------------------------------
+MP_MEV = 938.27208816            # proton rest energy
+ME_MEV = 0.51099895000           # electron rest energy
+K_BB = 0.307075                  # 4 pi N_A r_e^2 m_e c^2 [MeV cm^2/mol]
+WATER_ZA = 10.0 / 18.01528       # Z/A of water
+WATER_I = 75.0 * 1e-6            # mean excitation energy [MeV]
+WATER_RHO = 1.0
+WATER_X0 = 36.08                 # radiation length [cm]
+E_HIGHLAND = 13.6
+E_LO, E_HI = 0.5, 400.0          # CSDA table span [MeV]
+P_R = 1.77                       # range-energy power used for the spread term
+BOHR_COEF = K_BB * WATER_ZA * WATER_RHO * ME_MEV   # physics.py:151
+INV_SQRT_2PI = 1.0 / np.sqrt(2.0 * np.pi)          # engines.py:43
+SIGMA_FLOOR = 1e-9                                 # engines.py:57, :266
+
+
+# ---------------------------------------------------------------------------
+# Stopping power and the CSDA range table
+# ---------------------------------------------------------------------------
+def kinematics(E):
+    """(beta, gamma, beta^2) of a proton of kinetic energy E. physics.py:72-77"""
+    E = np.asarray(E, dtype=np.float64)
+    g = 1.0 + E / MP_MEV
+    b2 = np.clip(1.0 - 1.0 / g**2, 1e-14, 1.0)
+    return np.sqrt(b2), g, b2
+
+
+def stopping_power(E):
+    """Bethe-Bloch linear stopping power of water [MeV/cm]. physics.py:80-105
+
+    No shell or density-effect term; I = 75 eV. The evaluation order of the
+    products follows the reference so the FP64 bits agree.
+    """
+    E = np.asarray(E, dtype=np.float64)
+    _, g, b2 = kinematics(E)
+    r = ME_MEV / MP_MEV
+    tmax = 2.0 * ME_MEV * b2 * g**2 / (1.0 + 2.0 * g * r + r**2)
+    arg = np.clip(2.0 * ME_MEV * b2 * g**2 * tmax / (WATER_I**2), 1.0 + 1e-12, None)
+    return K_BB * WATER_ZA * WATER_RHO / b2 * (0.5 * np.log(arg) - b2)
+
+
+def energy_grid():
+    """The 3797-point energy grid of the range table. physics.py:118-123
+
+    Four linspace segments whose shared end points (2, 20, 60 MeV) are
+    removed by ``np.unique``.
+    """
+    parts = [np.linspace(E_LO, 2.0, 800), np.linspace(2.0, 20.0, 900),
+             np.linspace(20.0, 60.0, 900), np.linspace(60.0, E_HI, 1200)]
+    return np.unique(np.concatenate(parts))
+
+
+@lru_cache(maxsize=1)
+def range_table():
+    """(E, R) with R(E) = cumulative trapezoid of 1/S from E_LO. physics.py:111-127"""
+    E = energy_grid()
+    inv = 1.0 / stopping_power(E)
+    seg = 0.5 * (inv[1:] + inv[:-1]) * np.diff(E)
+    R = np.concatenate([[0.0], np.cumsum(seg)])
+    E.setflags(write=False)
+    R.setflags(write=False)
+    return E, R
+
+
+def csda_range(E):
+    """CSDA range [cm] by linear interpolation in the table. physics.py:130-133"""
+    Eg, Rg = range_table()
+    return np.interp(np.asarray(E, dtype=np.float64), Eg, Rg)
+
+
+def energy_from_range(R):
+    """Inverse of csda_range; R is clipped to [0, R_max]. physics.py:136-139"""
+    Eg, Rg = range_table()
+    return np.interp(np.clip(np.asarray(R, dtype=np.float64), 0.0, Rg[-1]), Rg, Eg)
+
+
+def clinical_layer_energies(n_layers=40, r_min=3.0, r_max=25.5):
+    """Energies of equally range-spaced layers. model.py:248-255"""
+    return energy_from_range(np.linspace(r_min, r_max, n_layers))
+
+
+# ---------------------------------------------------------------------------
+# Range straggling and the IDD model
+# ---------------------------------------------------------------------------
+def straggling_bohr(E0, n=4000):
+    """Bohr range straggling sigma [cm]. physics.py:154-174
+
+    sigma^2 = int_{E_LO}^{E0} BOHR_COEF / (beta^2 S^3) dE, trapezoid on a
+    uniform n-point grid.
+    """
+    E = np.linspace(E_LO, float(E0), n)
+    if E[-1] <= E[0]:
+        return 0.0
+    S = stopping_power(E)
+    _, _, b2 = kinematics(E)
+    f = BOHR_COEF / (np.clip(b2, 1e-12, None) * S**3)
+    return float(np.sqrt(np.trapezoid(f, E)))
+
+
+def sigma_range(E0, energy_spread=0.0033, straggling="bohr",
+                icru_coef=0.012, icru_exp=0.935):
+    """Total range spread hypot(straggling, 1.77*spread*R0). physics.py:310-322"""
+    R0 = float(csda_range(E0))
+    if straggling == "bohr":
+        s = straggling_bohr(E0)
+    elif straggling == "icru49":
+        s = icru_coef * R0 ** icru_exp
+    elif straggling == "none":
+        s = 0.0
+    else:
+        raise ValueError(straggling)
+    e = P_R * energy_spread * R0
+    return float(np.hypot(s, e)), float(s), float(e)
+
+
+def idd_pristine(z, E0, energy_spread=0.0033, attenuation_length=150.0,
+                 n_fine=6000, straggling="bohr", normalize=False, full=False):
+    """Fluence-normalised pristine Bragg curve on z. physics.py:328-387
+
+    1. energy-conserving CSDA binning on a 6000-point grid, with 6001 bin
+       edges on [0, R0 + 8 sigma_R + 0.2];
+    2. Gaussian range-spread convolution through a zero-padded real FFT;
+    3. linear interpolation onto z with zero outside the fine grid.
+    """
+    z = np.asarray(z, dtype=np.float64)
+    R0 = float(csda_range(E0))
+    sR = sigma_range(E0, energy_spread, straggling)[0]
+    top = R0 + 8.0 * sR + 0.2
+    zf = np.linspace(0.0, top, n_fine)
+    h = zf[1] - zf[0]
+    edges = np.linspace(0.0, top, n_fine + 1)
+    Ee = np.where(edges < R0, energy_from_range(np.clip(R0 - edges, 0.0, None)), 0.0)
+    d_csda = np.exp(-zf / attenuation_length) * (Ee[:-1] - Ee[1:]) / h
+    nfft = int(2 ** np.ceil(np.log2(2 * n_fine)))
+    f = np.fft.rfftfreq(nfft, d=h)
+    H = np.exp(-2.0 * (np.pi * f * sR) ** 2)
+    d_fine = np.fft.irfft(np.fft.rfft(d_csda, nfft) * H, nfft)[:n_fine]
+    out = np.interp(z, zf, d_fine, left=0.0, right=0.0)
+    pk = float(out.max()) if out.max() > 0 else 1.0
+    if normalize:
+        out = out / pk
+    if full:
+        return dict(idd=out, sigma_R=sR, R0=R0, z_fine=zf, dose_fine=d_fine,
+                    dose_csda=d_csda, h=h, peak=pk)
+    return out
+
+
+def idd_direct_convolution(E0, energy_spread=0.0033, attenuation_length=150.0,
+                           n_fine=6000):
+    """Closed-form check of the FFT step: the zero-padded FFT with the Gaussian
+    multiplier exp(-2 (pi f sigma)^2) equals a direct sum against the sampled
+    Gaussian h/(sqrt(2 pi) sigma) exp(-(k h)^2 / (2 sigma^2)) (Poisson summation;
+    the 16384-point period is far beyond the kernel support)."""
+    o = idd_pristine([0.0], E0, energy_spread, attenuation_length, n_fine, full=True)
+    sR, h, x = o["sigma_R"], o["h"], o["dose_csda"]
+    k = np.arange(-(n_fine - 1), n_fine)
+    g = h * INV_SQRT_2PI / sR * np.exp(-(k * h) ** 2 / (2.0 * sR * sR))
+    return np.convolve(x, g)[n_fine - 1: 2 * n_fine - 1], o["dose_fine"]
+
+
+# ---------------------------------------------------------------------------
+# Fermi-Eyges moments
+# ---------------------------------------------------------------------------
+def scattering_power(E):
+    """Local Highland scattering power T [rad^2/cm] with L = 1. physics.py:185-218
+
+    With ``z_over_X0=None`` the reference uses ln L = 0, so T is the base
+    term times (1 + 0) * (1.076 + 0).
+    """
+    beta, g, _ = kinematics(E)
+    pc = np.sqrt(np.clip(g**2 - 1.0, 0.0, None)) * MP_MEV
+    bp = np.clip(beta * pc, 1e-9, None)
+    base = (E_HIGHLAND / bp) ** 2 / WATER_X0
+    lnL = np.log(1.0)
+    return base * (1.0 + 0.038 * lnL) * (1.076 + 0.038 * lnL)
+
+
+def fine_count(z, n_sub=4, max_step=0.02):
+    """Fine-grid size of the moment integrals. physics.py:256-258"""
+    z = np.asarray(z, dtype=np.float64)
+    span = float(z[-1] - z[0])
+    need = int(np.ceil(span / max(max_step, 1e-6))) + 1 if span > 0 else 2
+    return max((len(z) - 1) * n_sub + 1, need, 2)
+
+
+def fe_moments(z, E0, sigma_x0=0.0, sigma_theta0=0.0, rho0=0.0, n_sub=4,
+               max_step=0.02, T_fn=None):
+    """Fermi-Eyges moments and lateral sigma on z (len(z) >= 2). physics.py:221-286
+
+    B_i = sum_{j : zf_i > zf_j} (zf_i - zf_j)^2 T_j h_j is the explicit double
+    sum (not 2*A2). ``T_fn`` replaces the scattering power (closed-form tests).
+    """
+    z = np.asarray(z, dtype=np.float64)
+    if z.size < 2:
+        raise ValueError("fe_moments needs at least two depths")
+    nf = fine_count(z, n_sub, max_step)
+    zf = np.linspace(z[0], z[-1], nf)
+    hf = np.gradient(zf)
+    R0 = float(csda_range(E0))
+    Ez = np.where(zf < R0, energy_from_range(np.clip(R0 - zf, 0.0, None)), 0.0)
+    if T_fn is None:
+        Tf = np.where(zf < R0, scattering_power(np.clip(Ez, 1e-3, None)), 0.0)
+    else:
+        Tf = np.asarray(T_fn(zf), dtype=np.float64) * np.ones_like(zf)
+    A0f = np.cumsum(Tf * hf)
+    A1f = np.cumsum(A0f * hf)
+    A2f = np.cumsum(A1f * hf)
+    dz = zf[:, None] - zf[None, :]
+    Bf = np.where(dz > 0, dz**2 * Tf[None, :] * hf[None, :], 0.0).sum(axis=1)
+    B = np.interp(z, zf, Bf)
+    var = (sigma_x0**2 + 2.0 * rho0 * sigma_x0 * sigma_theta0 * z
+           + sigma_theta0**2 * z**2 + B)
+    return dict(A0=np.interp(z, zf, A0f), A1=np.interp(z, zf, A1f),
+                A2=np.interp(z, zf, A2f), B=B, n_fine=nf,
+                sigma_x=np.sqrt(np.clip(var, 0.0, None)))
+
+
+def layer_luts(z, energies, sigma_x0=0.30, sigma_theta0=0.0, rho0=0.0,
+               energy_spread=0.0033, attenuation_length=150.0):
+    """(idd, sigma) tables of shape (L, nz), one row per layer, as the field
+    builders make them (model.py:225-233, :312-320)."""
+    z = np.asarray(z, dtype=np.float64)
+    L = len(energies)
+    idd = np.zeros((L, z.size))
+    sig = np.zeros((L, z.size))
+    for l, E in enumerate(np.asarray(energies, dtype=np.float64)):
+        idd[l] = idd_pristine(z, E, energy_spread, attenuation_length)
+        sig[l] = fe_moments(z, E, sigma_x0, sigma_theta0, rho0)["sigma_x"]
+    return idd, sig
+
+
+def shifted_energies(energies, range_shift):
+    """Energies whose CSDA range is R0 + shift (cm), layer by layer."""
+    R0 = csda_range(np.asarray(energies, dtype=np.float64))
+    return energy_from_range(R0 + np.asarray(range_shift, dtype=np.float64))
+
+
+# ---------------------------------------------------------------------------
+# Problem model (model.py:27-131)
+# ---------------------------------------------------------------------------
+def axis(n, d, o):
+    """Voxel centres o + k d. model.py:48-52"""
+    return o + np.arange(n) * d
+
+
+def centered_grid(n, d):
+    """DoseGrid.centered(n, d): x/y centred on 0, z from 0. model.py:73-79"""
+    h = (n - 1) * d / 2.0
+    return dict(nx=n, ny=n, nz=n, dx=d, dy=d, dz=d, ox=-h, oy=-h, oz=0.0)
+
+
+def spot_axis(n, ds, x0=0.0):
+    """Centred spot positions x0 + (i - (n-1)/2) ds. model.py:107-113"""
+    return x0 + (np.arange(n) - (n - 1) / 2.0) * ds
+
+
+def grid_axes(grid):
+    return (axis(grid["nx"], grid["dx"], grid["ox"]),
+            axis(grid["ny"], grid["dy"], grid["oy"]),
+            axis(grid["nz"], grid["dz"], grid["oz"]))
+
+
+# ---------------------------------------------------------------------------
+# Dose engines
+# ---------------------------------------------------------------------------
+def lateral(vox, spots, sigma):
+    """G[..., k, i] = exp(-(v_k - s_i)^2 / (2 s^2)) / (sqrt(2 pi) s), s >= 1e-9.
+    engines.py:49-61"""
+    d2 = (np.asarray(vox)[:, None] - np.asarray(spots)[None, :]) ** 2
+    s = np.clip(np.asarray(sigma, dtype=np.float64), SIGMA_FLOOR, None)
+    return np.exp(-d2 / (2.0 * s[..., None, None] ** 2)) * (INV_SQRT_2PI / s)[..., None, None]
+
+
+def separable_dose(x, y, sx, sy, w, idd, sigma, z_idx=None):
+    """FP64 per-layer-exact separable dose (nz, ny, nx). engines.py:172-202
+
+    D[z] = sum_l IDD_l(z) Gy_l(z) W_l Gx_l(z)^T, evaluated slice-block by
+    slice-block to bound memory; the contraction is the same two einsums.
+    With ``z_idx`` only those slices are computed (returned in that order).
+    """
+    w = np.asarray(w, dtype=np.float64)
+    idd = np.asarray(idd, dtype=np.float64)
+    sigma = np.asarray(sigma, dtype=np.float64)
+    L, nz = idd.shape
+    zs = np.arange(nz) if z_idx is None else np.asarray(z_idx, dtype=np.int64)
+    D = np.zeros((len(zs), len(y), len(x)))
+    zb = max(1, int(4_000_000 // max(1, L * len(x) * len(sx))))
+    for b0 in range(0, len(zs), zb):
+        sel = zs[b0:b0 + zb]
+        Gx = lateral(x, sx, sigma[:, sel])                 # (L, b, nx, nsx)
+        Gy = lateral(y, sy, sigma[:, sel])                 # (L, b, ny, nsy)
+        A = idd[:, sel, None, None] * w[:, None, :, :]     # (L, b, nsy, nsx)
+        T = np.einsum("lbji,lbvj->lbiv", A, Gy)            # (L, b, nsx, ny)
+        Dl = np.einsum("lbai,lbiy->lbay", Gx, T)           # (L, b, nx, ny)
+        D[b0:b0 + len(sel)] = Dl.sum(axis=0).transpose(0, 2, 1)
+    return D
+
+
+def brute_force_dose(x, y, z_count, sx, sy, w, idd, sigma):
+    """Textbook voxel x spot loop (tiny problems only). engines.py:102-136"""
+    w = np.asarray(w, dtype=np.float64)
+    L, nsy, nsx = w.shape
+    D = np.zeros((z_count, len(y), len(x)))
+    for kz in range(z_count):
+        for l in range(L):
+            s = max(float(sigma[l, kz]), SIGMA_FLOOR)
+            c = float(idd[l, kz]) / (2.0 * math.pi * s * s)
+            for j in range(nsy):
+                gy = np.exp(-(y - sy[j]) ** 2 / (2.0 * s * s))
+                for i in range(nsx):
+                    if w[l, j, i] == 0.0:
+                        continue
+                    gx = np.exp(-(x - sx[i]) ** 2 / (2.0 * s * s))
+                    D[kz] += w[l, j, i] * c * np.outer(gy, gx)
+    return D
+
+
+def relative_error(a, b, mask=None):
+    """Global-normalised error metrics. engines.py:364-378"""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if mask is None:
+        mask = np.ones(b.shape, dtype=bool)
+    d = (a - b)[mask]
+    ref = float(np.max(np.abs(b))) or 1.0
+    if d.size == 0:
+        return dict(max_abs_err=0.0, max_rel_err_global=0.0, rms_err=0.0,
+                    rms_rel_global=0.0, n_vox=int(mask.sum()))
+    mx = float(np.abs(d).max())
+    rms = float(np.sqrt((d**2).mean()))
+    return dict(max_abs_err=mx, max_rel_err_global=mx / ref, rms_err=rms,
+                rms_rel_global=rms / ref, n_vox=int(mask.sum()))
+
+
+def mean_rel_error(a, b, thr=0.10):
+    """Mean and max pointwise relative error on voxels >= thr * max|b|."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    m = np.abs(b) >= thr * np.abs(b).max()
+    r = np.abs(a[m] - b[m]) / np.abs(b[m])
+    return float(r.mean()), float(r.max())
+
+
+# ---------------------------------------------------------------------------
+# Gamma index (gamma.py:41-160)
+# ---------------------------------------------------------------------------
+def dta_offsets(spacing, dta_cm, max_vox=6):
+    """Integer voxel offsets within the DTA sphere and their lengths. gamma.py:41-60"""
+    dz, dy, dx = spacing
+    n = [min(int(np.floor(dta_cm / s + 1e-9)), max_vox) for s in (dz, dy, dx)]
+    offs, dist = [], []
+    for a in range(-n[0], n[0] + 1):
+        for b in range(-n[1], n[1] + 1):
+            for c in range(-n[2], n[2] + 1):
+                r = math.sqrt((a * dz) ** 2 + (b * dy) ** 2 + (c * dx) ** 2)
+                if r <= dta_cm + 1e-12:
+                    offs.append((a, b, c))
+                    dist.append(r)
+    return np.array(offs, dtype=np.int64), np.array(dist)
+
+
+def _shifted(a, off):
+    """a shifted by off with zero fill: out[p] = a[p + off]. gamma.py:63-75"""
+    out = np.zeros_like(a)
+    src, dst = [], []
+    for o, n in zip(off, a.shape):
+        src.append(slice(max(0, o), n + min(0, o)))
+        dst.append(slice(max(0, -o), n + min(0, -o)))
+    out[tuple(dst)] = a[tuple(src)]
+    return out
+
+
+def gamma_index(ref, ev, spacing=(0.2, 0.2, 0.2), dose_pct=3.0, dta_mm=3.0,
+                threshold_pct=10.0):
+    """Global gamma index over integer-voxel DTA offsets. gamma.py:81-160"""
+    ref = np.asarray(ref, dtype=np.float64)
+    ev = np.asarray(ev, dtype=np.float64)
+    dta = dta_mm / 10.0
+    rmax = float(ref.max())
+    crit = dose_pct / 100.0 * rmax
+    mask = ref >= threshold_pct / 100.0 * rmax
+    if mask.sum() == 0:
+        mask = ref > 0
+    offs, dist = dta_offsets(spacing, dta)
+    g2 = np.full(ref.shape, np.inf)
+    for off, r in zip(offs, dist):
+        dd = (_shifted(ev, tuple(off)) - ref) / crit
+        np.minimum(g2, (r / dta) ** 2 + dd * dd, out=g2)
+    g = np.sqrt(g2)[mask]
+    return dict(pass_rate=float((g <= 1.0).mean() * 100.0), gamma_max=float(g.max()),
+                gamma_mean=float(g.mean()), n_eval=int(g.size),
+                n_pass=int((g <= 1.0).sum()), n_dta_offsets=len(offs))
+
+
+# ---------------------------------------------------------------------------
+# Objective and its adjoint (weighted quadratic; FP64)
+# ---------------------------------------------------------------------------
+def presc_cube(grid, half=3.0, dose=2.0, w_in=1.0, w_out=0.1):
+    """D_presc = dose inside the centred cube |x|,|y| <= half, z within
+    +-half of the middle of the z range; voxel weights w_in/w_out."""
+    x, y, z = grid_axes(grid)
+    zc = 0.5 * (z[0] + z[-1])
+    inside = ((np.abs(z - zc) <= half + 1e-9)[:, None, None]
+              & (np.abs(y) <= half + 1e-9)[None, :, None]
+              & (np.abs(x) <= half + 1e-9)[None, None, :])
+    Dp = np.where(inside, dose, 0.0)
+    wv = np.where(inside, w_in, w_out)
+    return Dp, wv
+
+
+def objective(D, Dp, wv):
+    """J = sum_v wv (D_v - Dp_v)^2."""
+    r = np.asarray(D, dtype=np.float64) - Dp
+    return float(np.sum(wv * r * r))
+
+
+def adjoint_w(x, y, sx, sy, idd, sigma, R):
+    """dJ/dW[l,j,i] = sum_z IDD_l(z) (Gy_l(z)^T R_z Gx_l(z))[j,i] for a
+    residual R = dJ/dD (nz, ny, nx). The transpose of separable_dose."""
+    idd = np.asarray(idd, dtype=np.float64)
+    L, nz = idd.shape
+    out = np.zeros((L, len(sy), len(sx)))
+    for kz in range(nz):
+        Gx = lateral(x, sx, sigma[:, kz])   # (L, nx, nsx)
+        Gy = lateral(y, sy, sigma[:, kz])   # (L, ny, nsy)
+        Q = np.einsum("yx,lxi->lyi", R[kz], Gx)
+        out += idd[:, kz, None, None] * np.einsum("lyj,lyi->lji", Gy, Q)
+    return out

@@ -1,0 +1,84 @@
+"""The C-ABI library: it loads, exports every declared symbol, and its pure
+host entry points (error strings, validation, workspace sizing) behave.
+No kernel is launched here."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+from pbdose import _lib
+
+HEADER = os.path.join(ROOT, "include", "pbdose_hip.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pbh_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.lib()
+    names = declared_symbols()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(lib, n), f"{n} declared in pbdose_hip.h but not exported"
+    # the ctypes table covers exactly the header
+    assert set(names) == set(_lib.SIGNATURES)
+
+
+def test_error_strings():
+    assert _lib.strerror(0) == "ok"
+    assert "null" in _lib.strerror(-1)
+    assert "workspace" in _lib.strerror(-7)
+    assert _lib.strerror(-99) == "unknown error"
+    assert b"gfx950" in _lib.lib().pbh_version()
+
+
+def test_validate_rejects_bad_problems():
+    L = _lib.lib()
+    g = _lib.PbhGrid(128, 128, 128, 0, 0.2, 0.2, 0.2, -12.7, -12.7, 0.0)
+    lat = _lib.PbhLattice(41, 41, 40, 0, 0.5, 0.5, 0.0, 0.0)
+    assert L.pbh_validate(g, lat) == 0
+    assert L.pbh_validate(None, lat) == -1
+    bad = _lib.PbhGrid(0, 128, 128, 0, 0.2, 0.2, 0.2, 0, 0, 0)
+    assert L.pbh_validate(bad, lat) == -2
+    bad = _lib.PbhGrid(128, 128, 128, 0, -0.2, 0.2, 0.2, 0, 0, 0)
+    assert L.pbh_validate(bad, lat) == -2
+    badl = _lib.PbhLattice(41, 41, 40, 0, 0.0, 0.5, 0.0, 0.0)
+    assert L.pbh_validate(g, badl) == -2
+    huge = _lib.PbhGrid(100000, 128, 128, 0, 0.2, 0.2, 0.2, 0, 0, 0)
+    assert L.pbh_validate(huge, lat) == -5
+
+
+def test_forward_rejects_null_and_bad_precision_without_launching():
+    L = _lib.lib()
+    g = _lib.PbhGrid(8, 8, 8, 0, 0.2, 0.2, 0.2, 0, 0, 0)
+    lat = _lib.PbhLattice(3, 3, 2, 0, 0.5, 0.5, 0.0, 0.0)
+    nul = C.c_void_p(0)
+    assert L.pbh_forward(g, lat, nul, nul, nul, nul, 0, nul, 0, nul) == -1
+    one = C.c_void_p(16)   # never dereferenced: the precision check fails first
+    assert L.pbh_forward(g, lat, one, one, one, one, 99, nul, 0, nul) == -5
+
+
+def test_lut_workspace_query():
+    from pbdose import physics
+    m = physics.beam_model()
+    n = _lib.lib().pbh_lut_workspace_bytes(m, 128, 40)
+    assert n > 40 * 6000 * 16   # binning + Gaussian kernel per layer
+    bad = physics.beam_model()
+    bad.n_fine_idd = 1
+    assert _lib.lib().pbh_lut_workspace_bytes(bad, 128, 40) == 0
+    assert _lib.lib().pbh_build_lut(m, 128, 0.0, 0.2, 40, None, None, None, None, None, 0, None) == -1
+
+
+def test_product_refuses_cpu_device():
+    import numpy as np
+    from pbdose import engines, model
+    g = model.DoseGrid(4, 4, 4, 0.2, 0.2, 0.2)
+    lat = model.SpotLattice(2, 2, 1, 0.5, 0.5, weights=np.ones((1, 2, 2)))
+    prob = model.PBProblem(g, lat, np.ones((1, 4)), np.ones((1, 4)))
+    with pytest.raises(ValueError, match="GPU only"):
+        engines.torch_separable(prob, "cpu")

@@ -1,0 +1,110 @@
+"""Forward dose on the GPU (through the C ABI) vs the FP64 oracle.
+
+Bar: max global relative error <= 1e-6 for the FP32 form
+(max |D - D_ref| / max |D_ref|, engines.py:364-378 convention).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import pb_oracle as O
+
+pytestmark = pytest.mark.gpu
+TOL_FP32 = 1e-6
+
+
+def make_problem(nx, ny, nz, d, nsx, nsy, ds, energies, w=None, x0=0.0, y0=0.0, oz=0.0,
+                 seed=0, sigma_x0=0.3):
+    from pbdose import model
+    g = model.DoseGrid(nx, ny, nz, d, d, d, -(nx - 1) * d / 2, -(ny - 1) * d / 2, oz)
+    energies = np.asarray(energies, dtype=np.float64)
+    if w is None:
+        w = np.random.default_rng(seed).lognormal(0.0, 0.5, (len(energies), nsy, nsx))
+    idd, sig = O.layer_luts(g.z, energies, sigma_x0=sigma_x0)
+    lat = model.SpotLattice(nsx, nsy, len(energies), ds, ds, x0, y0, energies=energies, weights=w)
+    return model.PBProblem(g, lat, idd, sig)
+
+
+def oracle_dose(p, z_idx=None):
+    return O.separable_dose(p.grid.x, p.grid.y, p.lattice.spot_x, p.lattice.spot_y,
+                            p.lattice.weights, p.idd, p.sigma, z_idx=z_idx)
+
+
+def check(p, tol=TOL_FP32, z_idx=None, precision="fp32"):
+    from pbdose import engines
+    D = engines.torch_separable(p, "cuda", precision=precision).double().cpu().numpy()
+    ref = oracle_dose(p, z_idx)
+    got = D if z_idx is None else D[z_idx]
+    r = O.relative_error(got, ref)
+    print(f"  max_rel_err_global={r['max_rel_err_global']:.3e} rms={r['rms_rel_global']:.3e}",
+          flush=True)
+    assert r["max_rel_err_global"] <= tol
+    return D, ref
+
+
+def test_engine_check_problem():
+    # engine_check.py:15-23 geometry: 24x24x32 @ 2.5 mm, 3 layers of 3x3 spots, seed 7
+    print("case engine_check 24x24x32", flush=True)
+    from pbdose import model
+    g = model.DoseGrid(24, 24, 32, 0.25, 0.25, 0.25, -3.0, -3.0, 0.0)
+    E = np.array([80.0, 130.0, 180.0])
+    w = np.random.default_rng(7).random((3, 3, 3))
+    idd, sig = O.layer_luts(g.z, E)
+    p = model.PBProblem(g, model.SpotLattice(3, 3, 3, 0.5, 0.5, energies=E, weights=w), idd, sig)
+    D, ref = check(p)
+    B = O.brute_force_dose(g.x, g.y, 4, p.lattice.spot_x, p.lattice.spot_y, w, idd, sig)
+    assert np.abs(D[:4] - B).max() <= TOL_FP32 * np.abs(B).max()
+
+
+def test_config_small_64cube():
+    # 64^3 @ 2 mm, 21x21 @ 5 mm, 10 layers 100-150 MeV, w = 1
+    print("case 64^3 / 21x21 / 10 layers", flush=True)
+    p = make_problem(64, 64, 64, 0.2, 21, 21, 0.5, np.linspace(100, 150, 10),
+                     w=np.ones((10, 21, 21)))
+    check(p)
+
+
+def test_config_clinical_128cube_sampled_slices():
+    print("case 128^3 / 41x41 / 40 layers (sampled slices)", flush=True)
+    p = make_problem(128, 128, 128, 0.2, 41, 41, 0.5, np.linspace(70, 180, 40))
+    check(p, z_idx=np.arange(0, 128, 9))
+
+
+@pytest.mark.parametrize("shape", [(37, 70, 5), (130, 65, 3), (1, 1, 2), (200, 33, 4)])
+def test_ragged_grids(shape):
+    nx, ny, nz = shape
+    print(f"case ragged grid {shape}", flush=True)
+    p = make_problem(nx, ny, nz, 0.2, 7, 5, 0.5, [90.0, 120.0], seed=3)
+    check(p)
+
+
+@pytest.mark.parametrize("spots", [(1, 1), (60, 50), (97, 13), (49, 49)])
+def test_lattice_chunking(spots):
+    nsx, nsy = spots
+    print(f"case lattice {spots}", flush=True)
+    p = make_problem(64, 48, 6, 0.25, nsx, nsy, 0.3, [100.0, 160.0], seed=5, x0=0.37, y0=-0.21)
+    check(p)
+
+
+def test_zero_idd_layers_and_zero_weights():
+    print("case zero IDD beyond range, zero weights", flush=True)
+    w = np.random.default_rng(2).random((3, 9, 9))
+    w[1] = 0.0
+    w[2, 3:6, :] = 0.0
+    p = make_problem(48, 48, 70, 0.2, 9, 9, 0.5, [60.0, 70.0, 120.0], w=w)
+    assert (p.idd[0] == 0).any()
+    check(p)
+
+
+def test_linearity_and_determinism():
+    from pbdose import engines
+    p = make_problem(64, 64, 16, 0.2, 21, 21, 0.5, np.linspace(100, 150, 6), seed=11)
+    dp = engines.DeviceProblem(p)
+    a = dp.forward()
+    b = dp.forward()
+    assert torch.equal(a, b)
+    w2 = torch.rand_like(dp.w)
+    d2 = dp.forward(w=w2)
+    d12 = dp.forward(w=(dp.w + 3.0 * w2).contiguous())
+    ref = (a.double() + 3.0 * d2.double())
+    assert (d12.double() - ref).abs().max() <= 1e-6 * ref.abs().max()

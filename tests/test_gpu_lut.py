@@ -1,0 +1,57 @@
+"""On-device FP64 look-up tables vs the reference's own values (fixtures) and
+the oracle. Tolerance: 1e-12 relative (transcendentals of the device library
+may differ from the host's by an ulp; sums are reordered)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import pb_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def test_range_table_bits(ref_physics):
+    from pbdose import physics
+    E, R = physics.range_table("cuda")
+    np.testing.assert_array_equal(E.cpu().numpy(), ref_physics["table_E"])
+    np.testing.assert_allclose(R.cpu().numpy(), ref_physics["table_R"], rtol=1e-13, atol=0)
+
+
+def test_range_interpolation(ref_physics):
+    from pbdose import physics
+    np.testing.assert_allclose(physics.csda_range(ref_physics["csda_E"]), ref_physics["csda_R"],
+                               rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(physics.energy_from_range(ref_physics["efr_R"]),
+                               ref_physics["efr_E"], rtol=1e-13)
+    e = physics.energy_from_range(np.linspace(3.0, 25.5, 40))
+    np.testing.assert_allclose(e, ref_physics["clin40"], rtol=1e-13)
+
+
+@pytest.mark.parametrize("case", ["small", "c1", "c2"])
+def test_layer_luts(ref_physics, case):
+    from pbdose import physics
+    z, E = ref_physics[f"{case}_z"], ref_physics[f"{case}_E"]
+    print(f"LUT case {case}: {len(E)} layers x {len(z)} depths", flush=True)
+    idd, sig, info = physics.build_luts(len(z), float(z[0]), float(z[1] - z[0]), E)
+    ri, rs = ref_physics[f"{case}_idd"], ref_physics[f"{case}_sigma"]
+    idd, sig, info = idd.cpu().numpy(), sig.cpu().numpy(), info.cpu().numpy()
+    assert np.abs(idd - ri).max() <= 1e-12 * np.abs(ri).max()
+    assert np.abs(sig - rs).max() <= 1e-12 * np.abs(rs).max()
+    np.testing.assert_allclose(info[:, 1], ref_physics[f"{case}_sigR"], rtol=1e-13)
+    np.testing.assert_allclose(info[:, 3], ri.max(axis=1), rtol=1e-12)
+
+
+def test_lut_with_divergence_and_icru_model():
+    from pbdose import physics
+    z = np.arange(50) * 0.3 - 0.6          # starts below 0: IDD left of the grid is 0
+    E = np.array([72.5, 133.0, 201.0])
+    model = physics.IDDModel(straggling_model="icru49", energy_spread=0.005,
+                             attenuation_length=120.0)
+    idd, sig, _ = physics.build_luts(len(z), float(z[0]), 0.3, E, idd_model=model,
+                                     sigma_x0=0.25, sigma_theta0=0.003, rho0=0.2)
+    for l, e in enumerate(E):
+        ri = O.idd_pristine(z, e, energy_spread=0.005, attenuation_length=120.0,
+                            straggling="icru49")
+        rs = O.fe_moments(z, e, 0.25, 0.003, 0.2)["sigma_x"]
+        assert np.abs(idd[l].cpu().numpy() - ri).max() <= 1e-12 * ri.max()
+        assert np.abs(sig[l].cpu().numpy() - rs).max() <= 1e-12 * rs.max()

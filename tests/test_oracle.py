@@ -1,0 +1,215 @@
+"""The FP64 oracle, pinned before it is trusted (CPU only).
+
+Pins: reference fixtures (tests/golden/ref_physics.npz, written by
+make_golden.py from the reference's own functions), published digits
+(03_validation_report.pdf, 02_performance_report.pdf, benchmark_full.json,
+physics.py docstring), closed forms and brute force on tiny problems.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import pb_oracle as O
+
+
+# ---------------------------------------------------------------- fixtures
+def test_range_table_matches_reference_bits(ref_physics):
+    E, R = O.range_table()
+    assert E.shape == (3797,)
+    np.testing.assert_array_equal(E, ref_physics["table_E"])
+    np.testing.assert_array_equal(R, ref_physics["table_R"])
+
+
+def test_stopping_power_and_interp_match_reference(ref_physics):
+    np.testing.assert_array_equal(O.stopping_power(ref_physics["sp_E"]), ref_physics["sp_S"])
+    np.testing.assert_array_equal(O.csda_range(ref_physics["csda_E"]), ref_physics["csda_R"])
+    np.testing.assert_array_equal(O.energy_from_range(ref_physics["efr_R"]), ref_physics["efr_E"])
+
+
+@pytest.mark.parametrize("case", ["c1", "c2", "small"])
+def test_layer_luts_match_reference(ref_physics, case):
+    z, E = ref_physics[f"{case}_z"], ref_physics[f"{case}_E"]
+    idd, sig = O.layer_luts(z, E)
+    np.testing.assert_array_equal(idd, ref_physics[f"{case}_idd"])
+    np.testing.assert_array_equal(sig, ref_physics[f"{case}_sigma"])
+    sR = np.array([O.sigma_range(e)[0] for e in E])
+    np.testing.assert_array_equal(sR, ref_physics[f"{case}_sigR"])
+    np.testing.assert_array_equal([O.straggling_bohr(e) for e in E], ref_physics[f"{case}_strag"])
+
+
+def test_moments_with_divergence_match_reference(ref_physics):
+    m = O.fe_moments(ref_physics["mom_z"], 200.0, 0.3, 0.004, 0.5)
+    for k in ("A0", "A1", "A2", "B", "sigma_x"):
+        np.testing.assert_array_equal(m[k], ref_physics[f"mom_{k}"])
+
+
+def test_clinical_energies_match_reference(ref_physics):
+    np.testing.assert_array_equal(O.clinical_layer_energies(40, 3.0, 25.5), ref_physics["clin40"])
+
+
+# ---------------------------------------------------------- published digits
+def test_published_layer_energy_endpoints():
+    # benchmark_full.json:176-179 (16 digits)
+    e = O.clinical_layer_energies(40, 3.0, 25.5)
+    assert e[0] == 59.05787709884295
+    assert e[-1] == 197.96848375290924
+
+
+def test_published_ranges_and_stopping_powers():
+    # physics.py:24-25 and 03_validation_report.pdf section 2
+    np.testing.assert_allclose(O.csda_range([200.0, 250.0, 300.0]), [25.951, 37.931, 51.444],
+                               atol=6e-4)
+    np.testing.assert_allclose(O.stopping_power([1.0, 10.0, 100.0, 200.0]),
+                               [269.654, 45.947, 7.290, 4.492], atol=6e-4)
+
+
+def test_published_bohr_straggling():
+    # 03_validation_report.pdf section 6.1
+    got = [O.straggling_bohr(e) for e in (100.0, 150.0, 200.0, 230.0)]
+    np.testing.assert_allclose(got, [0.2249, 0.3655, 0.5110, 0.5992], atol=6e-5)
+    assert abs(O.BOHR_COEF - 0.08711) < 1e-5     # physics.py:151 quotes 4 digits
+
+
+def test_published_idd_peaks():
+    # 02_performance_report.pdf section 2: IDD peak 43.8 / 19.4 MeV/cm at 59 / 198 MeV
+    z = np.arange(0.0, 34.0, 0.02)
+    p1 = O.idd_pristine(z, 59.05787709884295).max()
+    p2 = O.idd_pristine(z, 197.96848375290924).max()
+    assert abs(p1 - 43.8) < 0.05 and abs(p2 - 19.4) < 0.06
+
+
+def test_published_pure_scatter_sigma():
+    # 03_validation_report.pdf section 7.2: sigma_x(10 cm, 200 MeV) = 0.124656 cm;
+    # the value depends on the depth grid at the 1e-3 level (grid_check.py:10-16).
+    s = O.fe_moments(np.linspace(0, 25, 2001), 200.0)["sigma_x"][800]
+    assert abs(s - 0.124656) < 2.5e-4
+
+
+# --------------------------------------------------------------- closed forms
+def test_range_energy_round_trip():
+    R = np.linspace(0.05, 30.0, 500)
+    np.testing.assert_allclose(O.csda_range(O.energy_from_range(R)), R, rtol=1e-12)
+
+
+def test_stopping_power_closed_form_at_100_mev():
+    # independent evaluation of Bethe-Bloch (no shell / density terms, I = 75 eV)
+    E = 100.0
+    g = 1 + E / 938.27208816
+    b2 = 1 - 1 / g**2
+    r = 0.51099895 / 938.27208816
+    tmax = 2 * 0.51099895 * b2 * g * g / (1 + 2 * g * r + r * r)
+    S = 0.307075 * (10 / 18.01528) / b2 * (0.5 * math.log(2 * 0.51099895 * b2 * g * g * tmax / 75e-6**2) - b2)
+    assert abs(O.stopping_power(E) / S - 1) < 1e-14
+
+
+def test_constant_scattering_power_moments():
+    # T = const: A0 = T z, A1 = T z^2 / 2, A2 = T z^3 / 6, B = T z^3 / 3 up to O(h)
+    z = np.linspace(0.0, 20.0, 41)
+    T = 2.5e-4
+    m = O.fe_moments(z, 200.0, T_fn=lambda zf: T)
+    zz = z[-1]
+    assert abs(m["A0"][-1] / (T * zz) - 1) < 2e-3
+    assert abs(m["A1"][-1] / (T * zz**2 / 2) - 1) < 6e-3
+    assert abs(m["A2"][-1] / (T * zz**3 / 6) - 1) < 9e-3
+    assert abs(m["B"][-1] / (T * zz**3 / 3) - 1) < 3e-3
+    # a dropped or doubled term would be off by O(1), and sigma_x0 enters in quadrature
+    m2 = O.fe_moments(z, 200.0, sigma_x0=0.3, T_fn=lambda zf: T)
+    np.testing.assert_allclose(m2["sigma_x"] ** 2, 0.09 + m["B"], rtol=1e-13)
+
+
+def test_fft_convolution_equals_direct_gaussian_sum():
+    for E in (70.0, 150.0, 230.0):
+        direct, fft = O.idd_direct_convolution(E)
+        assert np.abs(direct - fft).max() / np.abs(fft).max() < 1e-13
+
+
+def test_csda_binning_conserves_energy():
+    o = O.idd_pristine([0.0], 150.0, attenuation_length=1e30, full=True)
+    deposited = o["dose_csda"].sum() * o["h"]
+    E0 = O.energy_from_range(O.csda_range(150.0))
+    assert abs(deposited - E0) < 1e-9 * E0
+
+
+# ------------------------------------------------------------------ engines
+def _tiny(seed=7, L=3, nsx=3, nsy=2, nx=9, ny=7, nz=5):
+    rng = np.random.default_rng(seed)
+    x = O.axis(nx, 0.3, -1.2)
+    y = O.axis(ny, 0.35, -1.0)
+    sx = O.spot_axis(nsx, 0.5, 0.1)
+    sy = O.spot_axis(nsy, 0.6, -0.2)
+    w = rng.random((L, nsy, nsx))
+    idd = rng.random((L, nz)) * 10
+    sig = 0.2 + rng.random((L, nz)) * 0.5
+    return x, y, sx, sy, w, idd, sig
+
+
+def test_separable_equals_brute_force():
+    x, y, sx, sy, w, idd, sig = _tiny()
+    D = O.separable_dose(x, y, sx, sy, w, idd, sig)
+    B = O.brute_force_dose(x, y, idd.shape[1], sx, sy, w, idd, sig)
+    assert np.abs(D - B).max() / np.abs(B).max() < 1e-14
+
+
+def test_single_spot_is_analytic_gaussian():
+    x = O.axis(41, 0.1, -2.0)
+    y = O.axis(31, 0.1, -1.5)
+    idd = np.array([[3.0, 5.0]])
+    sig = np.array([[0.3, 0.45]])
+    D = O.separable_dose(x, y, [0.2], [-0.1], np.ones((1, 1, 1)), idd, sig)
+    for k in range(2):
+        s = sig[0, k]
+        ref = idd[0, k] / (2 * np.pi * s * s) * np.exp(
+            -((x[None, :] - 0.2) ** 2 + (y[:, None] + 0.1) ** 2) / (2 * s * s))
+        np.testing.assert_allclose(D[k], ref, rtol=1e-13, atol=0)
+
+
+def test_separable_linearity_and_mirror_symmetry():
+    x, y, sx, sy, w, idd, sig = _tiny(nsx=4, nsy=4)
+    w2 = np.random.default_rng(1).random(w.shape)
+    D1 = O.separable_dose(x, y, sx, sy, w, idd, sig)
+    D2 = O.separable_dose(x, y, sx, sy, w2, idd, sig)
+    D12 = O.separable_dose(x, y, sx, sy, 2.0 * w - 0.5 * w2, idd, sig)
+    np.testing.assert_allclose(D12, 2.0 * D1 - 0.5 * D2, rtol=1e-12, atol=1e-12 * np.abs(D1).max())
+    # symmetric grid and lattice: mirroring the weights in x mirrors the dose
+    xs = O.axis(10, 0.3, -1.35)
+    sxs = O.spot_axis(4, 0.5)
+    Da = O.separable_dose(xs, xs, sxs, sxs, w, idd, sig)
+    Db = O.separable_dose(xs, xs, sxs, sxs, w[:, :, ::-1], idd, sig)
+    np.testing.assert_allclose(Db, Da[:, :, ::-1], rtol=1e-12, atol=1e-13 * np.abs(Da).max())
+
+
+def test_sigma_floor_applies():
+    x = O.axis(5, 0.1, -0.2)
+    D = O.separable_dose(x, x, [0.0], [0.0], np.ones((1, 1, 1)), np.ones((1, 1)),
+                         np.zeros((1, 1)))
+    assert np.isinf(D).any() or D.max() > 1e15     # 1/(2 pi 1e-18) at the spot voxel
+
+
+def test_relative_error_hand_case():
+    a = np.array([1.0, 2.0, 3.5])
+    b = np.array([1.0, 2.5, 4.0])
+    r = O.relative_error(a, b)
+    assert r["max_abs_err"] == 0.5 and r["max_rel_err_global"] == 0.125
+    assert abs(r["rms_err"] - math.sqrt(0.5 / 3)) < 1e-15
+
+
+def test_gamma_identity_and_shift():
+    rng = np.random.default_rng(3)
+    ref = rng.random((6, 7, 8)) + 0.5
+    g = O.gamma_index(ref, ref, dose_pct=2.0, dta_mm=2.0)
+    assert g["pass_rate"] == 100.0 and g["gamma_max"] == 0.0
+    # 2 mm DTA at 2 mm voxels: 7 offsets (gamma.py docstring)
+    assert g["n_dta_offsets"] == 7
+    # a 3 % global dose error fails 2 %/2 mm wherever no neighbour rescues it
+    ev = ref * 1.0 + 0.03 * ref.max()
+    g2 = O.gamma_index(ref, ev, dose_pct=2.0, dta_mm=2.0)
+    assert g2["pass_rate"] < 100.0
+
+
+def test_adjoint_dot_product():
+    x, y, sx, sy, w, idd, sig = _tiny(nsx=4, nsy=3)
+    R = np.random.default_rng(5).standard_normal((idd.shape[1], len(y), len(x)))
+    D = O.separable_dose(x, y, sx, sy, w, idd, sig)
+    G = O.adjoint_w(x, y, sx, sy, idd, sig, R)
+    assert abs(np.sum(R * D) - np.sum(G * w)) < 1e-12 * abs(np.sum(R * D))
