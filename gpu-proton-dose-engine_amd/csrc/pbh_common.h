@@ -68,4 +68,16 @@ __device__ __forceinline__ float gauss2(float d2, float k2)
     return __builtin_amdgcn_exp2f(-d2 * k2);
 }
 
+// Arguments of the forward kernels (geometry in FP64, device pointers).
+struct FwdArgs {
+    int nx, ny, nz, nsx, nsy, L;
+    double dx, dy, ox, oy;           // voxel axes (x, y)
+    double dsx, dsy, sx0, sy0;       // spot i centre = sx0 + i * dsx (sx0 = x0 - (nsx-1)/2 dsx)
+    const float* w;                  // (L, nsy, nsx)
+    const double* idd;               // (L, nz)
+    const double* sigma;             // (L, nz)
+    float* dose;                     // (nz, ny, nx)
+    int tiles_x, tiles_y;
+};
+
 }  // namespace pbh

@@ -25,16 +25,6 @@ constexpr int FWD_THREADS = 256;
 constexpr int SC = PBH_SPOT_CHUNK;   // 48
 constexpr int LDS_STRIDE = SC + 1;   // odd stride: spreads rows over banks
 
-struct FwdArgs {
-    int nx, ny, nz, nsx, nsy, L;
-    double dx, dy, ox, oy;           // voxel axes (x, y)
-    double dsx, dsy, sx0, sy0;       // spot i centre = sx0 + i * dsx (sx0 = x0 - (nsx-1)/2 dsx)
-    const float* w;                  // (L, nsy, nsx)
-    const double* idd;               // (L, nz)
-    const double* sigma;             // (L, nz)
-    float* dose;                     // (nz, ny, nx)
-    int tiles_x, tiles_y;
-};
 
 __global__ __launch_bounds__(FWD_THREADS)
 void fwd_fp32_kernel(FwdArgs a)
@@ -121,7 +111,10 @@ void fwd_fp32_kernel(FwdArgs a)
                 }
                 __syncthreads();
 
-                // stage B: D[64 x 32 columns of this wave] += (cU) Gx^T
+                // stage B: D[64 x 32 columns of this wave] += (cU) Gx^T. The layer's
+                // partial is summed in fresh accumulators and added to D once, which
+                // keeps the FP32 rounding chain per voxel at (K steps + layers).
+                v16f p0 = {}, p1 = {};
 #pragma unroll
                 for (int kk = 0; kk < SC / 2; ++kk) {
                     if (kk < nkB) {
@@ -129,10 +122,12 @@ void fwd_fp32_kernel(FwdArgs a)
                         const int k = 2 * kk + (lane >> 5);
                         const float a0 = Us[(lane & 31) * LDS_STRIDE + k];
                         const float a1 = Us[(32 + (lane & 31)) * LDS_STRIDE + k];
-                        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, g, acc0, 0, 0, 0);
-                        acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, g, acc1, 0, 0, 0);
+                        p0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, g, p0, 0, 0, 0);
+                        p1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, g, p1, 0, 0, 0);
                     }
                 }
+                acc0 += p0;
+                acc1 += p1;
                 __syncthreads();
             }
         }
@@ -189,7 +184,8 @@ FwdArgs make_fwd_args(const PbhGrid* g, const PbhLattice* lat, const float* w,
     return a;
 }
 
-int pbh_forward_mixed(const FwdArgs& a, int precision, void* ws, size_t ws_bytes, hipStream_t s);
+int pbh_forward_mixed(const FwdArgs& a, double oz, double dz, int precision, void* ws, size_t ws_bytes,
+                      hipStream_t s);
 size_t pbh_forward_mixed_ws(const PbhGrid* g, const PbhLattice* lat, int precision);
 
 extern "C" size_t pbh_forward_workspace_bytes(const PbhGrid* g, const PbhLattice* lat, int precision)
@@ -210,7 +206,7 @@ extern "C" int pbh_forward(const PbhGrid* g, const PbhLattice* lat, const float*
     const FwdArgs a = make_fwd_args(g, lat, w, idd, sigma, dose);
     hipStream_t s = (hipStream_t)stream;
     if (precision != PBH_PREC_FP32)
-        return pbh_forward_mixed(a, precision, workspace, workspace_bytes, s);
+        return pbh_forward_mixed(a, g->oz, g->dz, precision, workspace, workspace_bytes, s);
     const long long blocks = (long long)a.tiles_x * a.tiles_y * g->nz;
     if (blocks > 0x7fffffffLL) return PBH_ERR_UNSUPPORTED;
     hipLaunchKernelGGL(fwd_fp32_kernel, dim3((unsigned)blocks), dim3(FWD_THREADS), 0, s, a);

@@ -108,3 +108,24 @@ def test_linearity_and_determinism():
     d12 = dp.forward(w=(dp.w + 3.0 * w2).contiguous())
     ref = (a.double() + 3.0 * d2.double())
     assert (d12.double() - ref).abs().max() <= 1e-6 * ref.abs().max()
+
+
+# Per-form bounds on the max global relative error vs the FP64 oracle. Single
+# 16-bit products carry the factor rounding (2^-9 BF16, 2^-12 FP16); the split
+# forms drop only lo*lo.
+MIXED_TOL = {"bf16": 5e-3, "fp16": 1e-3, "bf16x3": 2e-5, "fp16x3": 2e-6,
+             "bf16x3_peak": 5e-3, "fp16x3_peak": 1e-3}
+
+
+@pytest.mark.parametrize("precision", list(MIXED_TOL))
+def test_mixed_precision_forms(precision):
+    print(f"case mixed {precision} on 64^3 / 21x21 / 10 layers", flush=True)
+    p = make_problem(64, 64, 64, 0.2, 21, 21, 0.5, np.linspace(100, 150, 10), seed=4)
+    check(p, tol=MIXED_TOL[precision], precision=precision)
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16x3"])
+def test_mixed_ragged_and_chunked(precision):
+    print(f"case mixed {precision} ragged 70x37x5, lattice 60x50", flush=True)
+    p = make_problem(70, 37, 5, 0.25, 60, 50, 0.3, [90.0, 150.0], seed=9, x0=0.2)
+    check(p, tol=MIXED_TOL[precision], precision=precision)
