@@ -1,155 +1,222 @@
-// pbh_forward.hip -- fused forward dose contraction, FP32 MFMA (gfx950).
+// pbh_forward.hip -- fused forward dose contraction, FP32 MFMA (gfx950), and the
+// host-side planning shared with the mixed-precision forms.
 //
-// For one depth slice z, the dose tile D[y0:y0+64, x0:x0+128] is
-//     D = sum_l sum_chunks  (c_lz * Gy_lz[:, J] W_l[J, I]) Gx_lz[:, I]^T
-// with c_lz = IDD_l(z) / (2 pi s^2). Each workgroup (4 waves) owns one tile and
-// loops over layers and spot chunks, keeping D in MFMA accumulators, so no
-// per-layer intermediate ever reaches HBM (the reference materialises Gx, Gy,
-// A, T and Dl per layer: engines.py:261-274).
+// For one depth slice z, a dose tile D[y0:y0+64, x0:x0+128] is
+//     D = sum_l sum_chunks (c_lz * Gy_lz[:, J] W_l[J, I]) Gx_lz[:, I]^T
+// with c_lz = IDD_l(z) / (2 pi s^2). A workgroup (4 waves) owns one tile and a
+// share of its layers (layer split S: layers s, s+S, ...), keeping D in MFMA
+// accumulators, so no per-layer intermediate reaches HBM (the reference
+// materialises Gx, Gy, A, T and Dl per layer: engines.py:261-274).
 //
-//   stage A  U[64 x 48]  = Gy[64 x J] * W[J x I]     v_mfma_f32_16x16x4f32
-//            wave w computes rows 16w..16w+15; Gy is built in registers
-//            (A operand), W is staged in LDS.
-//   stage B  D[64 x 128] += (c U)[64 x I] * Gx^T      v_mfma_f32_32x32x2f32
-//            wave w computes columns 32w..32w+31; Gx is built in registers
-//            (B operand), c*U is read from LDS.
-// Squared voxel-spot distances are evaluated in FP64 once per chunk and kept
-// in registers; per layer only d2 * k2 and one exp2 remain per factor.
-#include "pbh_common.h"
+//   stage A  U[64 x 48]  = Gy[64 x J] W[J x I]      v_mfma_f32_16x16x4f32
+//            wave w: rows 16w..16w+15; Gy is the A operand, W (LDS) the B operand.
+//   stage B  D[64 x 128] += (c U)[64 x I] Gx^T       v_mfma_f32_32x32x2f32
+//            wave w: columns 32w..32w+31; (c U) from LDS, Gx the B operand.
+//
+// Gaussian factors. On regular grids whose voxel and spot pitches have a small
+// integer ratio (dx = P u, ds = Q u: 2 mm voxels and 5 mm spots give P = 2,
+// Q = 5), voxel-spot distances within a tile take only P (T-1) + Q (K-1) + 1
+// values, so each (layer, depth) builds one small table per axis in LDS (FP64
+// argument, one exp2 each) and the MFMA operands are gathered from it: ~490
+// exponentials per 128 x 48 tile instead of ~6000. Other grids evaluate every
+// factor directly (FP64 distance, FP32 exponent).
+//
+// Each layer's contribution is summed in fresh accumulators and added to the
+// running dose once (rounding chain per voxel: K steps + layers). Tables, W and
+// U are double-buffered in LDS: two barriers per layer.
+#include "pbh_fwd_core.h"
 
 namespace pbh {
 
-constexpr int FWD_TY = 64;
-constexpr int FWD_TX = 128;
-constexpr int FWD_THREADS = 256;
-constexpr int SC = PBH_SPOT_CHUNK;   // 48
-constexpr int LDS_STRIDE = SC + 1;   // odd stride: spreads rows over banks
-
-
-__global__ __launch_bounds__(FWD_THREADS)
-void fwd_fp32_kernel(FwdArgs a)
+// Per spot chunk, the active layers of the depth slice are processed as a
+// software pipeline with one barrier per layer: phase k runs stage B of layer k
+// and stage A of layer k+1, and builds the x table of k+1 and the y table and
+// W chunk of k+2 (double-buffered by parity).
+// Stage A: 16x16 blocks of U (2 row blocks x NCB column blocks); wave w takes
+// block (w % 2, w / 2) and, when NCB = 3, block (w % 2, 2) for w < 2.
+// Stage B: wave w owns the 32 x 32 dose block at columns 32 w.
+template <bool TAB, int SCP>
+__global__ __launch_bounds__(FT_THREADS, 2)
+void fwd_fp32_kernel(FwdArgs a, FwdPlan pl)
 {
-    __shared__ float Ws[SC * LDS_STRIDE];        // W chunk [j][i]
-    __shared__ float Us[FWD_TY * LDS_STRIDE];    // c * U   [y][i]
+    constexpr int KA = SCP / 4;            // stage A K steps
+    constexpr int KB = SCP / 2;            // stage B K steps
+    constexpr int NCB = (SCP + 15) / 16;   // stage A column blocks
+    constexpr int UW = 16 * NCB;           // U / W columns
+    constexpr int US = UW + 1;             // odd LDS row stride
+    constexpr int NW = (SCP * UW + FT_THREADS - 1) / FT_THREADS;   // W elements per thread
+
+    __shared__ float Ws[2][SCP * US];       // W chunk [j][i]
+    __shared__ float Us[2][FT_TY * US];     // c U [y][i]
+    __shared__ float Tx[TAB ? 2 : 1][TAB ? FT_TAB : 1];
+    __shared__ float Ty[TAB ? 2 : 1][TAB ? FT_TAB : 1];
+    __shared__ LayerList ll;
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    int b = blockIdx.x;
-    const int tx = b % a.tiles_x; b /= a.tiles_x;
-    const int ty = b % a.tiles_y; b /= a.tiles_y;
-    const int z = b;
-    const int x0 = tx * FWD_TX, y0 = ty * FWD_TY;
+    const int s = blockIdx.x % pl.split;
+    const int tile = blockIdx.x / pl.split;
+    const int tx = tile % a.tiles_x;
+    const int ty = (tile / a.tiles_x) % a.tiles_y;
+    const int z = tile / (a.tiles_x * a.tiles_y);
+    const int x0 = tx * FT_TX, y0 = ty * FT_TY;
 
-    // stage-A lane geometry: row yA (A operand row = lane % 16), k = 4 kk + lane / 16
-    const int yA = y0 + 16 * wave + (lane & 15);
-    // stage-B lane geometry: column xB (B operand column = lane % 32), k = 2 kk + lane / 32
-    const int xB = x0 + 32 * wave + (lane & 31);
-    const double yAc = a.oy + (double)yA * a.dy;
-    const double xBc = a.ox + (double)xB * a.dx;
-
-    v16f acc0 = {}, acc1 = {};
+    const int rbA = wave & 1;                        // stage A row block
+    const int bA = 16 * rbA + (lane & 15);           // stage A row in the tile
+    const int cbA0 = wave >> 1;                      // first column block
+    const bool oneA = cbA0 < NCB;
+    const bool twoA = NCB == 3 && wave < 2;          // second block: column block 2
+    const int cnB = 32 * wave;                       // stage B columns
+    const int aB = cnB + (lane & 31);
+    const double yAc = a.oy + (double)(y0 + bA) * a.dy;
+    const double xBc = a.ox + (double)(x0 + aB) * a.dx;
+    // table entry of this lane's k = 0 operand, and its step per K step
+    const int tA0 = pl.ty.Q * (lane >> 4) - pl.ty.P * bA + pl.ty.P * (FT_TY - 1);
+    const int tB0 = pl.tx.Q * (lane >> 5) - pl.tx.P * aB + pl.tx.P * (FT_TX - 1);
+    const int ntY = pl.ty.P * (FT_TY - 1) + pl.ty.Q * (SCP - 1) + 1;
+    const int ntX = pl.tx.P * (FT_TX - 1) + pl.tx.Q * (SCP - 1) + 1;
 
     const int ncx = chunk_count(a.nsx), ncy = chunk_count(a.nsy);
-    for (int cy = 0; cy < ncy; ++cy) {
-        const int j0 = chunk_begin(a.nsy, ncy, cy), nj = chunk_begin(a.nsy, ncy, cy + 1) - j0;
-        const int nkA = (nj + 3) >> 2;   // K steps of 4 in stage A
-        // d2 for Gy: k = 4 kk + lane/16, kk < 12 (48 / 4)
-        float dyA[SC / 4];
-#pragma unroll
-        for (int kk = 0; kk < SC / 4; ++kk) {
-            const int j = 4 * kk + (lane >> 4);
-            const double d = yAc - (a.sy0 + (double)(j0 + j) * a.dsy);
-            dyA[kk] = (j < nj && yA < a.ny) ? (float)(d * d) : -1.0f;   // -1: padded (factor 0)
-        }
-        for (int cx = 0; cx < ncx; ++cx) {
-            const int i0 = chunk_begin(a.nsx, ncx, cx), ni = chunk_begin(a.nsx, ncx, cx + 1) - i0;
-            const int nkB = (ni + 1) >> 1;   // K steps of 2 in stage B
-            float dxB[SC / 2];
-#pragma unroll
-            for (int kk = 0; kk < SC / 2; ++kk) {
-                const int i = 2 * kk + (lane >> 5);
-                const double d = xBc - (a.sx0 + (double)(i0 + i) * a.dsx);
-                dxB[kk] = (i < ni) ? (float)(d * d) : -1.0f;
-            }
-            for (int l = 0; l < a.L; ++l) {
-                const double iddv = a.idd[(size_t)l * a.nz + z];
-                if (iddv == 0.0) continue;   // layer deposits nothing at this depth (uniform branch)
-                const LayerScalars sc = layer_scalars(iddv, a.sigma[(size_t)l * a.nz + z]);
 
-                // stage W chunk into LDS (zero padded to 48 x 48)
-                const float* wl = a.w + (size_t)l * a.nsy * a.nsx;
-                for (int e = threadIdx.x; e < SC * SC; e += FWD_THREADS) {
-                    const int j = e / SC, i = e % SC;
-                    Ws[j * LDS_STRIDE + i] = (j < nj && i < ni) ? wl[(size_t)(j0 + j) * a.nsx + i0 + i] : 0.0f;
-                }
-                __syncthreads();
+    v16f acc = {};
+    float wreg[NW];
 
-                // stage A: U[16 rows of this wave][48] = Gy * W
-                v4f u0 = {}, u1 = {}, u2 = {};
+    for (int lb = s; lb < a.L; lb += FT_LB * pl.split) {
+        const int nact = collect_layers(a, lb, pl.split, z, 0.0f, nullptr, ll);
+        if (nact == 0) continue;
+        for (int cy = 0; cy < ncy; ++cy) {
+            const int j0 = chunk_begin(a.nsy, ncy, cy), nj = chunk_begin(a.nsy, ncy, cy + 1) - j0;
+            const double Cy = (a.oy + (double)y0 * a.dy) - (a.sy0 + (double)j0 * a.dsy);
+            for (int cx = 0; cx < ncx; ++cx) {
+                const int i0 = chunk_begin(a.nsx, ncx, cx), ni = chunk_begin(a.nsx, ncx, cx + 1) - i0;
+                const double Cx = (a.ox + (double)x0 * a.dx) - (a.sx0 + (double)i0 * a.dsx);
+
+                auto load_w = [&](int m) {
+                    const float* wl = a.w + (size_t)ll.lay[m] * a.nsy * a.nsx + (size_t)j0 * a.nsx + i0;
 #pragma unroll
-                for (int kk = 0; kk < SC / 4; ++kk) {
-                    if (kk < nkA) {
-                        const float g = dyA[kk] >= 0.0f ? gauss2(dyA[kk], sc.k2) : 0.0f;
-                        const int k = 4 * kk + (lane >> 4);
-                        const float* wr = &Ws[k * LDS_STRIDE + (lane & 15)];
-                        u0 = __builtin_amdgcn_mfma_f32_16x16x4f32(g, wr[0], u0, 0, 0, 0);
-                        u1 = __builtin_amdgcn_mfma_f32_16x16x4f32(g, wr[16], u1, 0, 0, 0);
-                        u2 = __builtin_amdgcn_mfma_f32_16x16x4f32(g, wr[32], u2, 0, 0, 0);
+                    for (int q = 0; q < NW; ++q) {
+                        const int e = threadIdx.x + q * FT_THREADS;
+                        const int j = e / UW, i = e % UW;
+                        wreg[q] = (e < SCP * UW && j < nj && i < ni) ? wl[(size_t)j * a.nsx + i] : 0.0f;
                     }
-                }
-                // C layout of 16x16x4: row = 4 (lane / 16) + r, column = lane % 16
-                {
-                    const int rbase = 16 * wave + 4 * (lane >> 4);
-                    const int col = lane & 15;
+                };
+                auto store_w = [&](int b) {
+#pragma unroll
+                    for (int q = 0; q < NW; ++q) {
+                        const int e = threadIdx.x + q * FT_THREADS;
+                        if (e < SCP * UW) Ws[b][(e / UW) * US + (e % UW)] = wreg[q];
+                    }
+                };
+                auto build_tx = [&](int m) {
+                    if (!TAB) return;
+                    const double k2d = ll.k2d[m];
+                    for (int t = threadIdx.x; t < ntX; t += FT_THREADS) {
+                        const double d = table_distance(pl.tx, Cx, FT_TX, t);
+                        Tx[m & 1][table_slot(pl.tx, t)] = __builtin_amdgcn_exp2f((float)(-(d * d) * k2d));
+                    }
+                };
+                auto build_ty = [&](int m) {
+                    if (!TAB) return;
+                    const double k2d = ll.k2d[m];
+                    for (int t = threadIdx.x; t < ntY; t += FT_THREADS) {
+                        const double d = table_distance(pl.ty, Cy, FT_TY, t);
+                        Ty[m & 1][table_slot(pl.ty, t)] = __builtin_amdgcn_exp2f((float)(-(d * d) * k2d));
+                    }
+                };
+                // stage A of layer m: U[m&1] = c Gy W
+                auto stage_a = [&](int m) {
+                    const int b = m & 1;
+                    const float c = ll.c[m], k2 = ll.k2[m];
+                    if (!oneA) return;
+                    float g[KA];
+#pragma unroll
+                    for (int kk = 0; kk < KA; ++kk) {
+                        if (TAB) {
+                            g[kk] = Ty[b][table_slot(pl.ty, tA0 + 4 * pl.ty.Q * kk)];
+                        } else {
+                            const int j = 4 * kk + (lane >> 4);
+                            const double d = yAc - (a.sy0 + (double)(j0 + j) * a.dsy);
+                            g[kk] = (j < nj) ? gauss2((float)(d * d), k2) : 0.0f;
+                        }
+                    }
+                    v4f u0 = {}, u1 = {};
+#pragma unroll
+                    for (int kk = 0; kk < KA; ++kk) {
+                        const float* wr = &Ws[b][(4 * kk + (lane >> 4)) * US + (lane & 15)];
+                        u0 = __builtin_amdgcn_mfma_f32_16x16x4f32(g[kk], wr[16 * cbA0], u0, 0, 0, 0);
+                        if (twoA) u1 = __builtin_amdgcn_mfma_f32_16x16x4f32(g[kk], wr[32], u1, 0, 0, 0);
+                    }
+                    // C layout of 16x16x4: row 4 (lane / 16) + r, column lane % 16
+                    const int rbase = 16 * rbA + 4 * (lane >> 4);
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
-                        float* urow = &Us[(rbase + r) * LDS_STRIDE + col];
-                        urow[0] = sc.c * u0[r];
-                        urow[16] = sc.c * u1[r];
-                        urow[32] = sc.c * u2[r];
+                        float* urow = &Us[b][(rbase + r) * US + (lane & 15)];
+                        urow[16 * cbA0] = c * u0[r];
+                        if (twoA) urow[32] = c * u1[r];
                     }
-                }
-                __syncthreads();
-
-                // stage B: D[64 x 32 columns of this wave] += (cU) Gx^T. The layer's
-                // partial is summed in fresh accumulators and added to D once, which
-                // keeps the FP32 rounding chain per voxel at (K steps + layers).
-                v16f p0 = {}, p1 = {};
+                };
+                // stage B of layer m: acc += (c U) Gx^T; operands are gathered first
+                auto stage_b = [&](int m) {
+                    const int b = m & 1;
+                    const float k2 = ll.k2[m];
+                    float gv[KB], av[KB];
 #pragma unroll
-                for (int kk = 0; kk < SC / 2; ++kk) {
-                    if (kk < nkB) {
-                        const float g = dxB[kk] >= 0.0f ? gauss2(dxB[kk], sc.k2) : 0.0f;
-                        const int k = 2 * kk + (lane >> 5);
-                        const float a0 = Us[(lane & 31) * LDS_STRIDE + k];
-                        const float a1 = Us[(32 + (lane & 31)) * LDS_STRIDE + k];
-                        p0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, g, p0, 0, 0, 0);
-                        p1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, g, p1, 0, 0, 0);
+                    for (int kk = 0; kk < KB; ++kk) {
+                        if (TAB) {
+                            gv[kk] = Tx[b][table_slot(pl.tx, tB0 + 2 * pl.tx.Q * kk)];
+                        } else {
+                            const int i = 2 * kk + (lane >> 5);
+                            const double d = xBc - (a.sx0 + (double)(i0 + i) * a.dsx);
+                            gv[kk] = (i < ni) ? gauss2((float)(d * d), k2) : 0.0f;
+                        }
+                        av[kk] = Us[b][(lane & 31) * US + 2 * kk + (lane >> 5)];
                     }
-                }
-                acc0 += p0;
-                acc1 += p1;
+                    v16f p = {};
+#pragma unroll
+                    for (int kk = 0; kk < KB; ++kk)
+                        p = __builtin_amdgcn_mfma_f32_32x32x2f32(av[kk], gv[kk], p, 0, 0, 0);
+                    acc += p;
+                };
+
+                // prologue: layer 0 staged, U(0) computed, Tx(0), W(1), Ty(1) staged
+                load_w(0);
+                store_w(0);
+                build_ty(0);
+                if (nact > 1) load_w(1);
                 __syncthreads();
+                stage_a(0);
+                build_tx(0);
+                if (nact > 1) {
+                    store_w(1);
+                    build_ty(1);
+                }
+                __syncthreads();
+                for (int m = 0; m < nact; ++m) {
+                    if (m + 2 < nact) load_w(m + 2);
+                    stage_b(m);
+                    if (m + 1 < nact) {
+                        stage_a(m + 1);
+                        build_tx(m + 1);
+                    }
+                    if (m + 2 < nact) {
+                        store_w(m & 1);
+                        build_ty(m + 2);
+                    }
+                    __syncthreads();
+                }
             }
         }
     }
-
-    // C layout of 32x32: row = 8 (r / 4) + 4 (lane / 32) + r % 4, column = lane % 32
-    if (xB < a.nx) {
-        float* base = a.dose + (size_t)z * a.ny * a.nx + xB;
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int row = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            const int y = y0 + row;
-            if (y < a.ny) base[(size_t)y * a.nx] = acc0[r];
-            if (y + 32 < a.ny) base[(size_t)(y + 32) * a.nx] = acc1[r];
-        }
-    }
+    store_block(a, pl, tile, s, z, y0, x0, 0, cnB, acc);
 }
 
 }  // namespace pbh
 
 using namespace pbh;
 
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
 static int validate_problem(const PbhGrid* g, const PbhLattice* lat)
 {
     if (!g || !lat) return PBH_ERR_NULL_ARG;
@@ -179,36 +246,143 @@ FwdArgs make_fwd_args(const PbhGrid* g, const PbhLattice* lat, const float* w,
     a.sx0 = lat->x0 - (lat->n_sx - 1) / 2.0 * lat->dx_spot;
     a.sy0 = lat->y0 - (lat->n_sy - 1) / 2.0 * lat->dy_spot;
     a.w = w; a.idd = idd; a.sigma = sigma; a.dose = dose;
-    a.tiles_x = cdiv(g->nx, FWD_TX);
-    a.tiles_y = cdiv(g->ny, FWD_TY);
+    a.tiles_x = cdiv(g->nx, FT_TX);
+    a.tiles_y = cdiv(g->ny, FT_TY);
     return a;
 }
 
-int pbh_forward_mixed(const FwdArgs& a, double oz, double dz, int precision, void* ws, size_t ws_bytes,
-                      hipStream_t s);
-size_t pbh_forward_mixed_ws(const PbhGrid* g, const PbhLattice* lat, int precision);
+// dv = P u, ds = Q u with small integers and a table that fits FT_TAB entries.
+static bool axis_table(double dv, double ds, int T, int K, AxisTable* out)
+{
+    for (int Q = 1; Q <= 64; ++Q) {
+        const double pr = dv * Q / ds;
+        const int P = (int)llround(pr);
+        if (P < 1 || P > 64) continue;
+        if (fabs(pr - P) > 1e-9 * pr) continue;
+        if (P * (T - 1) + Q * (K - 1) + 2 > FT_TAB) return false;
+        out->P = P;
+        out->Q = Q;
+        out->u = dv / P;
+        out->half = (P == 2) ? (P * (T - 1) + Q * (K - 1) + 2) / 2 : 0;
+        return true;
+    }
+    return false;
+}
+
+static int device_cus()
+{
+    static int cus[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (cus[dev] == 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+            n = 256;
+        cus[dev] = n;
+    }
+    return cus[dev];
+}
+
+static long long tile_count(const FwdArgs& a) { return (long long)a.tiles_x * a.tiles_y * a.nz; }
+
+// Smallest split (1, 2, 4, 8) that gives at least one workgroup per CU.
+int pbh_auto_split(const FwdArgs& a)
+{
+    const long long target = (long long)device_cus();
+    int s = 1;
+    while (s < 8 && tile_count(a) * s < target && s * 2 <= a.L) s *= 2;
+    return s;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Workspace layout: [counters][layer peaks (mixed "_peak" forms)][partial tiles].
+size_t pbh_forward_ws_bytes(const FwdArgs& a, int split)
+{
+    size_t n = align256((size_t)tile_count(a) * sizeof(unsigned));
+    n += align256((size_t)a.L * sizeof(float));
+    if (split > 1) n += (size_t)split * tile_count(a) * FT_TY * FT_TX * sizeof(float);
+    return n;
+}
+
+float* pbh_forward_ws_peaks(const FwdArgs& a, void* ws)
+{
+    return (float*)((char*)ws + align256((size_t)tile_count(a) * sizeof(unsigned)));
+}
+
+int pbh_plan_forward(const FwdArgs& a, int kx_pad, int ky_pad, void* ws, size_t ws_bytes,
+                     int split_override, FwdPlan* plan)
+{
+    int split = split_override > 0 ? split_override : pbh_auto_split(a);
+    if (split != 1 && split != 2 && split != 4 && split != 8) return PBH_ERR_UNSUPPORTED;
+    if (tile_count(a) * split > 0x7fffffffLL) return PBH_ERR_UNSUPPORTED;
+    plan->split = split;
+    plan->table = axis_table(a.dx, a.dsx, FT_TX, kx_pad, &plan->tx) &&
+                  axis_table(a.dy, a.dsy, FT_TY, ky_pad, &plan->ty);
+    if (!plan->table) {
+        plan->tx = AxisTable{0, 0, 0.0, 0};
+        plan->ty = AxisTable{0, 0, 0.0, 0};
+    }
+    plan->part = nullptr;
+    plan->ctr = nullptr;
+    if (split > 1) {
+        if (!ws) return PBH_ERR_NULL_ARG;
+        if (ws_bytes < pbh_forward_ws_bytes(a, split)) return PBH_ERR_WORKSPACE;
+        char* p = (char*)ws;
+        plan->ctr = (unsigned*)p;
+        plan->part = (float*)(p + align256((size_t)tile_count(a) * sizeof(unsigned))
+                              + align256((size_t)a.L * sizeof(float)));
+    }
+    return PBH_OK;
+}
+
+int pbh_forward_mixed(const FwdArgs& a, double oz, double dz, int precision, int split,
+                      void* ws, size_t ws_bytes, hipStream_t s);
 
 extern "C" size_t pbh_forward_workspace_bytes(const PbhGrid* g, const PbhLattice* lat, int precision)
 {
-    if (!g || !lat) return 0;
-    if (precision == PBH_PREC_FP32) return 0;
-    return pbh_forward_mixed_ws(g, lat, precision);
+    if (validate_problem(g, lat) != PBH_OK) return 0;
+    (void)precision;
+    const FwdArgs a = make_fwd_args(g, lat, nullptr, nullptr, nullptr, nullptr);
+    return pbh_forward_ws_bytes(a, 8);   // enough for any layer split
+}
+
+extern "C" int pbh_forward_tuned(const PbhGrid* g, const PbhLattice* lat, const float* w,
+                                 const double* idd, const double* sigma, float* dose, int precision,
+                                 int split, void* workspace, size_t workspace_bytes, void* stream)
+{
+    int rc = validate_problem(g, lat);
+    if (rc != PBH_OK) return rc;
+    if (!w || !idd || !sigma || !dose) return PBH_ERR_NULL_ARG;
+    if (precision < PBH_PREC_FP32 || precision > PBH_PREC_FP16X3_PEAK) return PBH_ERR_UNSUPPORTED;
+    if (split < 0) return PBH_ERR_BAD_DIM;
+    const FwdArgs a = make_fwd_args(g, lat, w, idd, sigma, dose);
+    hipStream_t s = (hipStream_t)stream;
+    if (precision != PBH_PREC_FP32)
+        return pbh_forward_mixed(a, g->oz, g->dz, precision, split, workspace, workspace_bytes, s);
+    FwdPlan pl;
+    rc = pbh_plan_forward(a, chunk_pad(a.nsx, a.nsy), chunk_pad(a.nsx, a.nsy), workspace, workspace_bytes, split, &pl);
+    if (rc != PBH_OK) return rc;
+    const dim3 grid((unsigned)(tile_count(a) * pl.split));
+    switch (chunk_pad(a.nsx, a.nsy)) {
+#define PBH_FP32_CASE(N) \
+    case N: \
+        if (pl.table) hipLaunchKernelGGL((fwd_fp32_kernel<true, N>), grid, dim3(FT_THREADS), 0, s, a, pl); \
+        else hipLaunchKernelGGL((fwd_fp32_kernel<false, N>), grid, dim3(FT_THREADS), 0, s, a, pl); \
+        break;
+    PBH_FP32_CASE(4) PBH_FP32_CASE(8) PBH_FP32_CASE(12) PBH_FP32_CASE(16) PBH_FP32_CASE(20)
+    PBH_FP32_CASE(24) PBH_FP32_CASE(28) PBH_FP32_CASE(32) PBH_FP32_CASE(36) PBH_FP32_CASE(40)
+    PBH_FP32_CASE(44) PBH_FP32_CASE(48)
+#undef PBH_FP32_CASE
+    default: return PBH_ERR_UNSUPPORTED;
+    }
+    return launch_rc("fwd_fp32_kernel");
 }
 
 extern "C" int pbh_forward(const PbhGrid* g, const PbhLattice* lat, const float* w,
                            const double* idd, const double* sigma, float* dose, int precision,
                            void* workspace, size_t workspace_bytes, void* stream)
 {
-    int rc = validate_problem(g, lat);
-    if (rc != PBH_OK) return rc;
-    if (!w || !idd || !sigma || !dose) return PBH_ERR_NULL_ARG;
-    if (precision < PBH_PREC_FP32 || precision > PBH_PREC_FP16X3_PEAK) return PBH_ERR_UNSUPPORTED;
-    const FwdArgs a = make_fwd_args(g, lat, w, idd, sigma, dose);
-    hipStream_t s = (hipStream_t)stream;
-    if (precision != PBH_PREC_FP32)
-        return pbh_forward_mixed(a, g->oz, g->dz, precision, workspace, workspace_bytes, s);
-    const long long blocks = (long long)a.tiles_x * a.tiles_y * g->nz;
-    if (blocks > 0x7fffffffLL) return PBH_ERR_UNSUPPORTED;
-    hipLaunchKernelGGL(fwd_fp32_kernel, dim3((unsigned)blocks), dim3(FWD_THREADS), 0, s, a);
-    return launch_rc("fwd_fp32_kernel");
+    return pbh_forward_tuned(g, lat, w, idd, sigma, dose, precision, 0, workspace,
+                             workspace_bytes, stream);
 }

@@ -1,33 +1,36 @@
 // pbh_mixed.hip -- mixed-precision forward dose: BF16 / FP16 Gaussian factors
 // on gfx950 MFMA with FP32 accumulation, optionally as split products.
 //
-// Same tiling as the FP32 kernel (one workgroup = 64 rows x 128 columns of one
-// depth slice, 4 waves, layers and spot chunks looped inside):
+// Same tiling, layer split, Gaussian tables and double buffering as the FP32
+// kernel (pbh_forward.hip):
 //   stage A  U[64 x 48]  = Gy[64 x J] W[J x I]      v_mfma_f32_16x16x32_{bf16,f16}
 //   stage B  D[64 x 128] += (c U) Gx^T              v_mfma_f32_32x32x16_{bf16,f16}
-// Factors are formed in FP32 (exp2 of an FP32 argument), then rounded to the
-// 16-bit type. In the split ("x3") form every operand a is stored as
-// hi = T(a), lo = T(a - hi) and a product is hi*hi + hi*lo + lo*hi (lo*lo is
-// dropped), which restores ~2^-16 (BF16) or ~2^-21 (FP16) relative accuracy per
-// product at three MFMAs instead of one. The "_peak" forms use the split
-// products only for (layer, depth) pairs within 1 cm of the layer's Bragg peak
-// (the depth of its IDD maximum) and single products elsewhere.
+// Factors are formed in FP32 (table: FP64 argument), then rounded to the 16-bit
+// type. In the split ("x3") forms every operand a is stored as hi = T(a),
+// lo = T(a - hi) and a product is hi*hi + hi*lo + lo*hi (lo*lo dropped), which
+// restores ~2^-16 (BF16) or ~2^-21 (FP16) relative accuracy per product at
+// three MFMAs instead of one. Table entries hold the (hi, lo) pair packed in 32
+// bits; operands are assembled with v_perm. The "_peak" forms use split products
+// only for (layer, depth) pairs within 1 cm of the layer's Bragg peak (depth of
+// its IDD maximum) and single products elsewhere.
 //
 // Each layer's contribution is summed in fresh FP32 accumulators and added to
 // the running dose once. FP16 has a narrow exponent range, so U is scaled per
-// wave by a power of two (its largest magnitude is brought into [1, 2)) and the
-// inverse scale is applied to the layer's partial when it joins the running
-// dose. BF16 shares FP32's exponent range and needs no scaling.
-#include "pbh_common.h"
+// wave by a power of two (largest magnitude into [1, 2)) and the inverse scale
+// is applied to the layer's partial when it joins the running dose. BF16
+// shares FP32's exponent range and needs no scaling.
+#include "pbh_fwd_core.h"
 
 namespace pbh {
 
-constexpr int MX_TY = 64, MX_TX = 128, MX_THREADS = 256;
 constexpr int SCH = PBH_SPOT_CHUNK;       // 48 spots per chunk
-constexpr int WT_STRIDE = 64 + 8;         // W^T rows: k (j) padded to 64, +8 halves of pad
-constexpr int U_STRIDE = SCH + 8;         // U rows: k (i) padded to 48, +8 halves of pad
+constexpr int KYP = 64;                   // stage A K (j) padded to 2 steps of 32
+constexpr int WT_STRIDE = KYP + 8;        // W^T rows in halves
+constexpr int U_STRIDE = SCH + 8;         // U rows in halves
 
 enum { MODE_PLAIN = 0, MODE_X3 = 1, MODE_PEAK = 2 };
+
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
 
 template <typename T> struct Vec8;
 template <> struct Vec8<__bf16> { typedef v8bf type; };
@@ -45,6 +48,32 @@ __device__ __forceinline__ void split(float x, T& hi, T& lo)
     lo = (T)(x - (float)hi);
 }
 
+template <typename T>
+__device__ __forceinline__ unsigned short bits16(T v) { return __builtin_bit_cast(unsigned short, v); }
+
+// (hi, lo) of x packed as hi | lo << 16
+template <typename T>
+__device__ __forceinline__ unsigned pack_split(float x)
+{
+    T h, l;
+    split<T>(x, h, l);
+    return (unsigned)bits16(h) | ((unsigned)bits16(l) << 16);
+}
+
+// Eight packed entries -> hi and lo operand vectors.
+template <typename V8>
+__device__ __forceinline__ void unpack8(const unsigned* t, V8& hi, V8& lo)
+{
+    u4v h, l;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        h[q] = __builtin_amdgcn_perm(t[2 * q + 1], t[2 * q], 0x05040100u);
+        l[q] = __builtin_amdgcn_perm(t[2 * q + 1], t[2 * q], 0x07060302u);
+    }
+    hi = __builtin_bit_cast(V8, h);
+    lo = __builtin_bit_cast(V8, l);
+}
+
 // r_peak[l] = depth of the first IDD maximum of layer l (np.argmax semantics).
 __global__ void peak_depth_f32_kernel(const double* idd, int L, int nz, double oz, double dz, float* r_peak)
 {
@@ -57,243 +86,328 @@ __global__ void peak_depth_f32_kernel(const double* idd, int L, int nz, double o
     r_peak[l] = (float)(oz + (double)best * dz);
 }
 
-template <typename T, int MODE>
-__global__ __launch_bounds__(MX_THREADS)
-void fwd_mixed_kernel(FwdArgs a, double oz, double dz, const float* r_peak)
+// 4 waves (two workgroups per CU), one-barrier software pipeline over the
+// active layers of each spot chunk, as in the FP32 kernel. Stage A: 16x16 blocks
+// of U (2 row blocks x 3 column blocks); wave w takes block (w % 2, w / 2) and,
+// for w < 2, block (w % 2, 2). Stage B: wave w owns the 32 x 32 dose block at
+// columns 32 w; KB K steps of 16 spots.
+template <typename T, int MODE, bool TAB, int KB>
+__global__ __launch_bounds__(FT_THREADS, 2)
+void fwd_mixed_kernel(FwdArgs a, FwdPlan pl, double oz, double dz, const float* r_peak)
 {
     typedef typename Vec8<T>::type V8;
-    constexpr bool F16 = sizeof(T) == 2 && !__is_same(T, __bf16);
+    constexpr bool F16 = !__is_same(T, __bf16);
     constexpr bool SPLIT = MODE != MODE_PLAIN;
+    constexpr int KX = 16 * KB;                 // stage B K (spots in x), padded
+    constexpr int NW = (SCH * KYP + FT_THREADS - 1) / FT_THREADS;   // W elements per thread
 
-    __shared__ __attribute__((aligned(16))) T Wh[SCH * WT_STRIDE];      // W^T hi [i][j]
-    __shared__ __attribute__((aligned(16))) T Wl[SPLIT ? SCH * WT_STRIDE : 8];
-    __shared__ __attribute__((aligned(16))) T Uh[MX_TY * U_STRIDE];     // c U hi [y][i]
-    __shared__ __attribute__((aligned(16))) T Ul[SPLIT ? MX_TY * U_STRIDE : 8];
-    __shared__ float uscale[4];
+    __shared__ __attribute__((aligned(16))) T Wh[2][SCH * WT_STRIDE];      // W^T hi [i][j]
+    __shared__ __attribute__((aligned(16))) T Wl[2][SPLIT ? SCH * WT_STRIDE : 8];
+    __shared__ __attribute__((aligned(16))) T Uh[2][FT_TY * U_STRIDE];     // c U hi [y][i]
+    __shared__ __attribute__((aligned(16))) T Ul[2][SPLIT ? FT_TY * U_STRIDE : 8];
+    __shared__ unsigned Tx[TAB ? 2 : 1][TAB ? FT_TAB : 1];
+    __shared__ unsigned Ty[TAB ? 2 : 1][TAB ? FT_TAB : 1];
+    __shared__ float uscale[2][2];
+    __shared__ float bmax[2][4];
+    __shared__ LayerList ll;
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    int b = blockIdx.x;
-    const int tx = b % a.tiles_x; b /= a.tiles_x;
-    const int ty = b % a.tiles_y; b /= a.tiles_y;
-    const int z = b;
-    const int x0 = tx * MX_TX, y0 = ty * MX_TY;
+    const int s = blockIdx.x % pl.split;
+    const int tile = blockIdx.x / pl.split;
+    const int tx = tile % a.tiles_x;
+    const int ty = (tile / a.tiles_x) % a.tiles_y;
+    const int z = tile / (a.tiles_x * a.tiles_y);
+    const int x0 = tx * FT_TX, y0 = ty * FT_TY;
     const float zc = (float)(oz + (double)z * dz);
 
-    const int yA = y0 + 16 * wave + (lane & 15);      // stage A row of this lane
-    const int gA = lane >> 4;                         // stage A k group (8 values)
-    const int xB = x0 + 32 * wave + (lane & 31);      // stage B column of this lane
-    const int gB = lane >> 5;                         // stage B k group (8 values)
-    const double yAc = a.oy + (double)yA * a.dy;
-    const double xBc = a.ox + (double)xB * a.dx;
-
-    v16f acc0 = {}, acc1 = {};
+    const int rbA = wave & 1;                       // stage A row block
+    const int bA = 16 * rbA + (lane & 15);          // stage A row in the tile
+    const int gA = lane >> 4;                       // stage A k group (8 values)
+    const int cbA0 = wave >> 1;                     // stage A column blocks: cbA0 and, w < 2, 2
+    const bool twoA = wave < 2;
+    const int cnB = 32 * wave;                      // stage B columns
+    const int aB = cnB + (lane & 31);
+    const int gB = lane >> 5;                       // stage B k group (8 values)
+    const double yAc = a.oy + (double)(y0 + bA) * a.dy;
+    const double xBc = a.ox + (double)(x0 + aB) * a.dx;
+    const int tA0 = 8 * pl.ty.Q * gA - pl.ty.P * bA + pl.ty.P * (FT_TY - 1);
+    const int tB0 = 8 * pl.tx.Q * gB - pl.tx.P * aB + pl.tx.P * (FT_TX - 1);
+    const int ntY = pl.ty.P * (FT_TY - 1) + pl.ty.Q * (KYP - 1) + 1;
+    const int ntX = pl.tx.P * (FT_TX - 1) + pl.tx.Q * (KX - 1) + 1;
 
     const int ncx = chunk_count(a.nsx), ncy = chunk_count(a.nsy);
-    for (int cy = 0; cy < ncy; ++cy) {
-        const int j0 = chunk_begin(a.nsy, ncy, cy), nj = chunk_begin(a.nsy, ncy, cy + 1) - j0;
-        const int nkA = (nj + 31) >> 5;               // K steps of 32
-        float dyA[2][8];
-#pragma unroll
-        for (int ks = 0; ks < 2; ++ks)
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int j = 32 * ks + 8 * gA + e;
-                const double d = yAc - (a.sy0 + (double)(j0 + j) * a.dsy);
-                dyA[ks][e] = (j < nj && yA < a.ny) ? (float)(d * d) : -1.0f;
-            }
-        for (int cx = 0; cx < ncx; ++cx) {
-            const int i0 = chunk_begin(a.nsx, ncx, cx), ni = chunk_begin(a.nsx, ncx, cx + 1) - i0;
-            const int nkB = (ni + 15) >> 4;           // K steps of 16
-            float dxB[3][8];
-#pragma unroll
-            for (int ks = 0; ks < 3; ++ks)
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const int i = 16 * ks + 8 * gB + e;
-                    const double d = xBc - (a.sx0 + (double)(i0 + i) * a.dsx);
-                    dxB[ks][e] = (i < ni) ? (float)(d * d) : -1.0f;
-                }
-            for (int l = 0; l < a.L; ++l) {
-                const double iddv = a.idd[(size_t)l * a.nz + z];
-                if (iddv == 0.0) continue;
-                const LayerScalars sc = layer_scalars(iddv, a.sigma[(size_t)l * a.nz + z]);
-                bool x3 = MODE == MODE_X3;
-                if (MODE == MODE_PEAK) x3 = fabsf(zc - r_peak[l]) <= 1.0f;
 
-                // W chunk -> LDS as W^T (hi, lo), zero padded
-                const float* wl = a.w + (size_t)l * a.nsy * a.nsx;
-                for (int e = threadIdx.x; e < SCH * 64; e += MX_THREADS) {
-                    const int i = e >> 6, j = e & 63;
-                    const float v = (i < ni && j < nj) ? wl[(size_t)(j0 + j) * a.nsx + i0 + i] : 0.0f;
-                    T h, lo;
-                    split<T>(v, h, lo);
-                    Wh[i * WT_STRIDE + j] = h;
-                    if (SPLIT) Wl[i * WT_STRIDE + j] = lo;
-                }
-                __syncthreads();
+    v16f acc = {};
+    float wreg[NW];
 
-                // stage A: 16 rows x 48 columns per wave
-                v4f u[3] = {};
+    for (int lb = s; lb < a.L; lb += FT_LB * pl.split) {
+        const int nact = collect_layers(a, lb, pl.split, z, zc, r_peak, ll);
+        if (nact == 0) continue;
+        for (int cy = 0; cy < ncy; ++cy) {
+            const int j0 = chunk_begin(a.nsy, ncy, cy), nj = chunk_begin(a.nsy, ncy, cy + 1) - j0;
+            const int nkA = TAB ? 2 : ((nj + 31) >> 5);
+            const double Cy = (a.oy + (double)y0 * a.dy) - (a.sy0 + (double)j0 * a.dsy);
+            for (int cx = 0; cx < ncx; ++cx) {
+                const int i0 = chunk_begin(a.nsx, ncx, cx), ni = chunk_begin(a.nsx, ncx, cx + 1) - i0;
+                const double Cx = (a.ox + (double)x0 * a.dx) - (a.sx0 + (double)i0 * a.dsx);
+
+                auto is_x3 = [&](int m) {
+                    if (MODE == MODE_X3) return true;
+                    if (MODE == MODE_PLAIN) return false;
+                    return ll.zpk[m] <= 1.0f;
+                };
+                auto load_w = [&](int m) {
+                    const float* wl = a.w + (size_t)ll.lay[m] * a.nsy * a.nsx + (size_t)j0 * a.nsx + i0;
 #pragma unroll
-                for (int ks = 0; ks < 2; ++ks) {
-                    if (ks < nkA) {
-                        V8 ah, al;
+                    for (int q = 0; q < NW; ++q) {
+                        const int e = threadIdx.x + q * FT_THREADS;
+                        const int i = e / KYP, j = e % KYP;
+                        wreg[q] = (e < SCH * KYP && i < ni && j < nj) ? wl[(size_t)j * a.nsx + i] : 0.0f;
+                    }
+                };
+                auto store_w = [&](int b) {
 #pragma unroll
-                        for (int e = 0; e < 8; ++e) {
-                            const float g = dyA[ks][e] >= 0.0f ? gauss2(dyA[ks][e], sc.k2) : 0.0f;
+                    for (int q = 0; q < NW; ++q) {
+                        const int e = threadIdx.x + q * FT_THREADS;
+                        if (e < SCH * KYP) {
                             T h, lo;
-                            split<T>(g, h, lo);
-                            ah[e] = h;
-                            al[e] = lo;
+                            split<T>(wreg[q], h, lo);
+                            const int off = (e / KYP) * WT_STRIDE + (e % KYP);
+                            Wh[b][off] = h;
+                            if (SPLIT) Wl[b][off] = lo;
                         }
+                    }
+                };
+                auto build_tx = [&](int m) {
+                    if (!TAB) return;
+                    const double k2d = ll.k2d[m];
+                    for (int t = threadIdx.x; t < ntX; t += FT_THREADS) {
+                        const double d = table_distance(pl.tx, Cx, FT_TX, t);
+                        Tx[m & 1][table_slot(pl.tx, t)] =
+                            pack_split<T>(__builtin_amdgcn_exp2f((float)(-(d * d) * k2d)));
+                    }
+                };
+                auto build_ty = [&](int m) {
+                    if (!TAB) return;
+                    const double k2d = ll.k2d[m];
+                    for (int t = threadIdx.x; t < ntY; t += FT_THREADS) {
+                        const double d = table_distance(pl.ty, Cy, FT_TY, t);
+                        Ty[m & 1][table_slot(pl.ty, t)] =
+                            pack_split<T>(__builtin_amdgcn_exp2f((float)(-(d * d) * k2d)));
+                    }
+                };
+                // stage A of layer m: U[m&1] = c Gy W, split into (hi, lo)
+                auto stage_a = [&](int m) {
+                    const int b = m & 1;
+                    const float c = ll.c[m], k2 = ll.k2[m];
+                    const bool x3 = is_x3(m);
+                    v4f u0 = {}, u1 = {};
 #pragma unroll
-                        for (int nb = 0; nb < 3; ++nb) {
-                            const int off = (16 * nb + (lane & 15)) * WT_STRIDE + 32 * ks + 8 * gA;
-                            const V8 bh = *(const V8*)&Wh[off];
-                            u[nb] = mfma16(ah, bh, u[nb]);
+                    for (int ks = 0; ks < 2; ++ks) {
+                        if (ks < nkA) {
+                            V8 ah, al;
+                            if (TAB) {
+                                unsigned t8[8];
+#pragma unroll
+                                for (int e = 0; e < 8; ++e)
+                                    t8[e] = Ty[b][table_slot(pl.ty, tA0 + pl.ty.Q * (32 * ks + e))];
+                                unpack8(t8, ah, al);
+                            } else {
+#pragma unroll
+                                for (int e = 0; e < 8; ++e) {
+                                    const int j = 32 * ks + 8 * gA + e;
+                                    const double d = yAc - (a.sy0 + (double)(j0 + j) * a.dsy);
+                                    const float g = (j < nj) ? gauss2((float)(d * d), k2) : 0.0f;
+                                    T h, lo;
+                                    split<T>(g, h, lo);
+                                    ah[e] = h;
+                                    al[e] = lo;
+                                }
+                            }
+                            const int kofs = 32 * ks + 8 * gA;
+                            const int off0 = (16 * cbA0 + (lane & 15)) * WT_STRIDE + kofs;
+                            const int off1 = (32 + (lane & 15)) * WT_STRIDE + kofs;
+                            const V8 b0h = *(const V8*)&Wh[b][off0];
+                            u0 = mfma16(ah, b0h, u0);
                             if (SPLIT && x3) {
-                                const V8 bl = *(const V8*)&Wl[off];
-                                u[nb] = mfma16(ah, bl, u[nb]);
-                                u[nb] = mfma16(al, bh, u[nb]);
+                                u0 = mfma16(ah, *(const V8*)&Wl[b][off0], u0);
+                                u0 = mfma16(al, b0h, u0);
+                            }
+                            if (twoA) {
+                                const V8 b1h = *(const V8*)&Wh[b][off1];
+                                u1 = mfma16(ah, b1h, u1);
+                                if (SPLIT && x3) {
+                                    u1 = mfma16(ah, *(const V8*)&Wl[b][off1], u1);
+                                    u1 = mfma16(al, b1h, u1);
+                                }
                             }
                         }
                     }
-                }
-                // c U -> LDS (hi, lo); FP16: scale the wave's rows into [1, 2)
-                float inv_scale = 1.0f, scale = 1.0f;
-                if (F16) {
-                    float m = 0.0f;
-#pragma unroll
-                    for (int nb = 0; nb < 3; ++nb)
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) m = fmaxf(m, fabsf(sc.c * u[nb][r]));
-                    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-                    if (m > 0.0f && m < 3.0e38f) {
-                        const int ex = ilogbf(m);            // 2^ex <= m < 2^(ex+1)
-                        scale = ldexpf(1.0f, ex);
-                        inv_scale = ldexpf(1.0f, -ex);
-                    }
-                    if (lane == 0) uscale[wave] = scale;
-                }
-                {
-                    const int rbase = 16 * wave + 4 * gA;
-#pragma unroll
-                    for (int nb = 0; nb < 3; ++nb)
+                    float inv_scale = 1.0f;
+                    if (F16) {
+                        // row block rbA is shared by waves rbA and rbA + 2: combine maxima in LDS
+                        float mx = 0.0f;
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
-                            T h, lo;
-                            split<T>(sc.c * u[nb][r] * inv_scale, h, lo);
-                            const int off = (rbase + r) * U_STRIDE + 16 * nb + (lane & 15);
-                            Uh[off] = h;
-                            if (SPLIT) Ul[off] = lo;
+                            mx = fmaxf(mx, fabsf(c * u0[r]));
+                            if (twoA) mx = fmaxf(mx, fabsf(c * u1[r]));
                         }
-                }
-                __syncthreads();
-
-                // stage B: 64 rows x 32 columns per wave
-                v16f p0 = {}, p1 = {};
-#pragma unroll
-                for (int ks = 0; ks < 3; ++ks) {
-                    if (ks < nkB) {
-                        V8 bh, bl;
-#pragma unroll
-                        for (int e = 0; e < 8; ++e) {
-                            const float g = dxB[ks][e] >= 0.0f ? gauss2(dxB[ks][e], sc.k2) : 0.0f;
-                            T h, lo;
-                            split<T>(g, h, lo);
-                            bh[e] = h;
-                            bl[e] = lo;
+                        for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+                        if (lane == 0) bmax[b][wave] = mx;
+                        __syncthreads();
+                        mx = fmaxf(bmax[b][rbA], bmax[b][rbA + 2]);
+                        float scale = 1.0f;
+                        if (mx > 0.0f && mx < 3.0e38f) {
+                            const int ex = ilogbf(mx);            // 2^ex <= mx < 2^(ex+1)
+                            scale = ldexpf(1.0f, ex);
+                            inv_scale = ldexpf(1.0f, -ex);
                         }
-                        const int k = 16 * ks + 8 * gB;
-                        const int off0 = (lane & 31) * U_STRIDE + k;
-                        const int off1 = (32 + (lane & 31)) * U_STRIDE + k;
-                        const V8 a0h = *(const V8*)&Uh[off0];
-                        const V8 a1h = *(const V8*)&Uh[off1];
-                        p0 = mfma32(a0h, bh, p0);
-                        p1 = mfma32(a1h, bh, p1);
+                        if (lane == 0 && wave < 2) uscale[b][wave] = scale;
+                    }
+                    const int rbase = 16 * rbA + 4 * gA;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = (rbase + r) * U_STRIDE + (lane & 15);
+                        T h, lo;
+                        split<T>(c * u0[r] * inv_scale, h, lo);
+                        Uh[b][row + 16 * cbA0] = h;
+                        if (SPLIT) Ul[b][row + 16 * cbA0] = lo;
+                        if (twoA) {
+                            split<T>(c * u1[r] * inv_scale, h, lo);
+                            Uh[b][row + 32] = h;
+                            if (SPLIT) Ul[b][row + 32] = lo;
+                        }
+                    }
+                };
+                // stage B of layer m: acc += (c U) Gx^T
+                auto stage_b = [&](int m) {
+                    const int b = m & 1;
+                    const bool x3 = is_x3(m);
+                    const float k2 = ll.k2[m];
+                    V8 bh[KB], bl[KB];
+#pragma unroll
+                    for (int ks = 0; ks < KB; ++ks) {
+                        if (TAB) {
+                            unsigned t8[8];
+#pragma unroll
+                            for (int e = 0; e < 8; ++e)
+                                t8[e] = Tx[b][table_slot(pl.tx, tB0 + pl.tx.Q * (16 * ks + e))];
+                            unpack8(t8, bh[ks], bl[ks]);
+                        } else {
+#pragma unroll
+                            for (int e = 0; e < 8; ++e) {
+                                const int i = 16 * ks + 8 * gB + e;
+                                const double d = xBc - (a.sx0 + (double)(i0 + i) * a.dsx);
+                                const float g = (i < ni) ? gauss2((float)(d * d), k2) : 0.0f;
+                                T h, lo;
+                                split<T>(g, h, lo);
+                                bh[ks][e] = h;
+                                bl[ks][e] = lo;
+                            }
+                        }
+                    }
+                    v16f p = {};
+#pragma unroll
+                    for (int ks = 0; ks < KB; ++ks) {
+                        const int off = (lane & 31) * U_STRIDE + 16 * ks + 8 * gB;
+                        const V8 ah = *(const V8*)&Uh[b][off];
+                        p = mfma32(ah, bh[ks], p);
                         if (SPLIT && x3) {
-                            const V8 a0l = *(const V8*)&Ul[off0];
-                            const V8 a1l = *(const V8*)&Ul[off1];
-                            p0 = mfma32(a0h, bl, p0);
-                            p0 = mfma32(a0l, bh, p0);
-                            p1 = mfma32(a1h, bl, p1);
-                            p1 = mfma32(a1l, bh, p1);
+                            p = mfma32(ah, bl[ks], p);
+                            p = mfma32(*(const V8*)&Ul[b][off], bh[ks], p);
                         }
                     }
-                }
-                // the layer's partial joins the accumulator once (short rounding chain);
-                // FP16: rows 0-15 / 16-31 of p0 came from stage-A waves 0 / 1, of p1 from 2 / 3
-                if (F16) {
-                    const float s0 = uscale[0], s1 = uscale[1], s2 = uscale[2], s3 = uscale[3];
+                    if (F16) {
+                        // rows 0-15 / 16-31 came from stage-A row blocks 0 / 1
+                        const float s0 = uscale[b][0], s1 = uscale[b][1];
 #pragma unroll
-                    for (int r = 0; r < 16; ++r) {
-                        acc0[r] += (r < 8 ? s0 : s1) * p0[r];
-                        acc1[r] += (r < 8 ? s2 : s3) * p1[r];
+                        for (int r = 0; r < 16; ++r) acc[r] += (r < 8 ? s0 : s1) * p[r];
+                    } else {
+                        acc += p;
                     }
-                } else {
-                    acc0 += p0;
-                    acc1 += p1;
+                };
+
+                load_w(0);
+                store_w(0);
+                build_ty(0);
+                if (nact > 1) load_w(1);
+                __syncthreads();
+                stage_a(0);
+                build_tx(0);
+                if (nact > 1) {
+                    store_w(1);
+                    build_ty(1);
                 }
                 __syncthreads();
+                for (int m = 0; m < nact; ++m) {
+                    if (m + 2 < nact) load_w(m + 2);
+                    stage_b(m);
+                    if (m + 1 < nact) {
+                        stage_a(m + 1);
+                        build_tx(m + 1);
+                    }
+                    if (m + 2 < nact) {
+                        store_w(m & 1);
+                        build_ty(m + 2);
+                    }
+                    __syncthreads();
+                }
             }
         }
     }
+    store_block(a, pl, tile, s, z, y0, x0, 0, cnB, acc);
+}
 
-    if (xB < a.nx) {
-        float* base = a.dose + (size_t)z * a.ny * a.nx + xB;
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int row = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            const int y = y0 + row;
-            if (y < a.ny) base[(size_t)y * a.nx] = acc0[r];
-            if (y + 32 < a.ny) base[(size_t)(y + 32) * a.nx] = acc1[r];
-        }
+template <typename T, int MODE>
+static int launch_mixed(const FwdArgs& a, const FwdPlan& pl, int kb, double oz, double dz,
+                        const float* rp, hipStream_t s)
+{
+    const dim3 grid((unsigned)((long long)a.tiles_x * a.tiles_y * a.nz * pl.split)), block(FT_THREADS);
+    if (!pl.table) {
+        hipLaunchKernelGGL((fwd_mixed_kernel<T, MODE, false, 3>), grid, block, 0, s, a, pl, oz, dz, rp);
+        return PBH_OK;
     }
+    switch (kb) {
+    case 1: hipLaunchKernelGGL((fwd_mixed_kernel<T, MODE, true, 1>), grid, block, 0, s, a, pl, oz, dz, rp); break;
+    case 2: hipLaunchKernelGGL((fwd_mixed_kernel<T, MODE, true, 2>), grid, block, 0, s, a, pl, oz, dz, rp); break;
+    case 3: hipLaunchKernelGGL((fwd_mixed_kernel<T, MODE, true, 3>), grid, block, 0, s, a, pl, oz, dz, rp); break;
+    default: return PBH_ERR_UNSUPPORTED;
+    }
+    return PBH_OK;
 }
 
 }  // namespace pbh
 
 using namespace pbh;
 
-size_t pbh_forward_mixed_ws(const PbhGrid*, const PbhLattice* lat, int precision)
-{
-    if (precision == PBH_PREC_BF16X3_PEAK || precision == PBH_PREC_FP16X3_PEAK)
-        return (size_t)lat->n_layers * sizeof(float);
-    return 0;
-}
+float* pbh_forward_ws_peaks(const FwdArgs& a, void* ws);
 
-int pbh_forward_mixed(const FwdArgs& a, double oz, double dz, int precision, void* ws,
+int pbh_forward_mixed(const FwdArgs& a, double oz, double dz, int precision, int split, void* ws,
                       size_t ws_bytes, hipStream_t s)
 {
-    const long long blocks = (long long)a.tiles_x * a.tiles_y * a.nz;
-    if (blocks > 0x7fffffffLL) return PBH_ERR_UNSUPPORTED;
-    const dim3 grid((unsigned)blocks), block(MX_THREADS);
+    const int kb = (chunk_pad(a.nsx, 1) + 15) / 16;   // stage B K steps of 16 spots
+    FwdPlan pl;
+    int rc = pbh_plan_forward(a, 16 * kb, KYP, ws, ws_bytes, split, &pl);
+    if (rc != PBH_OK) return rc;
     float* r_peak = nullptr;
     if (precision == PBH_PREC_BF16X3_PEAK || precision == PBH_PREC_FP16X3_PEAK) {
         if (!ws) return PBH_ERR_NULL_ARG;
-        if (ws_bytes < (size_t)a.L * sizeof(float)) return PBH_ERR_WORKSPACE;
-        r_peak = (float*)ws;
+        if (ws_bytes < pbh_forward_ws_bytes(a, 1)) return PBH_ERR_WORKSPACE;
+        r_peak = pbh_forward_ws_peaks(a, ws);
         hipLaunchKernelGGL(peak_depth_f32_kernel, dim3(cdiv(a.L, 64)), dim3(64), 0, s,
                            a.idd, a.L, a.nz, oz, dz, r_peak);
-        int rc = launch_rc("peak_depth_f32_kernel");
-        if (rc != PBH_OK) return rc;
+        if ((rc = launch_rc("peak_depth_f32_kernel")) != PBH_OK) return rc;
     }
     switch (precision) {
-    case PBH_PREC_BF16:
-        hipLaunchKernelGGL((fwd_mixed_kernel<__bf16, MODE_PLAIN>), grid, block, 0, s, a, oz, dz, r_peak); break;
-    case PBH_PREC_FP16:
-        hipLaunchKernelGGL((fwd_mixed_kernel<_Float16, MODE_PLAIN>), grid, block, 0, s, a, oz, dz, r_peak); break;
-    case PBH_PREC_BF16X3:
-        hipLaunchKernelGGL((fwd_mixed_kernel<__bf16, MODE_X3>), grid, block, 0, s, a, oz, dz, r_peak); break;
-    case PBH_PREC_FP16X3:
-        hipLaunchKernelGGL((fwd_mixed_kernel<_Float16, MODE_X3>), grid, block, 0, s, a, oz, dz, r_peak); break;
-    case PBH_PREC_BF16X3_PEAK:
-        hipLaunchKernelGGL((fwd_mixed_kernel<__bf16, MODE_PEAK>), grid, block, 0, s, a, oz, dz, r_peak); break;
-    case PBH_PREC_FP16X3_PEAK:
-        hipLaunchKernelGGL((fwd_mixed_kernel<_Float16, MODE_PEAK>), grid, block, 0, s, a, oz, dz, r_peak); break;
-    default:
-        return PBH_ERR_UNSUPPORTED;
+    case PBH_PREC_BF16: rc = launch_mixed<__bf16, MODE_PLAIN>(a, pl, kb, oz, dz, r_peak, s); break;
+    case PBH_PREC_FP16: rc = launch_mixed<_Float16, MODE_PLAIN>(a, pl, kb, oz, dz, r_peak, s); break;
+    case PBH_PREC_BF16X3: rc = launch_mixed<__bf16, MODE_X3>(a, pl, kb, oz, dz, r_peak, s); break;
+    case PBH_PREC_FP16X3: rc = launch_mixed<_Float16, MODE_X3>(a, pl, kb, oz, dz, r_peak, s); break;
+    case PBH_PREC_BF16X3_PEAK: rc = launch_mixed<__bf16, MODE_PEAK>(a, pl, kb, oz, dz, r_peak, s); break;
+    case PBH_PREC_FP16X3_PEAK: rc = launch_mixed<_Float16, MODE_PEAK>(a, pl, kb, oz, dz, r_peak, s); break;
+    default: return PBH_ERR_UNSUPPORTED;
     }
+    if (rc != PBH_OK) return rc;
     return launch_rc("fwd_mixed_kernel");
 }

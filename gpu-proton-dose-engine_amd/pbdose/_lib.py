@@ -56,6 +56,7 @@ SIGNATURES = {
                                 _P, C.c_size_t, _P]),
     "pbh_forward_workspace_bytes": (C.c_size_t, [_G, _L, C.c_int]),
     "pbh_forward": (C.c_int, [_G, _L, _P, _P, _P, _P, C.c_int, _P, C.c_size_t, _P]),
+    "pbh_forward_tuned": (C.c_int, [_G, _L, _P, _P, _P, _P, C.c_int, C.c_int, _P, C.c_size_t, _P]),
     "pbh_adjoint_workspace_bytes": (C.c_size_t, [_G, _L, C.c_int]),
     "pbh_adjoint": (C.c_int, [_G, _L, _P, _P, _P, _P, _P, _P, _P, C.c_int, _P, C.c_size_t, _P]),
     "pbh_objective_workspace_bytes": (C.c_size_t, [_G]),
@@ -114,8 +115,8 @@ def ptr(t) -> C.c_void_p:
 
 
 def workspace(nbytes: int, dev: torch.device) -> torch.Tensor:
-    """Caller-owned scratch for the C ABI (the library never allocates)."""
-    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=dev)
+    """Caller-owned, zero-filled scratch for the C ABI (the library never allocates)."""
+    return torch.zeros(max(int(nbytes), 1), dtype=torch.uint8, device=dev)
 
 
 def grid_struct(g) -> PbhGrid:

@@ -58,8 +58,9 @@ class DeviceProblem:
         return self._ws[key]
 
     def forward(self, precision: str = "fp32", out: Optional[torch.Tensor] = None,
-                w: Optional[torch.Tensor] = None) -> torch.Tensor:
-        """Dose (nz, ny, nx) float32, asynchronous on the current stream."""
+                w: Optional[torch.Tensor] = None, split: int = 0) -> torch.Tensor:
+        """Dose (nz, ny, nx) float32, asynchronous on the current stream.
+        ``split``: workgroups per dose tile sharing its layers (0 = automatic)."""
         p = _lib.PREC[precision]
         if out is None:
             out = torch.empty(self.shape, dtype=torch.float32, device=self.dev)
@@ -69,9 +70,11 @@ class DeviceProblem:
         if wt.dtype != torch.float32 or tuple(wt.shape) != tuple(self.w.shape) or not wt.is_contiguous():
             raise ValueError("weights must be a contiguous float32 (L, nsy, nsx) tensor")
         ws = self.workspace("fwd", p)
-        _lib.check(_lib.lib().pbh_forward(self.grid_s, self.lat_s, _lib.ptr(wt), _lib.ptr(self.idd),
-                                          _lib.ptr(self.sigma), _lib.ptr(out), p, _lib.ptr(ws),
-                                          ws.numel(), _lib.stream_of(self.dev)), "pbh_forward")
+        _lib.check(_lib.lib().pbh_forward_tuned(self.grid_s, self.lat_s, _lib.ptr(wt),
+                                                _lib.ptr(self.idd), _lib.ptr(self.sigma),
+                                                _lib.ptr(out), p, int(split), _lib.ptr(ws),
+                                                ws.numel(), _lib.stream_of(self.dev)),
+                   "pbh_forward_tuned")
         return out
 
 

@@ -144,6 +144,17 @@ int pbh_forward(const PbhGrid* g, const PbhLattice* lat,
                 float* dose, int precision,
                 void* workspace, size_t workspace_bytes, void* stream);
 
+/* pbh_forward with an explicit layer split: `split` workgroups share the
+ * layers of each dose tile (1, 2, 4 or 8; 0 = automatic). Results do not depend
+ * on the split beyond FP32 summation order. The workspace of a forward call
+ * (pbh_forward_workspace_bytes, valid for every split) must be zero-filled
+ * before its first use; calls leave it ready for the next call. Calls that
+ * share one workspace must not overlap. */
+int pbh_forward_tuned(const PbhGrid* g, const PbhLattice* lat,
+                      const float* w, const double* idd, const double* sigma,
+                      float* dose, int precision, int split,
+                      void* workspace, size_t workspace_bytes, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Adjoint of the forward map (inverse planning).
  * ------------------------------------------------------------------------- */

@@ -129,3 +129,36 @@ def test_mixed_ragged_and_chunked(precision):
     print(f"case mixed {precision} ragged 70x37x5, lattice 60x50", flush=True)
     p = make_problem(70, 37, 5, 0.25, 60, 50, 0.3, [90.0, 150.0], seed=9, x0=0.2)
     check(p, tol=MIXED_TOL[precision], precision=precision)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "fp16x3"])
+def test_layer_split_parity_and_determinism(precision):
+    from pbdose import engines
+    print(f"case layer split sweep {precision}", flush=True)
+    p = make_problem(128, 96, 12, 0.2, 41, 33, 0.5, np.linspace(80, 170, 16), seed=21)
+    dp = engines.DeviceProblem(p)
+    ref = oracle_dose(p)
+    tol = TOL_FP32 if precision == "fp32" else MIXED_TOL[precision]
+    for split in (1, 2, 4, 8):
+        a = dp.forward(precision, split=split)
+        b = dp.forward(precision, split=split)
+        assert torch.equal(a, b), f"split {split} not deterministic"
+        r = O.relative_error(a.double().cpu().numpy(), ref)["max_rel_err_global"]
+        print(f"  split {split}: {r:.3e}", flush=True)
+        assert r <= tol
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "fp16x3", "bf16"])
+def test_direct_factor_path(precision):
+    # spot pitch 0.37 cm against 0.2 cm voxels has no small integer ratio:
+    # every Gaussian factor is evaluated directly instead of from a table
+    print(f"case direct factors {precision}", flush=True)
+    p = make_problem(100, 90, 8, 0.2, 23, 29, 0.37, [95.0, 125.0, 165.0], seed=13)
+    check(p, tol=TOL_FP32 if precision == "fp32" else MIXED_TOL[precision], precision=precision)
+
+
+def test_quarter_pitch_grid():
+    # 2.5 mm voxels, 5 mm spots: table with P = 1, Q = 2
+    print("case 2.5 mm voxels / 5 mm spots", flush=True)
+    p = make_problem(96, 96, 10, 0.25, 31, 31, 0.5, np.linspace(90, 160, 8), seed=17)
+    check(p)
