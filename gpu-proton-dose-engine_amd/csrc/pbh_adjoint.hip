@@ -1,0 +1,405 @@
+// pbh_adjoint.hip -- adjoint of the forward dose map, FP32 MFMA (gfx950), plus the
+// weighted-quadratic objective and the range-shift chain rule.
+//
+// Forward: D_z = sum_l c_lz Gy_lz W_l Gx_lz^T, c_lz = IDD_l(z) / (2 pi s^2),
+// G[k][i] = exp(-(v_k - s_i)^2 / (2 s^2)), s = max(sigma_l(z), 1e-9).
+// For a dose cotangent R (nz, ny, nx) and P_lz = Gy_lz^T R_z Gx_lz (nsy x nsx):
+//   dJ/dW_l       = sum_z c_lz P_lz
+//   dJ/dIDD_l(z)  = <W_l, P_lz> / (2 pi s^2)
+//   dJ/dsigma_l(z)= c_lz [ -(2/s) <W_l, P_lz> + <W_l, Gy'^T R Gx + Gy^T R Gx'> ]  (sigma > 1e-9)
+//   with G' = dG/ds = G (v - s_i)^2 / s^3.
+//
+// Kernel 1 (one workgroup per depth slice and share of layers): the slice of R is
+// staged in LDS tile by tile (64 x 128), then per layer
+//   Q  = R Gx, Q' = R Gx'     (M = 64 rows, N = spots, K = 128 columns; v_mfma_f32_16x16x4f32)
+//   P += Gy^T Q, P2 += Gy'^T Q + Gy^T Q'   (M = N = spots, K = 64 rows)
+// with the Gaussians gathered from per-(layer, depth) difference tables as in the
+// forward kernels (direct evaluation on grids without a small pitch ratio). The
+// per-slice products c P go to a workspace; <W, P> and <W, P2> are reduced in
+// the workgroup in a fixed order.
+// Kernel 2 sums the per-slice partials over z in FP64 in increasing z: the weight
+// gradient is deterministic (no atomics).
+#include "pbh_fwd_core.h"
+
+namespace pbh {
+
+constexpr int AJ_TY = 64, AJ_TX = 128, AJ_THREADS = 256;
+constexpr int AJ_SC = PBH_SPOT_CHUNK;            // 48
+constexpr int AJ_RS = AJ_TX + 2;                 // R tile row stride: lanes (c, g) hit banks 2c + g
+constexpr int AJ_QS = AJ_SC + 1;                 // Q row stride
+
+struct AdjArgs {
+    FwdArgs f;
+    const float* R;        // (nz, ny, nx)
+    double* part;          // [nz][L][nsy][nsx] per-slice c P (FP64 copy of FP32 sums)
+    double* grad_idd;      // (L, nz) or null
+    double* grad_sigma;    // (L, nz) or null
+    int split;
+    int tab;
+    AxisTable tx, ty;
+};
+
+// Slot of table entry t for axes of AJ_TX / AJ_TY voxels (same layout rules as the forward).
+__device__ __forceinline__ float tab_get(const float* T, const AxisTable& g, int t) { return T[table_slot(g, t)]; }
+
+template <bool TAB, bool SIG>
+__global__ __launch_bounds__(AJ_THREADS, 2)
+void adj_fp32_kernel(AdjArgs A)
+{
+    const FwdArgs& a = A.f;
+    __shared__ float Rs[AJ_TY * AJ_RS];
+    __shared__ float Qs[AJ_TY * AJ_QS];
+    __shared__ float Q2s[SIG ? AJ_TY * AJ_QS : 1];
+    __shared__ float Tx[TAB ? FT_TAB : 1], Tx2[(TAB && SIG) ? FT_TAB : 1];
+    __shared__ float Ty[TAB ? FT_TAB : 1], Ty2[(TAB && SIG) ? FT_TAB : 1];
+    __shared__ double red[2][AJ_THREADS / 64];
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int s = blockIdx.x % A.split;
+    const int z = blockIdx.x / A.split;
+    const int g = lane >> 4, c16 = lane & 15;
+    const int tiles_x = cdiv(a.nx, AJ_TX), tiles_y = cdiv(a.ny, AJ_TY);
+    const int ncx = chunk_count(a.nsx), ncy = chunk_count(a.nsy);
+    const float* Rz = A.R + (size_t)z * a.ny * a.nx;
+
+    for (int l = s; l < a.L; l += A.split) {
+        const double iddv = a.idd[(size_t)l * a.nz + z];
+        const double sg = a.sigma[(size_t)l * a.nz + z];
+        const double sd = sg > kSigmaFloor ? sg : kSigmaFloor;
+        const double k2d = kLog2e / (2.0 * sd * sd);
+        const float k2 = (float)k2d;
+        const float is3 = (float)(1.0 / (sd * sd * sd));        // dG/ds = G d^2 / s^3
+        const double norm = 1.0 / (2.0 * kPi * sd * sd);
+        double wp = 0.0, wp2 = 0.0;                               // <W, P>, <W, P2> (lane partials)
+
+        for (int cy = 0; cy < ncy; ++cy) {
+            const int j0 = chunk_begin(a.nsy, ncy, cy), nj = chunk_begin(a.nsy, ncy, cy + 1) - j0;
+            for (int cx = 0; cx < ncx; ++cx) {
+                const int i0 = chunk_begin(a.nsx, ncx, cx), ni = chunk_begin(a.nsx, ncx, cx + 1) - i0;
+                // P blocks (3 x 3 of 16 x 16 over j, i): wave w owns blocks w, w + 4, w + 8
+                v4f P[3] = {}, P2[3] = {};
+                for (int tyi = 0; tyi < tiles_y; ++tyi) {
+                    for (int txi = 0; txi < tiles_x; ++txi) {
+                        const int y0 = tyi * AJ_TY, x0 = txi * AJ_TX;
+                        __syncthreads();   // previous tile's readers are done
+                        for (int e = threadIdx.x; e < AJ_TY * AJ_TX; e += AJ_THREADS) {
+                            const int yy = e / AJ_TX, xx = e % AJ_TX;
+                            const int y = y0 + yy, x = x0 + xx;
+                            Rs[yy * AJ_RS + xx] = (y < a.ny && x < a.nx) ? Rz[(size_t)y * a.nx + x] : 0.0f;
+                        }
+                        const double Cx = (a.ox + (double)x0 * a.dx) - (a.sx0 + (double)i0 * a.dsx);
+                        const double Cy = (a.oy + (double)y0 * a.dy) - (a.sy0 + (double)j0 * a.dsy);
+                        if (TAB) {
+                            const int ntX = A.tx.P * (AJ_TX - 1) + A.tx.Q * (AJ_SC - 1) + 1;
+                            const int ntY = A.ty.P * (AJ_TY - 1) + A.ty.Q * (AJ_SC - 1) + 1;
+                            for (int t = threadIdx.x; t < ntX; t += AJ_THREADS) {
+                                const double d = table_distance(A.tx, Cx, AJ_TX, t);
+                                const float gv = __builtin_amdgcn_exp2f((float)(-(d * d) * k2d));
+                                Tx[table_slot(A.tx, t)] = gv;
+                                if (SIG) Tx2[table_slot(A.tx, t)] = gv * (float)(d * d) * is3;
+                            }
+                            for (int t = threadIdx.x; t < ntY; t += AJ_THREADS) {
+                                const double d = table_distance(A.ty, Cy, AJ_TY, t);
+                                const float gv = __builtin_amdgcn_exp2f((float)(-(d * d) * k2d));
+                                Ty[table_slot(A.ty, t)] = gv;
+                                if (SIG) Ty2[table_slot(A.ty, t)] = gv * (float)(d * d) * is3;
+                            }
+                        }
+                        __syncthreads();
+
+                        // Gaussian of voxel column xx (tile) and spot i (chunk): table or direct
+                        auto gx = [&](int xx, int i, float* dg) -> float {
+                            if (TAB) {
+                                const int t = A.tx.Q * i - A.tx.P * xx + A.tx.P * (AJ_TX - 1);
+                                if (SIG) *dg = tab_get(Tx2, A.tx, t);
+                                return tab_get(Tx, A.tx, t);
+                            }
+                            const double d = (a.ox + (double)(x0 + xx) * a.dx) - (a.sx0 + (double)(i0 + i) * a.dsx);
+                            const float d2 = (float)(d * d);
+                            const float gv = i < ni ? gauss2(d2, k2) : 0.0f;
+                            if (SIG) *dg = gv * d2 * is3;
+                            return gv;
+                        };
+                        auto gy = [&](int yy, int j, float* dg) -> float {
+                            if (TAB) {
+                                const int t = A.ty.Q * j - A.ty.P * yy + A.ty.P * (AJ_TY - 1);
+                                if (SIG) *dg = tab_get(Ty2, A.ty, t);
+                                return tab_get(Ty, A.ty, t);
+                            }
+                            const double d = (a.oy + (double)(y0 + yy) * a.dy) - (a.sy0 + (double)(j0 + j) * a.dsy);
+                            const float d2 = (float)(d * d);
+                            const float gv = j < nj ? gauss2(d2, k2) : 0.0f;
+                            if (SIG) *dg = gv * d2 * is3;
+                            return gv;
+                        };
+
+                        // stage 1: Q[y][i] = sum_x R[y][x] Gx[x][i]; wave w owns rows 16w..16w+15
+                        {
+                            v4f q[3] = {}, q2[3] = {};
+#pragma unroll 4
+                            for (int kk = 0; kk < AJ_TX / 4; ++kk) {
+                                const int xx = 4 * kk + g;
+                                const float av = Rs[(16 * wave + c16) * AJ_RS + xx];
+#pragma unroll
+                                for (int nb = 0; nb < 3; ++nb) {
+                                    float dg = 0.0f;
+                                    const float bv = gx(xx, 16 * nb + c16, &dg);
+                                    q[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, q[nb], 0, 0, 0);
+                                    if (SIG) q2[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, dg, q2[nb], 0, 0, 0);
+                                }
+                            }
+#pragma unroll
+                            for (int nb = 0; nb < 3; ++nb)
+#pragma unroll
+                                for (int r = 0; r < 4; ++r) {
+                                    const int off = (16 * wave + 4 * g + r) * AJ_QS + 16 * nb + c16;
+                                    Qs[off] = q[nb][r];
+                                    if (SIG) Q2s[off] = q2[nb][r];
+                                }
+                        }
+                        __syncthreads();
+
+                        // stage 2: P[j][i] += sum_y Gy[y][j] Q[y][i] (A = Gy^T, B = Q)
+#pragma unroll
+                        for (int q = 0; q < 3; ++q) {
+                            const int blk = wave + 4 * q;
+                            if (blk < 9) {
+                                const int mb = blk / 3, nb = blk % 3;
+                                v4f p = P[q], p2 = P2[q];
+#pragma unroll 4
+                                for (int kk = 0; kk < AJ_TY / 4; ++kk) {
+                                    const int yy = 4 * kk + g;
+                                    float dgy = 0.0f;
+                                    const float av = gy(yy, 16 * mb + c16, &dgy);
+                                    const float bv = Qs[yy * AJ_QS + 16 * nb + c16];
+                                    p = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, p, 0, 0, 0);
+                                    if (SIG) {
+                                        p2 = __builtin_amdgcn_mfma_f32_16x16x4f32(dgy, bv, p2, 0, 0, 0);
+                                        p2 = __builtin_amdgcn_mfma_f32_16x16x4f32(av, Q2s[yy * AJ_QS + 16 * nb + c16], p2, 0, 0, 0);
+                                    }
+                                }
+                                P[q] = p;
+                                P2[q] = p2;
+                            }
+                        }
+                    }
+                }
+                // epilogue for (layer, chunk): partial c P, and <W, P>, <W, P2>
+                const float* wl = a.w + (size_t)l * a.nsy * a.nsx;
+                double* pz = A.part + ((size_t)z * a.L + l) * a.nsy * a.nsx;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const int blk = wave + 4 * q;
+                    if (blk < 9) {
+                        const int mb = blk / 3, nb = blk % 3;
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int j = 16 * mb + 4 * g + r, i = 16 * nb + c16;
+                            if (j < nj && i < ni) {
+                                const size_t off = (size_t)(j0 + j) * a.nsx + i0 + i;
+                                const double wv = (double)wl[off];
+                                pz[off] = iddv * norm * (double)P[q][r];
+                                wp += wv * (double)P[q][r];
+                                if (SIG) wp2 += wv * (double)P2[q][r];
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        // workgroup sums of <W, P>, <W, P2> in a fixed order
+        if (A.grad_idd || (SIG && A.grad_sigma)) {
+            for (int o = 32; o > 0; o >>= 1) {
+                wp += __shfl_xor(wp, o, 64);
+                wp2 += __shfl_xor(wp2, o, 64);
+            }
+            __syncthreads();
+            if (lane == 0) {
+                red[0][wave] = wp;
+                red[1][wave] = wp2;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double t0 = 0.0, t1 = 0.0;
+                for (int w = 0; w < AJ_THREADS / 64; ++w) {
+                    t0 += red[0][w];
+                    t1 += red[1][w];
+                }
+                if (A.grad_idd) A.grad_idd[(size_t)l * a.nz + z] = t0 * norm;
+                if (SIG && A.grad_sigma)
+                    A.grad_sigma[(size_t)l * a.nz + z] =
+                        sg > kSigmaFloor ? iddv * norm * (-(2.0 / sd) * t0 + t1) : 0.0;
+            }
+        }
+    }
+}
+
+// grad_w[e] = sum_z part[z][e] (increasing z), e over L * nsy * nsx
+__global__ void adj_reduce_kernel(const double* part, int nz, size_t n, float* grad_w)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    double acc = 0.0;
+    for (int z = 0; z < nz; ++z) acc += part[(size_t)z * n + e];
+    grad_w[e] = (float)acc;
+}
+
+// ---- weighted quadratic objective: R = 2 wv (D - Dp), J = sum wv (D - Dp)^2 ----
+constexpr int OBJ_THREADS = 256;
+
+__global__ __launch_bounds__(OBJ_THREADS) void objective_kernel(const float* D, const float* Dp, const float* wv,
+                                                                size_t n, float* R, double* partial)
+{
+    __shared__ double red[OBJ_THREADS / 64];
+    double acc = 0.0;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const double r = (double)D[e] - (double)Dp[e];
+        const double w = wv ? (double)wv[e] : 1.0;
+        acc += w * r * r;
+        if (R) R[e] = (float)(2.0 * w * r);
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < OBJ_THREADS / 64; ++w) t += red[w];
+        partial[blockIdx.x] = t;
+    }
+}
+
+__global__ void sum_partials_kernel(const double* partial, int n, double* out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < n; ++i) t += partial[i];
+        out[0] = t;
+    }
+}
+
+// out[l] = sum_z (g_idd[l,z] didd[l,z] + g_sig[l,z] dsig[l,z]) (one wave per layer, fixed order)
+__global__ void lut_chain_kernel(const double* g_idd, const double* g_sig, const double* didd,
+                                 const double* dsig, int L, int nz, double* out)
+{
+    const int l = blockIdx.x;
+    const int lane = threadIdx.x;
+    double acc = 0.0;
+    for (int k = lane; k < nz; k += 64) {
+        const size_t e = (size_t)l * nz + k;
+        acc += g_idd[e] * didd[e] + g_sig[e] * dsig[e];
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (lane == 0) out[l] = acc;
+}
+
+}  // namespace pbh
+
+using namespace pbh;
+
+FwdArgs make_fwd_args(const PbhGrid* g, const PbhLattice* lat, const float* w,
+                      const double* idd, const double* sigma, float* dose);
+extern "C" int pbh_validate(const PbhGrid* g, const PbhLattice* lat);
+
+static int adj_split(const PbhGrid* g, const PbhLattice* lat)
+{
+    int s = 1;
+    while (s < 16 && (long long)g->nz * s < 512 && s * 2 <= lat->n_layers) s *= 2;
+    return s;
+}
+
+extern "C" size_t pbh_adjoint_workspace_bytes(const PbhGrid* g, const PbhLattice* lat, int precision)
+{
+    (void)precision;
+    if (pbh_validate(g, lat) != PBH_OK) return 0;
+    return (size_t)g->nz * lat->n_layers * lat->n_sy * lat->n_sx * sizeof(double);
+}
+
+static bool adj_axis(double dv, double ds, int T, int K, AxisTable* out)
+{
+    for (int Q = 1; Q <= 64; ++Q) {
+        const double pr = dv * Q / ds;
+        const int P = (int)llround(pr);
+        if (P < 1 || P > 64) continue;
+        if (fabs(pr - P) > 1e-9 * pr) continue;
+        if (P * (T - 1) + Q * (K - 1) + 2 > FT_TAB) return false;
+        out->P = P;
+        out->Q = Q;
+        out->u = dv / P;
+        out->half = (P == 2) ? (P * (T - 1) + Q * (K - 1) + 2) / 2 : 0;
+        return true;
+    }
+    return false;
+}
+
+extern "C" int pbh_adjoint(const PbhGrid* g, const PbhLattice* lat, const float* w, const double* idd,
+                           const double* sigma, const float* R, float* grad_w, double* grad_idd,
+                           double* grad_sigma, int precision, void* workspace, size_t workspace_bytes,
+                           void* stream)
+{
+    int rc = pbh_validate(g, lat);
+    if (rc != PBH_OK) return rc;
+    if (!w || !idd || !sigma || !R || !grad_w || !workspace) return PBH_ERR_NULL_ARG;
+    if (precision != PBH_PREC_FP32) return PBH_ERR_UNSUPPORTED;
+    if (workspace_bytes < pbh_adjoint_workspace_bytes(g, lat, precision)) return PBH_ERR_WORKSPACE;
+    hipStream_t s = (hipStream_t)stream;
+    AdjArgs A;
+    A.f = make_fwd_args(g, lat, w, idd, sigma, nullptr);
+    A.R = R;
+    A.part = (double*)workspace;
+    A.grad_idd = grad_idd;
+    A.grad_sigma = grad_sigma;
+    A.split = adj_split(g, lat);
+    A.tab = adj_axis(g->dx, lat->dx_spot, AJ_TX, AJ_SC, &A.tx) && adj_axis(g->dy, lat->dy_spot, AJ_TY, AJ_SC, &A.ty);
+    if (!A.tab) {
+        A.tx = AxisTable{0, 0, 0.0, 0};
+        A.ty = AxisTable{0, 0, 0.0, 0};
+    }
+    const dim3 grid((unsigned)(g->nz * A.split)), block(AJ_THREADS);
+    const bool sig = grad_sigma != nullptr;
+    if (A.tab) {
+        if (sig) hipLaunchKernelGGL((adj_fp32_kernel<true, true>), grid, block, 0, s, A);
+        else hipLaunchKernelGGL((adj_fp32_kernel<true, false>), grid, block, 0, s, A);
+    } else {
+        if (sig) hipLaunchKernelGGL((adj_fp32_kernel<false, true>), grid, block, 0, s, A);
+        else hipLaunchKernelGGL((adj_fp32_kernel<false, false>), grid, block, 0, s, A);
+    }
+    if ((rc = launch_rc("adj_fp32_kernel")) != PBH_OK) return rc;
+    const size_t n = (size_t)lat->n_layers * lat->n_sy * lat->n_sx;
+    hipLaunchKernelGGL(adj_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       (const double*)A.part, g->nz, n, grad_w);
+    return launch_rc("adj_reduce_kernel");
+}
+
+static constexpr int OBJ_BLOCKS = 1024;
+
+extern "C" size_t pbh_objective_workspace_bytes(const PbhGrid* g)
+{
+    (void)g;
+    return OBJ_BLOCKS * sizeof(double);
+}
+
+extern "C" int pbh_objective(const PbhGrid* g, const float* dose, const float* presc, const float* vox_w,
+                             float* R, double* j_out, void* workspace, size_t workspace_bytes, void* stream)
+{
+    if (!g || !dose || !presc || !j_out || !workspace) return PBH_ERR_NULL_ARG;
+    if (g->nx <= 0 || g->ny <= 0 || g->nz <= 0) return PBH_ERR_BAD_DIM;
+    if (workspace_bytes < pbh_objective_workspace_bytes(g)) return PBH_ERR_WORKSPACE;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t n = (size_t)g->nx * g->ny * g->nz;
+    double* part = (double*)workspace;
+    hipLaunchKernelGGL(objective_kernel, dim3(OBJ_BLOCKS), dim3(OBJ_THREADS), 0, s, dose, presc, vox_w, n, R, part);
+    int rc = launch_rc("objective_kernel");
+    if (rc != PBH_OK) return rc;
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(64), 0, s, (const double*)part, OBJ_BLOCKS, j_out);
+    return launch_rc("sum_partials_kernel");
+}
+
+extern "C" int pbh_lut_chain(const double* grad_idd, const double* grad_sigma, const double* didd,
+                             const double* dsigma, int n_layers, int nz, double* out, void* stream)
+{
+    if (!grad_idd || !grad_sigma || !didd || !dsigma || !out) return PBH_ERR_NULL_ARG;
+    if (n_layers <= 0 || nz <= 0) return PBH_ERR_BAD_DIM;
+    hipLaunchKernelGGL(lut_chain_kernel, dim3(n_layers), dim3(64), 0, (hipStream_t)stream, grad_idd, grad_sigma,
+                       didd, dsigma, n_layers, nz, out);
+    return launch_rc("lut_chain_kernel");
+}

@@ -26,7 +26,7 @@ constexpr double kPi = 3.14159265358979323846;
 constexpr double kLog2e = 1.4426950408889634074;
 constexpr double kSigmaFloor = 1e-9;   // engines.py:57, :266
 
-static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+__host__ __device__ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 // Translates a HIP status into a library code, reporting it once on stderr.
 static inline int hip_rc(hipError_t e, const char* what)

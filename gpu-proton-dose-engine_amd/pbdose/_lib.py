@@ -61,6 +61,7 @@ SIGNATURES = {
     "pbh_adjoint": (C.c_int, [_G, _L, _P, _P, _P, _P, _P, _P, _P, C.c_int, _P, C.c_size_t, _P]),
     "pbh_objective_workspace_bytes": (C.c_size_t, [_G]),
     "pbh_objective": (C.c_int, [_G, _P, _P, _P, _P, _P, _P, C.c_size_t, _P]),
+    "pbh_lut_chain": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_int, _P, _P]),
     "pbh_gamma_workspace_bytes": (C.c_size_t, [_G]),
     "pbh_gamma_index": (C.c_int, [_G, _P, _P, C.c_double, C.c_double, C.c_double, C.c_double,
                                   _P, _P, _P, C.c_size_t, _P]),

@@ -125,7 +125,10 @@ def build_luts(nz: int, z0: float, dz: float, energies, *, idd_model: IDDModel =
     """
     dev = _lib.require_cuda(device)
     L = _lib.lib()
-    en = torch.as_tensor(np.asarray(energies, dtype=np.float64), device=dev).reshape(-1).contiguous()
+    if torch.is_tensor(energies):
+        en = energies.to(device=dev, dtype=torch.float64).reshape(-1).contiguous()
+    else:
+        en = torch.as_tensor(np.asarray(energies, dtype=np.float64), device=dev).reshape(-1).contiguous()
     nl = en.numel()
     m = beam_model(idd_model, sigma_x0, sigma_theta0, rho0)
     nbytes = L.pbh_lut_workspace_bytes(m, int(nz), nl)

@@ -1,0 +1,176 @@
+"""Inverse-planning layer: weighted-quadratic objective, hand-written adjoint and a
+``torch.autograd.Function`` over spot weights and per-layer range shifts.
+
+    J(w, delta) = sum_v wv_v (D_v(w, delta) - Dp_v)^2
+
+D is the forward dose of ``engines.DeviceProblem`` (HIP kernels). A per-layer
+range shift delta_l (cm) moves layer l's CSDA range R0_l -> R0_l + delta_l: the
+layer is rebuilt at energy E(R0_l + delta_l) by the on-device FP64 LUT builder.
+The gradients come from the adjoint kernels:
+
+    dJ/dw      pbh_adjoint (deterministic sum over depth)
+    dJ/ddelta  pbh_adjoint's dJ/dIDD and dJ/dsigma, contracted over depth with the
+               FP64 derivatives of the look-up tables with respect to delta
+               (central difference of the LUT builder with step ``lut_step``)
+               by pbh_lut_chain.
+
+No counterpart exists in the reference (its engines are forward-only,
+engines.py:208-283); the objective follows the workload definition used for
+the dose benchmarks: D_presc inside a central cube, voxel weights 1 inside and
+0.1 outside.
+"""
+from __future__ import annotations
+
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _lib, physics
+from .engines import DeviceProblem
+from .model import DoseGrid, PBProblem, SpotLattice
+
+
+class FieldModel:
+    """A field whose per-layer tables are rebuilt on the device for range shifts.
+
+    grid, lattice (with energies), beam/IDD model. ``luts(delta)`` returns FP64
+    (idd, sigma) device tensors for the energies E(R0 + delta)."""
+
+    def __init__(self, grid: DoseGrid, lattice: SpotLattice, *, idd_model=physics.DEFAULT_IDD,
+                 sigma_x0: float = 0.30, sigma_theta0: float = 0.0, rho0: float = 0.0,
+                 device="cuda", lut_step: float = 1e-3):
+        if lattice.energies is None:
+            raise ValueError("lattice.energies is required")
+        self.dev = _lib.require_cuda(device)
+        self.grid, self.lattice = grid, lattice
+        self.kw = dict(idd_model=idd_model, sigma_x0=sigma_x0, sigma_theta0=sigma_theta0, rho0=rho0)
+        self.lut_step = float(lut_step)
+        self.E0 = torch.as_tensor(np.asarray(lattice.energies, dtype=np.float64), device=self.dev)
+        E, R = physics.range_table(self.dev)
+        self.R0 = self._interp(E, R, self.E0)
+        self._cache = {}
+        idd, sig = self.luts(None)
+        self.problem = PBProblem(grid, lattice, idd.cpu().numpy(), sig.cpu().numpy())
+        self.dp = DeviceProblem(self.problem, self.dev, idd=idd, sigma=sig)
+
+    def _interp(self, xp, fp, x):
+        x = x.contiguous()
+        out = torch.empty_like(x)
+        _lib.check(_lib.lib().pbh_interp_table(_lib.ptr(xp), _lib.ptr(fp), xp.numel(), _lib.ptr(x),
+                                               _lib.ptr(out), x.numel(), _lib.stream_of(self.dev)),
+                   "pbh_interp_table")
+        return out
+
+    def energies(self, delta: Optional[torch.Tensor]) -> torch.Tensor:
+        if delta is None:
+            return self.E0
+        E, R = physics.range_table(self.dev)
+        return self._interp(R, E, self.R0 + delta.to(torch.float64))
+
+    def luts(self, delta: Optional[torch.Tensor]):
+        g = self.grid
+        idd, sig, _ = physics.build_luts(g.nz, g.oz, g.dz, self.energies(delta), device=self.dev,
+                                         **self.kw)
+        return idd, sig
+
+    def lut_derivatives(self, delta: Optional[torch.Tensor]):
+        """d(idd)/d(delta_l), d(sigma)/d(delta_l) (L, nz), FP64 central difference."""
+        h = self.lut_step
+        d0 = torch.zeros(self.lattice.n_layers, dtype=torch.float64, device=self.dev) if delta is None \
+            else delta.to(torch.float64)
+        ip, sp = self.luts(d0 + h)
+        im, sm = self.luts(d0 - h)
+        return (ip - im) / (2.0 * h), (sp - sm) / (2.0 * h)
+
+
+def objective(dp: DeviceProblem, dose: torch.Tensor, presc: torch.Tensor,
+              vox_w: Optional[torch.Tensor] = None, want_residual: bool = True):
+    """(J, R): J = sum wv (D - Dp)^2 (FP64 scalar tensor), R = dJ/dD = 2 wv (D - Dp)."""
+    L = _lib.lib()
+    R = torch.empty_like(dose) if want_residual else None
+    j = torch.empty(1, dtype=torch.float64, device=dp.dev)
+    ws = _lib.workspace(L.pbh_objective_workspace_bytes(dp.grid_s), dp.dev)
+    _lib.check(L.pbh_objective(dp.grid_s, _lib.ptr(dose), _lib.ptr(presc), _lib.ptr(vox_w), _lib.ptr(R),
+                               _lib.ptr(j), _lib.ptr(ws), ws.numel(), _lib.stream_of(dp.dev)),
+               "pbh_objective")
+    return j[0], R
+
+
+def adjoint(dp: DeviceProblem, R: torch.Tensor, w: Optional[torch.Tensor] = None,
+            lut_grads: bool = False):
+    """dJ/dw (L, nsy, nsx) FP32 for the dose cotangent R; with ``lut_grads`` also
+    dJ/dIDD and dJ/dsigma (L, nz) FP64."""
+    L = _lib.lib()
+    if R.dtype != torch.float32 or tuple(R.shape) != tuple(dp.shape):
+        raise ValueError("R must be float32 with the dose shape")
+    R = R.contiguous()
+    wt = dp.w if w is None else w.contiguous()
+    gw = torch.empty_like(dp.w)
+    gi = torch.empty_like(dp.idd) if lut_grads else None
+    gs = torch.empty_like(dp.sigma) if lut_grads else None
+    ws = dp.workspace("adj", 0)
+    _lib.check(L.pbh_adjoint(dp.grid_s, dp.lat_s, _lib.ptr(wt), _lib.ptr(dp.idd), _lib.ptr(dp.sigma),
+                             _lib.ptr(R), _lib.ptr(gw), _lib.ptr(gi), _lib.ptr(gs), 0, _lib.ptr(ws),
+                             ws.numel(), _lib.stream_of(dp.dev)), "pbh_adjoint")
+    return (gw, gi, gs) if lut_grads else gw
+
+
+def lut_chain(grad_idd, grad_sigma, didd, dsigma) -> torch.Tensor:
+    """Per-layer sum over depth of grad_idd * didd + grad_sigma * dsigma (FP64)."""
+    Lr, nz = grad_idd.shape
+    out = torch.empty(Lr, dtype=torch.float64, device=grad_idd.device)
+    _lib.check(_lib.lib().pbh_lut_chain(_lib.ptr(grad_idd), _lib.ptr(grad_sigma), _lib.ptr(didd.contiguous()),
+                                        _lib.ptr(dsigma.contiguous()), Lr, nz, _lib.ptr(out),
+                                        _lib.stream_of(grad_idd.device)), "pbh_lut_chain")
+    return out
+
+
+class _Dose(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, w, delta, model: FieldModel, precision: str):
+        dp = model.dp
+        if delta is not None:
+            idd, sig = model.luts(delta.detach())
+            dp = DeviceProblem(model.problem, model.dev, weights=w.detach(), idd=idd, sigma=sig)
+        D = dp.forward(precision, w=w.detach().contiguous())
+        ctx.model, ctx.dp = model, dp
+        ctx.save_for_backward(w, delta if delta is not None else torch.empty(0))
+        ctx.has_delta = delta is not None
+        return D
+
+    @staticmethod
+    def backward(ctx, gD):
+        w, delta = ctx.saved_tensors
+        need_delta = ctx.has_delta and ctx.needs_input_grad[1]
+        out = adjoint(ctx.dp, gD.contiguous().float(), w=w.detach(), lut_grads=need_delta)
+        if need_delta:
+            gw, gi, gs = out
+            didd, dsig = ctx.model.lut_derivatives(delta.detach())
+            gdelta = lut_chain(gi, gs, didd, dsig).to(delta.dtype)
+        else:
+            gw, gdelta = out, None
+        return (gw if ctx.needs_input_grad[0] else None), gdelta, None, None
+
+
+def dose(w: torch.Tensor, model: FieldModel, delta: Optional[torch.Tensor] = None,
+         precision: str = "fp32") -> torch.Tensor:
+    """Differentiable forward dose D(w, delta) (nz, ny, nx) float32 on the device."""
+    if w.dtype != torch.float32:
+        raise ValueError("spot weights must be float32")
+    return _Dose.apply(w, delta, model, precision)
+
+
+def central_cube(grid: DoseGrid, half: float = 3.0, dose_value: float = 2.0, w_in: float = 1.0,
+                 w_out: float = 0.1, device="cuda"):
+    """(Dp, wv) device tensors: dose_value inside the cube |x|, |y| <= half,
+    |z - z_mid| <= half, 0 outside; voxel weights w_in inside, w_out outside."""
+    x, y, z = grid.x, grid.y, grid.z
+    zc = 0.5 * (z[0] + z[-1])
+    inside = ((np.abs(z - zc) <= half + 1e-9)[:, None, None]
+              & (np.abs(y) <= half + 1e-9)[None, :, None]
+              & (np.abs(x) <= half + 1e-9)[None, None, :])
+    dev = _lib.require_cuda(device)
+    Dp = torch.as_tensor(np.where(inside, dose_value, 0.0), dtype=torch.float32, device=dev)
+    wv = torch.as_tensor(np.where(inside, w_in, w_out), dtype=torch.float32, device=dev)
+    return Dp.contiguous(), wv.contiguous()

@@ -31,7 +31,7 @@
  *   pencil_beam.cu:647-699 PB_ERR_* codes and pb_strerror         -> PBH_ERR_*, pbh_strerror
  *   pencil_beam.cu:717-730 pb_problem_validate                    -> pbh_validate
  *   pencil_beam.cu:907-919 pb_problem_bytes (sizing query)        -> *_workspace_bytes
- * The weighted-quadratic adjoint (pbh_adjoint_w, pbh_adjoint_sigma) and the
+ * The adjoint (pbh_adjoint, pbh_objective, pbh_lut_chain) and the
  * robust-scenario batch (pbh_batch_*) have no counterpart in the reference;
  * they serve inverse planning on top of the forward model above.
  */
@@ -176,6 +176,12 @@ size_t pbh_objective_workspace_bytes(const PbhGrid* g);
 int pbh_objective(const PbhGrid* g, const float* dose, const float* presc,
                   const float* vox_w, float* R, double* j_out,
                   void* workspace, size_t workspace_bytes, void* stream);
+
+/* out[l] = sum_z (grad_idd[l,z] didd[l,z] + grad_sigma[l,z] dsigma[l,z]) for
+ * (L, nz) FP64 arrays: the chain rule from the look-up tables to a per-layer
+ * parameter (e.g. a range shift, with didd/dsigma the tables' derivatives). */
+int pbh_lut_chain(const double* grad_idd, const double* grad_sigma, const double* didd,
+                  const double* dsigma, int n_layers, int nz, double* out, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Gamma index (global, integer-voxel DTA offsets). gamma.py:81-160

@@ -471,3 +471,43 @@ def adjoint_w(x, y, sx, sy, idd, sigma, R):
         Q = np.einsum("yx,lxi->lyi", R[kz], Gx)
         out += idd[:, kz, None, None] * np.einsum("lyj,lyi->lji", Gy, Q)
     return out
+
+
+def adjoint_lut(x, y, sx, sy, w, idd, sigma, R):
+    """dJ/dIDD_l(z) and dJ/dsigma_l(z) (L, nz) for a dose cotangent R, FP64.
+
+    With P = Gy^T R_z Gx (unnormalised Gaussians g = exp(-d^2 / 2 s^2)) and
+    n = 1 / (2 pi s^2):  dJ/dIDD = n <W, P>;
+    dJ/ds = IDD n [ -(2/s) <W, P> + <W, Gy'^T R Gx + Gy^T R Gx'> ], g' = g d^2 / s^3,
+    zero where sigma is at the 1e-9 floor."""
+    w = np.asarray(w, dtype=np.float64)
+    idd = np.asarray(idd, dtype=np.float64)
+    sigma = np.asarray(sigma, dtype=np.float64)
+    L, nz = idd.shape
+    dx2 = (np.asarray(x)[:, None] - np.asarray(sx)[None, :]) ** 2
+    dy2 = (np.asarray(y)[:, None] - np.asarray(sy)[None, :]) ** 2
+    gi = np.zeros((L, nz))
+    gs = np.zeros((L, nz))
+    for l in range(L):
+        for k in range(nz):
+            s = max(float(sigma[l, k]), SIGMA_FLOOR)
+            gx = np.exp(-dx2 / (2 * s * s))
+            gy = np.exp(-dy2 / (2 * s * s))
+            Q = R[k] @ gx
+            P = gy.T @ Q
+            P2 = (gy * dy2 / s**3).T @ Q + gy.T @ (R[k] @ (gx * dx2 / s**3))
+            n = 1.0 / (2 * math.pi * s * s)
+            wp = float(np.sum(w[l] * P))
+            gi[l, k] = n * wp
+            gs[l, k] = idd[l, k] * n * (-(2.0 / s) * wp + float(np.sum(w[l] * P2))) \
+                if sigma[l, k] > SIGMA_FLOOR else 0.0
+    return gi, gs
+
+
+def spot_dose(x, y, sx_i, sy_j, idd_l, sigma_l):
+    """Dose (nz, ny, nx) of one spot of unit weight in one layer."""
+    s = np.clip(np.asarray(sigma_l, dtype=np.float64), SIGMA_FLOOR, None)
+    gx = np.exp(-(np.asarray(x)[None, :] - sx_i) ** 2 / (2 * s[:, None] ** 2))
+    gy = np.exp(-(np.asarray(y)[None, :] - sy_j) ** 2 / (2 * s[:, None] ** 2))
+    c = np.asarray(idd_l) / (2 * np.pi * s * s)
+    return c[:, None, None] * gy[:, :, None] * gx[:, None, :]

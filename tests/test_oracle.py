@@ -213,3 +213,31 @@ def test_adjoint_dot_product():
     D = O.separable_dose(x, y, sx, sy, w, idd, sig)
     G = O.adjoint_w(x, y, sx, sy, idd, sig, R)
     assert abs(np.sum(R * D) - np.sum(G * w)) < 1e-12 * abs(np.sum(R * D))
+
+
+def test_adjoint_lut_matches_finite_differences():
+    x, y, sx, sy, w, idd, sig = _tiny(nsx=3, nsy=3, nz=3)
+    R = np.random.default_rng(8).standard_normal((3, len(y), len(x)))
+    gi, gs = O.adjoint_lut(x, y, sx, sy, w, idd, sig, R)
+    J = lambda i_, s_: float(np.sum(R * O.separable_dose(x, y, sx, sy, w, i_, s_)))
+    for (l, k) in [(0, 0), (1, 2), (2, 1)]:
+        h = 1e-6
+        ip, im = idd.copy(), idd.copy()
+        ip[l, k] += h
+        im[l, k] -= h
+        fd = (J(ip, sig) - J(im, sig)) / (2 * h)
+        assert abs(fd - gi[l, k]) <= 1e-7 * max(1.0, abs(fd))
+        sp, sm = sig.copy(), sig.copy()
+        sp[l, k] += h
+        sm[l, k] -= h
+        fd = (J(idd, sp) - J(idd, sm)) / (2 * h)
+        assert abs(fd - gs[l, k]) <= 1e-6 * max(1.0, abs(fd))
+
+
+def test_spot_dose_is_unit_weight_column_of_separable():
+    x, y, sx, sy, w, idd, sig = _tiny()
+    e = np.zeros_like(w)
+    e[1, 1, 2] = 1.0
+    D = O.separable_dose(x, y, sx, sy, e, idd, sig)
+    np.testing.assert_allclose(O.spot_dose(x, y, sx[2], sy[1], idd[1], sig[1]), D,
+                               rtol=1e-13, atol=1e-16)
