@@ -1,4 +1,5 @@
-"""Benchmark: dose fields/s for the 128^3-voxel, 67,240-spot forward dose on one GPU.
+"""Benchmark: dose fields/s for the 128^3-voxel, 67,240-spot forward dose on one GPU,
+plus forward + adjoint time and the mixed-precision vs FP32 comparison.
 
 Workload (one step = one forward dose of the whole field, inputs resident in HBM):
   water grid 128 x 128 x 128 at 2 mm; 41 x 41 spot lattice at 5 mm; 40 energy
@@ -6,10 +7,14 @@ Workload (one step = one forward dose of the whole field, inputs resident in HBM
   per-layer IDD and sigma from the on-device FP64 LUT builder.
 
 Prints ONE JSON line. ``value`` = fields/s of the selected arithmetic form
-(``--precision``). ``roofline`` is for the dominant kernel (the fused forward),
-timed per launch with HIP events on the launch stream. ``cpu_baseline`` times
-the host restatement of the reference's PyTorch batched-matmul engine
-(oracle/torch_cpu.py, FP64, all host threads) on full forwards of the same field.
+(``--precision``, default the BF16 split-product form); the FP32 form, the
+forward + adjoint time (adjoint of the weighted-quadratic objective: dose
+residual kernel + FP32 adjoint + depth reduction) and the error of each form
+against the FP64 oracle are reported beside it. ``roofline`` is for the dominant
+kernel (the fused forward of the selected form), timed per launch with HIP
+events on the launch stream. ``cpu_baseline`` times the host restatement of the
+reference's PyTorch batched-matmul engine (oracle/torch_cpu.py, FP64, all host
+threads) on full forwards of the same field.
 """
 from __future__ import annotations
 
@@ -27,8 +32,13 @@ for _p in (ROOT, os.path.join(ROOT, "gpu-proton-dose-engine_amd")):
 import numpy as np  # noqa: E402
 import torch  # noqa: E402
 
-PEAK = {"fp32": (157.3, "mfma"), "bf16": (2500.0, "mfma"), "fp16": (2500.0, "mfma")}
-KERNEL = {"fp32": "fwd_fp32_kernel"}
+# published dense MFMA peaks of MI355X (TFLOP/s) for the arithmetic each form runs on
+PEAK = {"fp32": 157.3, "bf16": 2500.0, "fp16": 2500.0}
+DTYPE = {"fp32": "f32", "bf16": "bf16", "fp16": "f16"}
+
+
+def base_type(precision: str) -> str:
+    return precision.split("x")[0].split("_")[0]
 
 
 def field_flops(nx, ny, nz, nsx, nsy, L):
@@ -67,14 +77,39 @@ def time_forward(dp, precision, steps, warmup):
     return out, wall * 1e3 / steps, launch_ms
 
 
-def parity(prob, D):
+def oracle_reference(prob, step=8):
     from oracle import pb_oracle as O
     idd, sig = O.layer_luts(prob.grid.z, prob.lattice.energies)
-    zs = np.arange(0, prob.grid.nz, 8)
+    zs = np.arange(0, prob.grid.nz, step)
     ref = O.separable_dose(prob.grid.x, prob.grid.y, prob.lattice.spot_x, prob.lattice.spot_y,
                            prob.lattice.weights, idd, sig, z_idx=zs)
+    return zs, ref
+
+
+def parity(ref_pack, D):
+    from oracle import pb_oracle as O
+    zs, ref = ref_pack
     r = O.relative_error(D.double().cpu().numpy()[zs], ref)
-    return r["max_rel_err_global"], r["rms_rel_global"], len(zs)
+    return r["max_rel_err_global"], r["rms_rel_global"]
+
+
+def time_adjoint(dp, D, steps, warmup):
+    """Objective residual + adjoint (dJ/dw) per call, events on the launch stream."""
+    from pbdose import planning
+    Dp, wv = planning.central_cube(dp.problem.grid, device=dp.dev)
+    for _ in range(warmup):
+        _, R = planning.objective(dp, D, Dp, wv)
+        planning.adjoint(dp, R)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    for s, e in ev:
+        s.record()
+        _, R = planning.objective(dp, D, Dp, wv)
+        planning.adjoint(dp, R)
+        e.record()
+    torch.cuda.synchronize()
+    return float(np.mean([s.elapsed_time(e) for s, e in ev]))
 
 
 def cpu_baseline(prob, budget_s=20.0, max_reps=15):
@@ -103,7 +138,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--precision", default="fp32")
+    ap.add_argument("--precision", default="bf16x3")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -119,12 +154,20 @@ def main():
     g, lat = prob.grid, prob.lattice
     flops = field_flops(g.nx, g.ny, g.nz, lat.n_sx, lat.n_sy, lat.n_layers)
 
+    ref = oracle_reference(prob)
     D, ms_step, launch_ms = time_forward(dp, args.precision, args.steps, args.warmup)
-    err_max, err_rms, n_sl = parity(prob, D)
-    peak, bound = PEAK.get(args.precision.split("x")[0].replace("_peak", ""), PEAK["fp32"])
+    err_max, err_rms = parity(ref, D)
+    if args.precision != "fp32":
+        D32, ms32, launch32 = time_forward(dp, "fp32", args.steps, args.warmup)
+        e32, _ = parity(ref, D32)
+    else:
+        D32, ms32, launch32, e32 = D, ms_step, launch_ms, err_max
+    adj_ms = time_adjoint(dp, D32, max(5, args.steps // 4), 3)
+    bt = base_type(args.precision)
+    peak = PEAK[bt]
     achieved = flops / (launch_ms * 1e-3) / 1e12
     line = {
-        "metric": "dose fields/s at 128^3 x 67,240 spots (forward)",
+        "metric": "dose fields/s at 128^3 x 67,240 spots (forward); fwd+adjoint ms; mixed vs FP32",
         "value": 1e3 / ms_step,
         "unit": "fields/s",
         "n_gpus": 1,
@@ -134,15 +177,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp32": "f32"}.get(args.precision, args.precision),
+        "dtype": DTYPE[bt],
         "data": "synthetic (lognormal spot weights, seed 0; on-device FP64 LUT)",
         "config": {"workload": "water 128x128x128 @ 2 mm, 41x41 spots @ 5 mm x 40 layers "
                                "70-180 MeV (67,240 spots), forward dose",
                    "precision": args.precision},
         "max_rel_err_vs_fp64": err_max,
         "rms_rel_err_vs_fp64": err_rms,
-        "parity_slices": n_sl,
-        "roofline": {"bound": bound, "kernel": KERNEL.get(args.precision, "fwd_mixed_kernel"),
+        "parity_slices": int(len(ref[0])),
+        "fp32": {"fields_per_s": 1e3 / ms32, "ms": ms32, "launch_ms": launch32,
+                 "max_rel_err_vs_fp64": e32},
+        "speedup_vs_fp32": ms32 / ms_step,
+        "adjoint_ms": adj_ms,
+        "fwd_adj_ms": ms_step + adj_ms,
+        "fwd_adj_ms_fp32": ms32 + adj_ms,
+        "roofline": {"bound": "mfma",
+                     "kernel": "fwd_fp32_kernel" if args.precision == "fp32" else "fwd_mixed_kernel",
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,
                      "algorithmic_flop_per_launch": flops, "launch_ms": launch_ms},
