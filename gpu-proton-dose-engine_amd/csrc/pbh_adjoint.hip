@@ -10,15 +10,15 @@
 //   with G' = dG/ds = G (v - s_i)^2 / s^3.
 //
 // Kernel 1 (one workgroup per depth slice and share of layers): the slice of R is
-// staged in LDS tile by tile (64 x 128), then per layer
+// staged in LDS tile by tile (64 x 128, once per tile), then per layer
 //   Q  = R Gx, Q' = R Gx'     (M = 64 rows, N = spots, K = 128 columns; v_mfma_f32_16x16x4f32)
 //   P += Gy^T Q, P2 += Gy'^T Q + Gy^T Q'   (M = N = spots, K = 64 rows)
 // with the Gaussians gathered from per-(layer, depth) difference tables as in the
 // forward kernels (direct evaluation on grids without a small pitch ratio). The
 // per-slice products c P go to a workspace; <W, P> and <W, P2> are reduced in
 // the workgroup in a fixed order.
-// Kernel 2 sums the per-slice partials over z in FP64 in increasing z: the weight
-// gradient is deterministic (no atomics).
+// Kernel 2 sums the per-slice FP32 partials over z in FP64 in increasing z: the
+// weight gradient is deterministic (no atomics).
 #include "pbh_fwd_core.h"
 
 namespace pbh {
@@ -31,7 +31,7 @@ constexpr int AJ_QS = AJ_SC + 1;                 // Q row stride
 struct AdjArgs {
     FwdArgs f;
     const float* R;        // (nz, ny, nx)
-    double* part;          // [nz][L][nsy][nsx] per-slice c P (FP64 copy of FP32 sums)
+    float* part;           // [nz][L][nsy][nsx] per-slice c P
     double* grad_idd;      // (L, nz) or null
     double* grad_sigma;    // (L, nz) or null
     int split;
@@ -41,6 +41,8 @@ struct AdjArgs {
 
 // Slot of table entry t for axes of AJ_TX / AJ_TY voxels (same layout rules as the forward).
 __device__ __forceinline__ float tab_get(const float* T, const AxisTable& g, int t) { return T[table_slot(g, t)]; }
+
+constexpr int AJ_LB = 64;   // layers of one workgroup handled per pass over the slice tiles
 
 template <bool TAB, bool SIG>
 __global__ __launch_bounds__(AJ_THREADS, 2)
@@ -53,6 +55,7 @@ void adj_fp32_kernel(AdjArgs A)
     __shared__ float Tx[TAB ? FT_TAB : 1], Tx2[(TAB && SIG) ? FT_TAB : 1];
     __shared__ float Ty[TAB ? FT_TAB : 1], Ty2[(TAB && SIG) ? FT_TAB : 1];
     __shared__ double red[2][AJ_THREADS / 64];
+    __shared__ double wacc[2][AJ_LB];          // <W, P>, <W, P2> per layer of the pass
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int s = blockIdx.x % A.split;
@@ -61,27 +64,23 @@ void adj_fp32_kernel(AdjArgs A)
     const int tiles_x = cdiv(a.nx, AJ_TX), tiles_y = cdiv(a.ny, AJ_TY);
     const int ncx = chunk_count(a.nsx), ncy = chunk_count(a.nsy);
     const float* Rz = A.R + (size_t)z * a.ny * a.nx;
+    const bool lut_grads = A.grad_idd || (SIG && A.grad_sigma);
 
-    for (int l = s; l < a.L; l += A.split) {
-        const double iddv = a.idd[(size_t)l * a.nz + z];
-        const double sg = a.sigma[(size_t)l * a.nz + z];
-        const double sd = sg > kSigmaFloor ? sg : kSigmaFloor;
-        const double k2d = kLog2e / (2.0 * sd * sd);
-        const float k2 = (float)k2d;
-        const float is3 = (float)(1.0 / (sd * sd * sd));        // dG/ds = G d^2 / s^3
-        const double norm = 1.0 / (2.0 * kPi * sd * sd);
-        double wp = 0.0, wp2 = 0.0;                               // <W, P>, <W, P2> (lane partials)
-
+    for (int lb = s; lb < a.L; lb += AJ_LB * A.split) {
+        const int nl = min(AJ_LB, (a.L - lb + A.split - 1) / A.split);
+        if (threadIdx.x < AJ_LB) {
+            wacc[0][threadIdx.x] = 0.0;
+            wacc[1][threadIdx.x] = 0.0;
+        }
         for (int cy = 0; cy < ncy; ++cy) {
             const int j0 = chunk_begin(a.nsy, ncy, cy), nj = chunk_begin(a.nsy, ncy, cy + 1) - j0;
             for (int cx = 0; cx < ncx; ++cx) {
                 const int i0 = chunk_begin(a.nsx, ncx, cx), ni = chunk_begin(a.nsx, ncx, cx + 1) - i0;
-                // P blocks (3 x 3 of 16 x 16 over j, i): wave w owns blocks w, w + 4, w + 8
-                v4f P[3] = {}, P2[3] = {};
                 for (int tyi = 0; tyi < tiles_y; ++tyi) {
                     for (int txi = 0; txi < tiles_x; ++txi) {
                         const int y0 = tyi * AJ_TY, x0 = txi * AJ_TX;
-                        __syncthreads();   // previous tile's readers are done
+                        const bool first_tile = tyi == 0 && txi == 0;
+                        __syncthreads();   // previous readers of Rs are done
                         for (int e = threadIdx.x; e < AJ_TY * AJ_TX; e += AJ_THREADS) {
                             const int yy = e / AJ_TX, xx = e % AJ_TX;
                             const int y = y0 + yy, x = x0 + xx;
@@ -89,158 +88,181 @@ void adj_fp32_kernel(AdjArgs A)
                         }
                         const double Cx = (a.ox + (double)x0 * a.dx) - (a.sx0 + (double)i0 * a.dsx);
                         const double Cy = (a.oy + (double)y0 * a.dy) - (a.sy0 + (double)j0 * a.dsy);
-                        if (TAB) {
-                            const int ntX = A.tx.P * (AJ_TX - 1) + A.tx.Q * (AJ_SC - 1) + 1;
-                            const int ntY = A.ty.P * (AJ_TY - 1) + A.ty.Q * (AJ_SC - 1) + 1;
-                            for (int t = threadIdx.x; t < ntX; t += AJ_THREADS) {
-                                const double d = table_distance(A.tx, Cx, AJ_TX, t);
-                                const float gv = __builtin_amdgcn_exp2f((float)(-(d * d) * k2d));
-                                Tx[table_slot(A.tx, t)] = gv;
-                                if (SIG) Tx2[table_slot(A.tx, t)] = gv * (float)(d * d) * is3;
-                            }
-                            for (int t = threadIdx.x; t < ntY; t += AJ_THREADS) {
-                                const double d = table_distance(A.ty, Cy, AJ_TY, t);
-                                const float gv = __builtin_amdgcn_exp2f((float)(-(d * d) * k2d));
-                                Ty[table_slot(A.ty, t)] = gv;
-                                if (SIG) Ty2[table_slot(A.ty, t)] = gv * (float)(d * d) * is3;
-                            }
-                        }
-                        __syncthreads();
 
-                        // Gaussian of voxel column xx (tile) and spot i (chunk): table or direct
-                        auto gx = [&](int xx, int i, float* dg) -> float {
+                        for (int q = 0; q < nl; ++q) {
+                            const int l = lb + q * A.split;
+                            const double iddv = a.idd[(size_t)l * a.nz + z];
+                            const double sg = a.sigma[(size_t)l * a.nz + z];
+                            const double sd = sg > kSigmaFloor ? sg : kSigmaFloor;
+                            const double k2d = kLog2e / (2.0 * sd * sd);
+                            const float k2 = (float)k2d;
+                            const float is3 = (float)(1.0 / (sd * sd * sd));   // dG/ds = G d^2 / s^3
+                            const double cn = iddv / (2.0 * kPi * sd * sd);
                             if (TAB) {
-                                const int t = A.tx.Q * i - A.tx.P * xx + A.tx.P * (AJ_TX - 1);
-                                if (SIG) *dg = tab_get(Tx2, A.tx, t);
-                                return tab_get(Tx, A.tx, t);
-                            }
-                            const double d = (a.ox + (double)(x0 + xx) * a.dx) - (a.sx0 + (double)(i0 + i) * a.dsx);
-                            const float d2 = (float)(d * d);
-                            const float gv = i < ni ? gauss2(d2, k2) : 0.0f;
-                            if (SIG) *dg = gv * d2 * is3;
-                            return gv;
-                        };
-                        auto gy = [&](int yy, int j, float* dg) -> float {
-                            if (TAB) {
-                                const int t = A.ty.Q * j - A.ty.P * yy + A.ty.P * (AJ_TY - 1);
-                                if (SIG) *dg = tab_get(Ty2, A.ty, t);
-                                return tab_get(Ty, A.ty, t);
-                            }
-                            const double d = (a.oy + (double)(y0 + yy) * a.dy) - (a.sy0 + (double)(j0 + j) * a.dsy);
-                            const float d2 = (float)(d * d);
-                            const float gv = j < nj ? gauss2(d2, k2) : 0.0f;
-                            if (SIG) *dg = gv * d2 * is3;
-                            return gv;
-                        };
-
-                        // stage 1: Q[y][i] = sum_x R[y][x] Gx[x][i]; wave w owns rows 16w..16w+15
-                        {
-                            v4f q[3] = {}, q2[3] = {};
-#pragma unroll 4
-                            for (int kk = 0; kk < AJ_TX / 4; ++kk) {
-                                const int xx = 4 * kk + g;
-                                const float av = Rs[(16 * wave + c16) * AJ_RS + xx];
-#pragma unroll
-                                for (int nb = 0; nb < 3; ++nb) {
-                                    float dg = 0.0f;
-                                    const float bv = gx(xx, 16 * nb + c16, &dg);
-                                    q[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, q[nb], 0, 0, 0);
-                                    if (SIG) q2[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, dg, q2[nb], 0, 0, 0);
+                                const int ntX = A.tx.P * (AJ_TX - 1) + A.tx.Q * (AJ_SC - 1) + 1;
+                                const int ntY = A.ty.P * (AJ_TY - 1) + A.ty.Q * (AJ_SC - 1) + 1;
+                                for (int t = threadIdx.x; t < ntX; t += AJ_THREADS) {
+                                    const double d = table_distance(A.tx, Cx, AJ_TX, t);
+                                    const float gv = __builtin_amdgcn_exp2f((float)(-(d * d) * k2d));
+                                    Tx[table_slot(A.tx, t)] = gv;
+                                    if (SIG) Tx2[table_slot(A.tx, t)] = gv * (float)(d * d) * is3;
+                                }
+                                for (int t = threadIdx.x; t < ntY; t += AJ_THREADS) {
+                                    const double d = table_distance(A.ty, Cy, AJ_TY, t);
+                                    const float gv = __builtin_amdgcn_exp2f((float)(-(d * d) * k2d));
+                                    Ty[table_slot(A.ty, t)] = gv;
+                                    if (SIG) Ty2[table_slot(A.ty, t)] = gv * (float)(d * d) * is3;
                                 }
                             }
-#pragma unroll
-                            for (int nb = 0; nb < 3; ++nb)
-#pragma unroll
-                                for (int r = 0; r < 4; ++r) {
-                                    const int off = (16 * wave + 4 * g + r) * AJ_QS + 16 * nb + c16;
-                                    Qs[off] = q[nb][r];
-                                    if (SIG) Q2s[off] = q2[nb][r];
-                                }
-                        }
-                        __syncthreads();
+                            __syncthreads();
 
-                        // stage 2: P[j][i] += sum_y Gy[y][j] Q[y][i] (A = Gy^T, B = Q)
-#pragma unroll
-                        for (int q = 0; q < 3; ++q) {
-                            const int blk = wave + 4 * q;
-                            if (blk < 9) {
-                                const int mb = blk / 3, nb = blk % 3;
-                                v4f p = P[q], p2 = P2[q];
+                            // Gaussian of voxel column xx (tile) and spot i (chunk): table or direct
+                            auto gx = [&](int xx, int i, float* dg) -> float {
+                                if (TAB) {
+                                    const int t = A.tx.Q * i - A.tx.P * xx + A.tx.P * (AJ_TX - 1);
+                                    if (SIG) *dg = tab_get(Tx2, A.tx, t);
+                                    return tab_get(Tx, A.tx, t);
+                                }
+                                const double d = (a.ox + (double)(x0 + xx) * a.dx) - (a.sx0 + (double)(i0 + i) * a.dsx);
+                                const float d2 = (float)(d * d);
+                                const float gv = i < ni ? gauss2(d2, k2) : 0.0f;
+                                if (SIG) *dg = gv * d2 * is3;
+                                return gv;
+                            };
+                            auto gy = [&](int yy, int j, float* dg) -> float {
+                                if (TAB) {
+                                    const int t = A.ty.Q * j - A.ty.P * yy + A.ty.P * (AJ_TY - 1);
+                                    if (SIG) *dg = tab_get(Ty2, A.ty, t);
+                                    return tab_get(Ty, A.ty, t);
+                                }
+                                const double d = (a.oy + (double)(y0 + yy) * a.dy) - (a.sy0 + (double)(j0 + j) * a.dsy);
+                                const float d2 = (float)(d * d);
+                                const float gv = j < nj ? gauss2(d2, k2) : 0.0f;
+                                if (SIG) *dg = gv * d2 * is3;
+                                return gv;
+                            };
+
+                            // stage 1: Q[y][i] = sum_x R[y][x] Gx[x][i]; wave w owns rows 16w..16w+15
+                            {
+                                v4f q1[3] = {}, q2[3] = {};
 #pragma unroll 4
-                                for (int kk = 0; kk < AJ_TY / 4; ++kk) {
-                                    const int yy = 4 * kk + g;
-                                    float dgy = 0.0f;
-                                    const float av = gy(yy, 16 * mb + c16, &dgy);
-                                    const float bv = Qs[yy * AJ_QS + 16 * nb + c16];
-                                    p = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, p, 0, 0, 0);
-                                    if (SIG) {
-                                        p2 = __builtin_amdgcn_mfma_f32_16x16x4f32(dgy, bv, p2, 0, 0, 0);
-                                        p2 = __builtin_amdgcn_mfma_f32_16x16x4f32(av, Q2s[yy * AJ_QS + 16 * nb + c16], p2, 0, 0, 0);
+                                for (int kk = 0; kk < AJ_TX / 4; ++kk) {
+                                    const int xx = 4 * kk + g;
+                                    const float av = Rs[(16 * wave + c16) * AJ_RS + xx];
+#pragma unroll
+                                    for (int nb = 0; nb < 3; ++nb) {
+                                        float dg = 0.0f;
+                                        const float bv = gx(xx, 16 * nb + c16, &dg);
+                                        q1[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, q1[nb], 0, 0, 0);
+                                        if (SIG) q2[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, dg, q2[nb], 0, 0, 0);
                                     }
                                 }
-                                P[q] = p;
-                                P2[q] = p2;
+#pragma unroll
+                                for (int nb = 0; nb < 3; ++nb)
+#pragma unroll
+                                    for (int r = 0; r < 4; ++r) {
+                                        const int off = (16 * wave + 4 * g + r) * AJ_QS + 16 * nb + c16;
+                                        Qs[off] = q1[nb][r];
+                                        if (SIG) Q2s[off] = q2[nb][r];
+                                    }
                             }
-                        }
-                    }
-                }
-                // epilogue for (layer, chunk): partial c P, and <W, P>, <W, P2>
-                const float* wl = a.w + (size_t)l * a.nsy * a.nsx;
-                double* pz = A.part + ((size_t)z * a.L + l) * a.nsy * a.nsx;
+                            __syncthreads();
+
+                            // stage 2: P[j][i] = sum_y Gy[y][j] Q[y][i] over this tile (A = Gy^T, B = Q);
+                            // the tile's c P joins the per-slice partial (first tile stores)
+                            const float* wl = a.w + (size_t)l * a.nsy * a.nsx;
+                            float* pz = A.part + ((size_t)z * a.L + l) * a.nsy * a.nsx;
+                            double wp = 0.0, wp2 = 0.0;
 #pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    const int blk = wave + 4 * q;
-                    if (blk < 9) {
-                        const int mb = blk / 3, nb = blk % 3;
+                            for (int qq = 0; qq < 3; ++qq) {
+                                const int blk = wave + 4 * qq;
+                                if (blk < 9) {
+                                    const int mb = blk / 3, nb = blk % 3;
+                                    v4f p = {}, p2 = {};
+#pragma unroll 4
+                                    for (int kk = 0; kk < AJ_TY / 4; ++kk) {
+                                        const int yy = 4 * kk + g;
+                                        float dgy = 0.0f;
+                                        const float av = gy(yy, 16 * mb + c16, &dgy);
+                                        const float bv = Qs[yy * AJ_QS + 16 * nb + c16];
+                                        p = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, p, 0, 0, 0);
+                                        if (SIG) {
+                                            p2 = __builtin_amdgcn_mfma_f32_16x16x4f32(dgy, bv, p2, 0, 0, 0);
+                                            p2 = __builtin_amdgcn_mfma_f32_16x16x4f32(av, Q2s[yy * AJ_QS + 16 * nb + c16], p2, 0, 0, 0);
+                                        }
+                                    }
 #pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const int j = 16 * mb + 4 * g + r, i = 16 * nb + c16;
-                            if (j < nj && i < ni) {
-                                const size_t off = (size_t)(j0 + j) * a.nsx + i0 + i;
-                                const double wv = (double)wl[off];
-                                pz[off] = iddv * norm * (double)P[q][r];
-                                wp += wv * (double)P[q][r];
-                                if (SIG) wp2 += wv * (double)P2[q][r];
+                                    for (int r = 0; r < 4; ++r) {
+                                        const int j = 16 * mb + 4 * g + r, i = 16 * nb + c16;
+                                        if (j < nj && i < ni) {
+                                            const size_t off = (size_t)(j0 + j) * a.nsx + i0 + i;
+                                            const float cp = (float)(cn * (double)p[r]);
+                                            pz[off] = first_tile ? cp : pz[off] + cp;
+                                            if (lut_grads) {
+                                                const double wv = (double)wl[off];
+                                                wp += wv * (double)p[r];
+                                                if (SIG) wp2 += wv * (double)p2[r];
+                                            }
+                                        }
+                                    }
+                                }
+                            }
+                            if (lut_grads) {   // fixed-order workgroup sum into the pass accumulators
+                                for (int o = 32; o > 0; o >>= 1) {
+                                    wp += __shfl_xor(wp, o, 64);
+                                    wp2 += __shfl_xor(wp2, o, 64);
+                                }
+                                if (lane == 0) {
+                                    red[0][wave] = wp;
+                                    red[1][wave] = wp2;
+                                }
+                            }
+                            __syncthreads();
+                            if (lut_grads && threadIdx.x == 0) {
+                                double t0 = 0.0, t1 = 0.0;
+                                for (int w = 0; w < AJ_THREADS / 64; ++w) {
+                                    t0 += red[0][w];
+                                    t1 += red[1][w];
+                                }
+                                wacc[0][q] += t0;
+                                wacc[1][q] += t1;
                             }
                         }
                     }
                 }
             }
         }
-        // workgroup sums of <W, P>, <W, P2> in a fixed order
-        if (A.grad_idd || (SIG && A.grad_sigma)) {
-            for (int o = 32; o > 0; o >>= 1) {
-                wp += __shfl_xor(wp, o, 64);
-                wp2 += __shfl_xor(wp2, o, 64);
-            }
-            __syncthreads();
-            if (lane == 0) {
-                red[0][wave] = wp;
-                red[1][wave] = wp2;
-            }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                double t0 = 0.0, t1 = 0.0;
-                for (int w = 0; w < AJ_THREADS / 64; ++w) {
-                    t0 += red[0][w];
-                    t1 += red[1][w];
-                }
-                if (A.grad_idd) A.grad_idd[(size_t)l * a.nz + z] = t0 * norm;
-                if (SIG && A.grad_sigma)
-                    A.grad_sigma[(size_t)l * a.nz + z] =
-                        sg > kSigmaFloor ? iddv * norm * (-(2.0 / sd) * t0 + t1) : 0.0;
-            }
+        __syncthreads();
+        if (lut_grads && threadIdx.x < nl) {
+            const int q = threadIdx.x, l = lb + q * A.split;
+            const double iddv = a.idd[(size_t)l * a.nz + z];
+            const double sg = a.sigma[(size_t)l * a.nz + z];
+            const double sd = sg > kSigmaFloor ? sg : kSigmaFloor;
+            const double norm = 1.0 / (2.0 * kPi * sd * sd);
+            if (A.grad_idd) A.grad_idd[(size_t)l * a.nz + z] = wacc[0][q] * norm;
+            if (SIG && A.grad_sigma)
+                A.grad_sigma[(size_t)l * a.nz + z] =
+                    sg > kSigmaFloor ? iddv * norm * (-(2.0 / sd) * wacc[0][q] + wacc[1][q]) : 0.0;
         }
+        __syncthreads();
     }
 }
 
-// grad_w[e] = sum_z part[z][e] (increasing z), e over L * nsy * nsx
-__global__ void adj_reduce_kernel(const double* part, int nz, size_t n, float* grad_w)
+// grad_w[e] = sum_z part[z][e] in FP64, increasing z (8 loads in flight per thread)
+__global__ void adj_reduce_kernel(const float* part, int nz, size_t n, float* grad_w)
 {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     double acc = 0.0;
-    for (int z = 0; z < nz; ++z) acc += part[(size_t)z * n + e];
+    int z = 0;
+    for (; z + 8 <= nz; z += 8) {
+        float v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = part[(size_t)(z + k) * n + e];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc += (double)v[k];
+    }
+    for (; z < nz; ++z) acc += (double)part[(size_t)z * n + e];
     grad_w[e] = (float)acc;
 }
 
@@ -268,13 +290,19 @@ __global__ __launch_bounds__(OBJ_THREADS) void objective_kernel(const float* D, 
     }
 }
 
-__global__ void sum_partials_kernel(const double* partial, int n, double* out)
+// out[0] = sum of n partials: each thread sums a fixed stride, then a fixed tree
+__global__ __launch_bounds__(OBJ_THREADS) void sum_partials_kernel(const double* partial, int n, double* out)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        double t = 0.0;
-        for (int i = 0; i < n; ++i) t += partial[i];
-        out[0] = t;
+    __shared__ double red[OBJ_THREADS];
+    double t = 0.0;
+    for (int i = threadIdx.x; i < n; i += OBJ_THREADS) t += partial[i];
+    red[threadIdx.x] = t;
+    __syncthreads();
+    for (int w = OBJ_THREADS / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) out[0] = red[0];
 }
 
 // out[l] = sum_z (g_idd[l,z] didd[l,z] + g_sig[l,z] dsig[l,z]) (one wave per layer, fixed order)
@@ -311,7 +339,7 @@ extern "C" size_t pbh_adjoint_workspace_bytes(const PbhGrid* g, const PbhLattice
 {
     (void)precision;
     if (pbh_validate(g, lat) != PBH_OK) return 0;
-    return (size_t)g->nz * lat->n_layers * lat->n_sy * lat->n_sx * sizeof(double);
+    return (size_t)g->nz * lat->n_layers * lat->n_sy * lat->n_sx * sizeof(float);
 }
 
 static bool adj_axis(double dv, double ds, int T, int K, AxisTable* out)
@@ -345,7 +373,7 @@ extern "C" int pbh_adjoint(const PbhGrid* g, const PbhLattice* lat, const float*
     AdjArgs A;
     A.f = make_fwd_args(g, lat, w, idd, sigma, nullptr);
     A.R = R;
-    A.part = (double*)workspace;
+    A.part = (float*)workspace;
     A.grad_idd = grad_idd;
     A.grad_sigma = grad_sigma;
     A.split = adj_split(g, lat);
@@ -366,7 +394,7 @@ extern "C" int pbh_adjoint(const PbhGrid* g, const PbhLattice* lat, const float*
     if ((rc = launch_rc("adj_fp32_kernel")) != PBH_OK) return rc;
     const size_t n = (size_t)lat->n_layers * lat->n_sy * lat->n_sx;
     hipLaunchKernelGGL(adj_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                       (const double*)A.part, g->nz, n, grad_w);
+                       (const float*)A.part, g->nz, n, grad_w);
     return launch_rc("adj_reduce_kernel");
 }
 
@@ -390,7 +418,7 @@ extern "C" int pbh_objective(const PbhGrid* g, const float* dose, const float* p
     hipLaunchKernelGGL(objective_kernel, dim3(OBJ_BLOCKS), dim3(OBJ_THREADS), 0, s, dose, presc, vox_w, n, R, part);
     int rc = launch_rc("objective_kernel");
     if (rc != PBH_OK) return rc;
-    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(64), 0, s, (const double*)part, OBJ_BLOCKS, j_out);
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(OBJ_THREADS), 0, s, (const double*)part, OBJ_BLOCKS, j_out);
     return launch_rc("sum_partials_kernel");
 }
 
