@@ -19,7 +19,7 @@
 // the workgroup in a fixed order.
 // Kernel 2 sums the per-slice FP32 partials over z in FP64 in increasing z: the
 // weight gradient is deterministic (no atomics).
-#include "pbh_fwd_core.h"
+#include "pbh_batch.h"
 
 namespace pbh {
 
@@ -28,25 +28,14 @@ constexpr int AJ_SC = PBH_SPOT_CHUNK;            // 48
 constexpr int AJ_RS = AJ_TX + 2;                 // R tile row stride: lanes (c, g) hit banks 2c + g
 constexpr int AJ_QS = AJ_SC + 1;                 // Q row stride
 
-struct AdjArgs {
-    FwdArgs f;
-    const float* R;        // (nz, ny, nx)
-    float* part;           // [nz][L][nsy][nsx] per-slice c P
-    double* grad_idd;      // (L, nz) or null
-    double* grad_sigma;    // (L, nz) or null
-    int split;
-    int tab;
-    AxisTable tx, ty;
-};
-
 // Slot of table entry t for axes of AJ_TX / AJ_TY voxels (same layout rules as the forward).
 __device__ __forceinline__ float tab_get(const float* T, const AxisTable& g, int t) { return T[table_slot(g, t)]; }
 
 constexpr int AJ_LB = 64;   // layers of one workgroup handled per pass over the slice tiles
 
+// blk: the workgroup's index in the problem's own grid (slice * split + share).
 template <bool TAB, bool SIG>
-__global__ __launch_bounds__(AJ_THREADS, 2)
-void adj_fp32_kernel(AdjArgs A)
+__device__ __forceinline__ void adj_fp32_body(const AdjArgs& A, int blk)
 {
     const FwdArgs& a = A.f;
     __shared__ float Rs[AJ_TY * AJ_RS];
@@ -58,8 +47,8 @@ void adj_fp32_kernel(AdjArgs A)
     __shared__ double wacc[2][AJ_LB];          // <W, P>, <W, P2> per layer of the pass
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int s = blockIdx.x % A.split;
-    const int z = blockIdx.x / A.split;
+    const int s = blk % A.split;
+    const int z = blk / A.split;
     const int g = lane >> 4, c16 = lane & 15;
     const int tiles_x = cdiv(a.nx, AJ_TX), tiles_y = cdiv(a.ny, AJ_TY);
     const int ncx = chunk_count(a.nsx), ncy = chunk_count(a.nsy);
@@ -248,10 +237,26 @@ void adj_fp32_kernel(AdjArgs A)
     }
 }
 
-// grad_w[e] = sum_z part[z][e] in FP64, increasing z (8 loads in flight per thread)
-__global__ void adj_reduce_kernel(const float* part, int nz, size_t n, float* grad_w)
+template <bool TAB, bool SIG>
+__global__ __launch_bounds__(AJ_THREADS, 2)
+void adj_fp32_kernel(AdjArgs A)
 {
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    adj_fp32_body<TAB, SIG>(A, blockIdx.x);
+}
+
+// One launch over a batch of problems sharing the template parameters.
+template <bool TAB, bool SIG>
+__global__ __launch_bounds__(AJ_THREADS, 2)
+void adj_fp32_batch_kernel(const AdjArgs* __restrict__ items, const int* __restrict__ wg0, int n)
+{
+    const int k = batch_item(wg0, n, blockIdx.x);
+    const AdjArgs A = items[k];
+    adj_fp32_body<TAB, SIG>(A, (int)blockIdx.x - wg0[k]);
+}
+
+// grad_w[e] = sum_z part[z][e] in FP64, increasing z (8 loads in flight per thread)
+__device__ __forceinline__ void adj_reduce_body(const float* part, int nz, size_t n, float* grad_w, size_t e)
+{
     if (e >= n) return;
     double acc = 0.0;
     int z = 0;
@@ -266,15 +271,29 @@ __global__ void adj_reduce_kernel(const float* part, int nz, size_t n, float* gr
     grad_w[e] = (float)acc;
 }
 
+__global__ void adj_reduce_kernel(const float* part, int nz, size_t n, float* grad_w)
+{
+    adj_reduce_body(part, nz, n, grad_w, (size_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ void adj_reduce_batch_kernel(const BatchRedItem* __restrict__ items, const int* __restrict__ wg0, int n)
+{
+    const int k = batch_item(wg0, n, blockIdx.x);
+    const BatchRedItem it = items[k];
+    adj_reduce_body(it.part, it.nz, it.n, it.grad_w,
+                    (size_t)((int)blockIdx.x - wg0[k]) * blockDim.x + threadIdx.x);
+}
+
 // ---- weighted quadratic objective: R = 2 wv (D - Dp), J = sum wv (D - Dp)^2 ----
 constexpr int OBJ_THREADS = 256;
 
-__global__ __launch_bounds__(OBJ_THREADS) void objective_kernel(const float* D, const float* Dp, const float* wv,
-                                                                size_t n, float* R, double* partial)
+// Workgroup blk of nblk: grid-stride loop over the voxels, partial[blk] = its sum.
+__device__ __forceinline__ void objective_body(const float* D, const float* Dp, const float* wv, size_t n,
+                                               float* R, double* partial, int blk, int nblk)
 {
     __shared__ double red[OBJ_THREADS / 64];
     double acc = 0.0;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    for (size_t e = (size_t)blk * OBJ_THREADS + threadIdx.x; e < n; e += (size_t)nblk * OBJ_THREADS) {
         const double r = (double)D[e] - (double)Dp[e];
         const double w = wv ? (double)wv[e] : 1.0;
         acc += w * r * r;
@@ -286,12 +305,18 @@ __global__ __launch_bounds__(OBJ_THREADS) void objective_kernel(const float* D, 
     if (threadIdx.x == 0) {
         double t = 0.0;
         for (int w = 0; w < OBJ_THREADS / 64; ++w) t += red[w];
-        partial[blockIdx.x] = t;
+        partial[blk] = t;
     }
 }
 
+__global__ __launch_bounds__(OBJ_THREADS) void objective_kernel(const float* D, const float* Dp, const float* wv,
+                                                                size_t n, float* R, double* partial)
+{
+    objective_body(D, Dp, wv, n, R, partial, blockIdx.x, gridDim.x);
+}
+
 // out[0] = sum of n partials: each thread sums a fixed stride, then a fixed tree
-__global__ __launch_bounds__(OBJ_THREADS) void sum_partials_kernel(const double* partial, int n, double* out)
+__device__ __forceinline__ void sum_partials_body(const double* partial, int n, double* out)
 {
     __shared__ double red[OBJ_THREADS];
     double t = 0.0;
@@ -303,6 +328,26 @@ __global__ __launch_bounds__(OBJ_THREADS) void sum_partials_kernel(const double*
         __syncthreads();
     }
     if (threadIdx.x == 0) out[0] = red[0];
+}
+
+__global__ __launch_bounds__(OBJ_THREADS) void sum_partials_kernel(const double* partial, int n, double* out)
+{
+    sum_partials_body(partial, n, out);
+}
+
+// Batched objective: OBJ_BATCH_BLOCKS workgroups per item.
+__global__ __launch_bounds__(OBJ_THREADS) void objective_batch_kernel(const BatchObjItem* __restrict__ items)
+{
+    const int k = blockIdx.x / OBJ_BATCH_BLOCKS;
+    const BatchObjItem it = items[k];
+    objective_body(it.D, it.Dp, it.wv, it.n, it.R, it.part, blockIdx.x % OBJ_BATCH_BLOCKS, OBJ_BATCH_BLOCKS);
+}
+
+// J of every item (one workgroup per item)
+__global__ __launch_bounds__(OBJ_THREADS) void sum_partials_batch_kernel(const BatchObjItem* __restrict__ items)
+{
+    const BatchObjItem it = items[blockIdx.x];
+    sum_partials_body(it.part, OBJ_BATCH_BLOCKS, it.j_out);
 }
 
 // out[l] = sum_z (g_idd[l,z] didd[l,z] + g_sig[l,z] dsig[l,z]) (one wave per layer, fixed order)
@@ -324,11 +369,9 @@ __global__ void lut_chain_kernel(const double* g_idd, const double* g_sig, const
 
 using namespace pbh;
 
-FwdArgs make_fwd_args(const PbhGrid* g, const PbhLattice* lat, const float* w,
-                      const double* idd, const double* sigma, float* dose);
 extern "C" int pbh_validate(const PbhGrid* g, const PbhLattice* lat);
 
-static int adj_split(const PbhGrid* g, const PbhLattice* lat)
+int pbh_adj_default_split(const PbhGrid* g, const PbhLattice* lat)
 {
     int s = 1;
     while (s < 16 && (long long)g->nz * s < 512 && s * 2 <= lat->n_layers) s *= 2;
@@ -359,6 +402,29 @@ static bool adj_axis(double dv, double ds, int T, int K, AxisTable* out)
     return false;
 }
 
+// Adjoint arguments of one problem; split 0 = default (split in 1, 2, 4, 8, 16, at most L).
+int pbh_make_adj_args(const PbhGrid* g, const PbhLattice* lat, const float* w, const double* idd,
+                      const double* sigma, const float* R, float* part, double* grad_idd,
+                      double* grad_sigma, int split, AdjArgs* out)
+{
+    if (split == 0) split = pbh_adj_default_split(g, lat);
+    if (split != 1 && split != 2 && split != 4 && split != 8 && split != 16) return PBH_ERR_UNSUPPORTED;
+    if ((long long)g->nz * split > 0x7fffffffLL) return PBH_ERR_UNSUPPORTED;
+    AdjArgs& A = *out;
+    A.f = make_fwd_args(g, lat, w, idd, sigma, nullptr);
+    A.R = R;
+    A.part = part;
+    A.grad_idd = grad_idd;
+    A.grad_sigma = grad_sigma;
+    A.split = split;
+    A.tab = adj_axis(g->dx, lat->dx_spot, AJ_TX, AJ_SC, &A.tx) && adj_axis(g->dy, lat->dy_spot, AJ_TY, AJ_SC, &A.ty);
+    if (!A.tab) {
+        A.tx = AxisTable{0, 0, 0.0, 0};
+        A.ty = AxisTable{0, 0, 0.0, 0};
+    }
+    return PBH_OK;
+}
+
 extern "C" int pbh_adjoint(const PbhGrid* g, const PbhLattice* lat, const float* w, const double* idd,
                            const double* sigma, const float* R, float* grad_w, double* grad_idd,
                            double* grad_sigma, int precision, void* workspace, size_t workspace_bytes,
@@ -371,17 +437,8 @@ extern "C" int pbh_adjoint(const PbhGrid* g, const PbhLattice* lat, const float*
     if (workspace_bytes < pbh_adjoint_workspace_bytes(g, lat, precision)) return PBH_ERR_WORKSPACE;
     hipStream_t s = (hipStream_t)stream;
     AdjArgs A;
-    A.f = make_fwd_args(g, lat, w, idd, sigma, nullptr);
-    A.R = R;
-    A.part = (float*)workspace;
-    A.grad_idd = grad_idd;
-    A.grad_sigma = grad_sigma;
-    A.split = adj_split(g, lat);
-    A.tab = adj_axis(g->dx, lat->dx_spot, AJ_TX, AJ_SC, &A.tx) && adj_axis(g->dy, lat->dy_spot, AJ_TY, AJ_SC, &A.ty);
-    if (!A.tab) {
-        A.tx = AxisTable{0, 0, 0.0, 0};
-        A.ty = AxisTable{0, 0, 0.0, 0};
-    }
+    rc = pbh_make_adj_args(g, lat, w, idd, sigma, R, (float*)workspace, grad_idd, grad_sigma, 0, &A);
+    if (rc != PBH_OK) return rc;
     const dim3 grid((unsigned)(g->nz * A.split)), block(AJ_THREADS);
     const bool sig = grad_sigma != nullptr;
     if (A.tab) {
@@ -430,4 +487,34 @@ extern "C" int pbh_lut_chain(const double* grad_idd, const double* grad_sigma, c
     hipLaunchKernelGGL(lut_chain_kernel, dim3(n_layers), dim3(64), 0, (hipStream_t)stream, grad_idd, grad_sigma,
                        didd, dsigma, n_layers, nz, out);
     return launch_rc("lut_chain_kernel");
+}
+
+// ---- batched launches (robust-scenario batches) ----
+int pbh_launch_adj_batch(const AdjArgs* items, const int* wg0, int n, int total, bool tab, bool sig,
+                         hipStream_t s)
+{
+    const dim3 grid((unsigned)total), block(AJ_THREADS);
+    if (tab) {
+        if (sig) hipLaunchKernelGGL((adj_fp32_batch_kernel<true, true>), grid, block, 0, s, items, wg0, n);
+        else hipLaunchKernelGGL((adj_fp32_batch_kernel<true, false>), grid, block, 0, s, items, wg0, n);
+    } else {
+        if (sig) hipLaunchKernelGGL((adj_fp32_batch_kernel<false, true>), grid, block, 0, s, items, wg0, n);
+        else hipLaunchKernelGGL((adj_fp32_batch_kernel<false, false>), grid, block, 0, s, items, wg0, n);
+    }
+    return launch_rc("adj_fp32_batch_kernel");
+}
+
+int pbh_launch_obj_batch(const BatchObjItem* items, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(objective_batch_kernel, dim3((unsigned)(n * OBJ_BATCH_BLOCKS)), dim3(OBJ_THREADS), 0, s, items);
+    int rc = launch_rc("objective_batch_kernel");
+    if (rc != PBH_OK) return rc;
+    hipLaunchKernelGGL(sum_partials_batch_kernel, dim3((unsigned)n), dim3(OBJ_THREADS), 0, s, items);
+    return launch_rc("sum_partials_batch_kernel");
+}
+
+int pbh_launch_reduce_batch(const BatchRedItem* items, const int* wg0, int n, int total, hipStream_t s)
+{
+    hipLaunchKernelGGL(adj_reduce_batch_kernel, dim3((unsigned)total), dim3(256), 0, s, items, wg0, n);
+    return launch_rc("adj_reduce_batch_kernel");
 }

@@ -1,0 +1,60 @@
+// pbh_batch.h -- descriptors and launchers shared by the single-problem adjoint
+// and the batched (robust-scenario) entry points.
+#pragma once
+
+#include "pbh_fwd_core.h"
+
+namespace pbh {
+
+// Adjoint of one problem (pbh_adjoint.hip).
+struct AdjArgs {
+    FwdArgs f;
+    const float* R;        // (nz, ny, nx)
+    float* part;           // [nz][L][nsy][nsx] per-slice c P
+    double* grad_idd;      // (L, nz) or null
+    double* grad_sigma;    // (L, nz) or null
+    int split;
+    int tab;
+    AxisTable tx, ty;
+};
+
+// Weighted-quadratic objective of one problem: R = 2 wv (D - Dp), partial sums
+// of wv (D - Dp)^2 per workgroup, J = their fixed-order sum.
+struct BatchObjItem {
+    const float* D;
+    const float* Dp;
+    const float* wv;       // null: weights 1
+    float* R;
+    double* part;          // [OBJ_BATCH_BLOCKS]
+    double* j_out;
+    size_t n;
+};
+
+// Depth reduction of one problem's per-slice weight-gradient partials.
+struct BatchRedItem {
+    const float* part;     // [nz][n]
+    float* grad_w;         // [n]
+    size_t n;
+    int nz;
+};
+
+constexpr int OBJ_BATCH_BLOCKS = 256;   // objective workgroups per item
+
+}  // namespace pbh
+
+// Host-side builders and launchers (pbh_forward.hip, pbh_mixed.hip, pbh_adjoint.hip).
+pbh::FwdArgs make_fwd_args(const PbhGrid* g, const PbhLattice* lat, const float* w,
+                           const double* idd, const double* sigma, float* dose);
+int pbh_make_adj_args(const PbhGrid* g, const PbhLattice* lat, const float* w, const double* idd,
+                      const double* sigma, const float* R, float* part, double* grad_idd,
+                      double* grad_sigma, int split, pbh::AdjArgs* out);
+int pbh_adj_default_split(const PbhGrid* g, const PbhLattice* lat);
+int pbh_launch_fwd_fp32_batch(const pbh::BatchFwdItem* items, const int* wg0, int n, int total, int scp,
+                              bool tab, hipStream_t s);
+int pbh_launch_fwd_mixed_batch(const pbh::BatchFwdItem* items, const int* wg0, int n, int total,
+                               int precision, int kb, bool tab, hipStream_t s);
+int pbh_launch_peak_batch(const pbh::BatchPeakItem* items, int n, hipStream_t s);
+int pbh_launch_adj_batch(const pbh::AdjArgs* items, const int* wg0, int n, int total, bool tab, bool sig,
+                         hipStream_t s);
+int pbh_launch_obj_batch(const pbh::BatchObjItem* items, int n, hipStream_t s);
+int pbh_launch_reduce_batch(const pbh::BatchRedItem* items, const int* wg0, int n, int total, hipStream_t s);

@@ -35,9 +35,10 @@ namespace pbh {
 // Stage A: 16x16 blocks of U (2 row blocks x NCB column blocks); wave w takes
 // block (w % 2, w / 2) and, when NCB = 3, block (w % 2, 2) for w < 2.
 // Stage B: wave w owns the 32 x 32 dose block at columns 32 w.
+//
+// blk: the workgroup's index in the problem's own grid (tile * split + share).
 template <bool TAB, int SCP>
-__global__ __launch_bounds__(FT_THREADS, 2)
-void fwd_fp32_kernel(FwdArgs a, FwdPlan pl)
+__device__ __forceinline__ void fwd_fp32_body(const FwdArgs& a, const FwdPlan& pl, int blk)
 {
     constexpr int KA = SCP / 4;            // stage A K steps
     constexpr int KB = SCP / 2;            // stage B K steps
@@ -54,8 +55,8 @@ void fwd_fp32_kernel(FwdArgs a, FwdPlan pl)
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int s = blockIdx.x % pl.split;
-    const int tile = blockIdx.x / pl.split;
+    const int s = blk % pl.split;
+    const int tile = blk / pl.split;
     const int tx = tile % a.tiles_x;
     const int ty = (tile / a.tiles_x) % a.tiles_y;
     const int z = tile / (a.tiles_x * a.tiles_y);
@@ -207,7 +208,25 @@ void fwd_fp32_kernel(FwdArgs a, FwdPlan pl)
             }
         }
     }
-    store_block(a, pl, tile, s, z, y0, x0, 0, cnB, acc);
+    store_block(a, pl, tile, s, z, y0, x0, 0, cnB, acc, (size_t)a.tiles_x * a.tiles_y * a.nz);
+}
+
+template <bool TAB, int SCP>
+__global__ __launch_bounds__(FT_THREADS, 2)
+void fwd_fp32_kernel(FwdArgs a, FwdPlan pl)
+{
+    fwd_fp32_body<TAB, SCP>(a, pl, blockIdx.x);
+}
+
+// One launch over a batch of problems sharing the template parameters.
+template <bool TAB, int SCP>
+__global__ __launch_bounds__(FT_THREADS, 2)
+void fwd_fp32_batch_kernel(const BatchFwdItem* __restrict__ items, const int* __restrict__ wg0, int n)
+{
+    const int k = batch_item(wg0, n, blockIdx.x);
+    const FwdArgs a = items[k].a;
+    const FwdPlan pl = items[k].pl;
+    fwd_fp32_body<TAB, SCP>(a, pl, (int)blockIdx.x - wg0[k]);
 }
 
 }  // namespace pbh
@@ -385,4 +404,24 @@ extern "C" int pbh_forward(const PbhGrid* g, const PbhLattice* lat, const float*
 {
     return pbh_forward_tuned(g, lat, w, idd, sigma, dose, precision, 0, workspace,
                              workspace_bytes, stream);
+}
+
+// Batched FP32 forward: items[0..n) share SCP (their padded chunk width) and the
+// table mode; total = wg0[n].
+int pbh_launch_fwd_fp32_batch(const BatchFwdItem* items, const int* wg0, int n, int total, int scp,
+                              bool tab, hipStream_t s)
+{
+    const dim3 grid((unsigned)total), block(FT_THREADS);
+    switch (scp) {
+#define PBH_FP32_BATCH_CASE(N) \
+    case N: \
+        if (tab) hipLaunchKernelGGL((fwd_fp32_batch_kernel<true, N>), grid, block, 0, s, items, wg0, n); \
+        else hipLaunchKernelGGL((fwd_fp32_batch_kernel<false, N>), grid, block, 0, s, items, wg0, n); \
+        break;
+    PBH_FP32_BATCH_CASE(16) PBH_FP32_BATCH_CASE(24) PBH_FP32_BATCH_CASE(32)
+    PBH_FP32_BATCH_CASE(40) PBH_FP32_BATCH_CASE(48)
+#undef PBH_FP32_BATCH_CASE
+    default: return PBH_ERR_UNSUPPORTED;
+    }
+    return launch_rc("fwd_fp32_batch_kernel");
 }

@@ -54,13 +54,12 @@ __device__ __forceinline__ double table_distance(const AxisTable& g, double C, i
 // (S in {2, 4, 8}), so a counter is a multiple of 8 between calls whatever S
 // was used, and the last arrival is the one that brings it to a multiple of 8.
 __device__ inline void store_block(const FwdArgs& a, const FwdPlan& pl, int tile, int s,
-                                   int z, int y0, int x0, int rm, int cn, v16f acc)
+                                   int z, int y0, int x0, int rm, int cn, v16f acc, size_t tiles)
 {
     __shared__ int last_flag;
     const int lane = threadIdx.x & 63;
     const int xl = cn + (lane & 31);
     if (pl.split > 1) {
-        const size_t tiles = gridDim.x / pl.split;
         float* mine = pl.part + ((size_t)s * tiles + tile) * (FT_TY * FT_TX);
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
@@ -167,6 +166,38 @@ __host__ __device__ inline int chunk_pad(int nsx, int nsy)
     for (int c = 0; c < cy; ++c) m = max(m, chunk_begin(nsy, cy, c + 1) - chunk_begin(nsy, cy, c));
     return (m + 3) & ~3;
 }
+
+// ---- batches of independent problems (robust scenarios) ----
+// The workgroups of one batched launch are split among its items by a prefix
+// table: item k owns workgroups wg0[k] .. wg0[k+1]-1 and runs them as the
+// single-problem kernel would run blocks 0 .. wg0[k+1]-wg0[k]-1.
+
+// Largest k with wg0[k] <= blk (wg0 ascending, wg0[0] = 0). blk is uniform in
+// the workgroup, so the search runs on scalar loads.
+__device__ __forceinline__ int batch_item(const int* __restrict__ wg0, int n, int blk)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (wg0[mid] <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+struct BatchFwdItem {
+    FwdArgs a;
+    FwdPlan pl;
+    double oz, dz;
+    const float* r_peak;   // per-layer Bragg-peak depth ("_peak" forms) or null
+};
+
+struct BatchPeakItem {
+    const double* idd;
+    int L, nz;
+    double oz, dz;
+    float* r_peak;
+};
 
 }  // namespace pbh
 

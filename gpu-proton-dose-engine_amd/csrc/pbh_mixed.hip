@@ -91,9 +91,11 @@ __global__ void peak_depth_f32_kernel(const double* idd, int L, int nz, double o
 // of U (2 row blocks x 3 column blocks); wave w takes block (w % 2, w / 2) and,
 // for w < 2, block (w % 2, 2). Stage B: wave w owns the 32 x 32 dose block at
 // columns 32 w; KB K steps of 16 spots.
+//
+// blk: the workgroup's index in the problem's own grid (tile * split + share).
 template <typename T, int MODE, bool TAB, int KB>
-__global__ __launch_bounds__(FT_THREADS, 2)
-void fwd_mixed_kernel(FwdArgs a, FwdPlan pl, double oz, double dz, const float* r_peak)
+__device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& pl, double oz, double dz,
+                                               const float* r_peak, int blk)
 {
     typedef typename Vec8<T>::type V8;
     constexpr bool F16 = !__is_same(T, __bf16);
@@ -113,8 +115,8 @@ void fwd_mixed_kernel(FwdArgs a, FwdPlan pl, double oz, double dz, const float* 
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int s = blockIdx.x % pl.split;
-    const int tile = blockIdx.x / pl.split;
+    const int s = blk % pl.split;
+    const int tile = blk / pl.split;
     const int tx = tile % a.tiles_x;
     const int ty = (tile / a.tiles_x) % a.tiles_y;
     const int z = tile / (a.tiles_x * a.tiles_y);
@@ -356,7 +358,55 @@ void fwd_mixed_kernel(FwdArgs a, FwdPlan pl, double oz, double dz, const float* 
             }
         }
     }
-    store_block(a, pl, tile, s, z, y0, x0, 0, cnB, acc);
+    store_block(a, pl, tile, s, z, y0, x0, 0, cnB, acc, (size_t)a.tiles_x * a.tiles_y * a.nz);
+}
+
+template <typename T, int MODE, bool TAB, int KB>
+__global__ __launch_bounds__(FT_THREADS, 2)
+void fwd_mixed_kernel(FwdArgs a, FwdPlan pl, double oz, double dz, const float* r_peak)
+{
+    fwd_mixed_body<T, MODE, TAB, KB>(a, pl, oz, dz, r_peak, blockIdx.x);
+}
+
+// One launch over a batch of problems sharing the template parameters.
+template <typename T, int MODE, bool TAB, int KB>
+__global__ __launch_bounds__(FT_THREADS, 2)
+void fwd_mixed_batch_kernel(const BatchFwdItem* __restrict__ items, const int* __restrict__ wg0, int n)
+{
+    const int k = batch_item(wg0, n, blockIdx.x);
+    const FwdArgs a = items[k].a;
+    const FwdPlan pl = items[k].pl;
+    fwd_mixed_body<T, MODE, TAB, KB>(a, pl, items[k].oz, items[k].dz, items[k].r_peak,
+                                     (int)blockIdx.x - wg0[k]);
+}
+
+// r_peak of every layer of every item (one workgroup per item)
+__global__ void peak_depth_batch_kernel(const BatchPeakItem* __restrict__ items)
+{
+    const BatchPeakItem it = items[blockIdx.x];
+    for (int l = threadIdx.x; l < it.L; l += blockDim.x) {
+        const double* row = it.idd + (size_t)l * it.nz;
+        int best = 0;
+        for (int k = 1; k < it.nz; ++k)
+            if (row[k] > row[best]) best = k;
+        it.r_peak[l] = (float)(it.oz + (double)best * it.dz);
+    }
+}
+
+template <typename T, int MODE>
+static void launch_mixed_batch(const BatchFwdItem* items, const int* wg0, int n, dim3 grid, int kb,
+                               bool tab, hipStream_t s)
+{
+    const dim3 block(FT_THREADS);
+    if (!tab) {
+        hipLaunchKernelGGL((fwd_mixed_batch_kernel<T, MODE, false, 3>), grid, block, 0, s, items, wg0, n);
+        return;
+    }
+    switch (kb) {
+    case 1: hipLaunchKernelGGL((fwd_mixed_batch_kernel<T, MODE, true, 1>), grid, block, 0, s, items, wg0, n); break;
+    case 2: hipLaunchKernelGGL((fwd_mixed_batch_kernel<T, MODE, true, 2>), grid, block, 0, s, items, wg0, n); break;
+    default: hipLaunchKernelGGL((fwd_mixed_batch_kernel<T, MODE, true, 3>), grid, block, 0, s, items, wg0, n); break;
+    }
 }
 
 template <typename T, int MODE>
@@ -410,4 +460,29 @@ int pbh_forward_mixed(const FwdArgs& a, double oz, double dz, int precision, int
     }
     if (rc != PBH_OK) return rc;
     return launch_rc("fwd_mixed_kernel");
+}
+
+int pbh_launch_peak_batch(const BatchPeakItem* items, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(peak_depth_batch_kernel, dim3((unsigned)n), dim3(64), 0, s, items);
+    return launch_rc("peak_depth_batch_kernel");
+}
+
+// Batched mixed-precision forward: items[0..n) share kb (stage B K steps) and
+// the table mode; total = wg0[n].
+int pbh_launch_fwd_mixed_batch(const BatchFwdItem* items, const int* wg0, int n, int total, int precision,
+                               int kb, bool tab, hipStream_t s)
+{
+    if (kb < 1 || kb > 3) return PBH_ERR_UNSUPPORTED;
+    const dim3 grid((unsigned)total);
+    switch (precision) {
+    case PBH_PREC_BF16: launch_mixed_batch<__bf16, MODE_PLAIN>(items, wg0, n, grid, kb, tab, s); break;
+    case PBH_PREC_FP16: launch_mixed_batch<_Float16, MODE_PLAIN>(items, wg0, n, grid, kb, tab, s); break;
+    case PBH_PREC_BF16X3: launch_mixed_batch<__bf16, MODE_X3>(items, wg0, n, grid, kb, tab, s); break;
+    case PBH_PREC_FP16X3: launch_mixed_batch<_Float16, MODE_X3>(items, wg0, n, grid, kb, tab, s); break;
+    case PBH_PREC_BF16X3_PEAK: launch_mixed_batch<__bf16, MODE_PEAK>(items, wg0, n, grid, kb, tab, s); break;
+    case PBH_PREC_FP16X3_PEAK: launch_mixed_batch<_Float16, MODE_PEAK>(items, wg0, n, grid, kb, tab, s); break;
+    default: return PBH_ERR_UNSUPPORTED;
+    }
+    return launch_rc("fwd_mixed_batch_kernel");
 }

@@ -38,6 +38,18 @@ class PbhBeamModel(C.Structure):
                 ("icru_coef", C.c_double), ("icru_exp", C.c_double)]
 
 
+class PbhBatchItem(C.Structure):
+    _fields_ = [("grid", PbhGrid), ("lattice", PbhLattice),
+                ("w", C.c_void_p), ("idd", C.c_void_p), ("sigma", C.c_void_p), ("dose", C.c_void_p),
+                ("presc", C.c_void_p), ("vox_w", C.c_void_p), ("j_out", C.c_void_p),
+                ("grad_w", C.c_void_p), ("grad_idd", C.c_void_p), ("grad_sigma", C.c_void_p)]
+
+
+class PbhBatchTuning(C.Structure):
+    _fields_ = [("fwd_split", C.c_int32), ("adj_split", C.c_int32), ("group", C.c_int32),
+                ("order", C.c_int32)]
+
+
 _P = C.c_void_p
 _G = C.POINTER(PbhGrid)
 _L = C.POINTER(PbhLattice)
@@ -62,6 +74,13 @@ SIGNATURES = {
     "pbh_objective_workspace_bytes": (C.c_size_t, [_G]),
     "pbh_objective": (C.c_int, [_G, _P, _P, _P, _P, _P, _P, C.c_size_t, _P]),
     "pbh_lut_chain": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_int, _P, _P]),
+    "pbh_batch_workspace_bytes": (C.c_size_t, [C.POINTER(PbhBatchItem), C.c_int, C.c_int,
+                                               C.POINTER(PbhBatchTuning)]),
+    "pbh_batch_create": (C.c_int, [C.POINTER(PbhBatchItem), C.c_int, C.c_int, C.POINTER(PbhBatchTuning),
+                                   _P, C.c_size_t, _P, C.POINTER(C.c_void_p)]),
+    "pbh_batch_forward_adjoint": (C.c_int, [_P, _P]),
+    "pbh_batch_launch_count": (C.c_int, [_P]),
+    "pbh_batch_destroy": (None, [_P]),
     "pbh_gamma_workspace_bytes": (C.c_size_t, [_G]),
     "pbh_gamma_index": (C.c_int, [_G, _P, _P, C.c_double, C.c_double, C.c_double, C.c_double,
                                   _P, _P, _P, C.c_size_t, _P]),

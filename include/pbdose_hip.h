@@ -184,6 +184,65 @@ int pbh_lut_chain(const double* grad_idd, const double* grad_sigma, const double
                   const double* dsigma, int n_layers, int nz, double* out, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Robust-scenario batches: many independent problems (fields x setup / range
+ * scenarios; grids and lattices may differ per item) computed by one launch
+ * per kernel stage: [peak depths] -> forward -> objective -> adjoint -> depth
+ * reduction. A setup shift is expressed by the caller as a shifted lattice
+ * origin (x0, y0) or depth origin (oz), a range scaling as tables built for the
+ * scaled energies, so every item is an ordinary problem.
+ * No counterpart in the reference (its engines run one field at a time,
+ * engines.py:208-283).
+ * ------------------------------------------------------------------------- */
+typedef struct PbhBatchItem {
+    PbhGrid grid;
+    PbhLattice lattice;
+    const float* w;          /* (L, nsy, nsx) spot weights */
+    const double* idd;       /* (L, nz) */
+    const double* sigma;     /* (L, nz) */
+    float* dose;             /* (nz, ny, nx) out */
+    const float* presc;      /* (nz, ny, nx) prescription, or NULL: forward only */
+    const float* vox_w;      /* (nz, ny, nx) voxel weights, or NULL: all 1 */
+    double* j_out;           /* J = sum vox_w (dose - presc)^2 (one FP64), or NULL */
+    float* grad_w;           /* (L, nsy, nsx) dJ/dw out; required when presc != NULL */
+    double* grad_idd;        /* (L, nz) dJ/dIDD out, or NULL */
+    double* grad_sigma;      /* (L, nz) dJ/dsigma out, or NULL */
+} PbhBatchItem;
+
+typedef struct PbhBatchTuning {
+    int32_t fwd_split;       /* forward layer split 1, 2, 4 or 8 (0 = 1) */
+    int32_t adj_split;       /* adjoint layer split 1, 2, 4, 8 or 16 (0 = per-item default) */
+    int32_t group;           /* 0: one forward launch at the widest spot chunk;
+                                1: one forward launch per chunk width */
+    int32_t order;           /* 0: items in the given order; 1: largest work first */
+} PbhBatchTuning;
+
+typedef struct PbhBatch PbhBatch;   /* opaque host-side plan (host memory only) */
+
+/* Workspace of a batch (descriptor tables, residuals, per-slice partials).
+ * tuning may be NULL (all fields 0). Returns 0 for invalid input. */
+size_t pbh_batch_workspace_bytes(const PbhBatchItem* items, int n_items, int precision,
+                                 const PbhBatchTuning* tuning);
+
+/* Validates the items, plans the launches and copies the device descriptor
+ * tables into the workspace (zero-filled by the caller before this call); it
+ * synchronises `stream` once. The items' device arrays must stay valid until
+ * pbh_batch_destroy. */
+int pbh_batch_create(const PbhBatchItem* items, int n_items, int precision,
+                     const PbhBatchTuning* tuning, void* workspace, size_t workspace_bytes,
+                     void* stream, PbhBatch** out);
+
+/* Runs the whole batch on `stream`: kernel launches only (no copies, memsets
+ * or allocations), so it can be captured into a HIP graph and replayed.
+ * Deterministic: results equal those of the single-problem entry points with
+ * the same layer splits. Calls on one batch must not overlap. */
+int pbh_batch_forward_adjoint(const PbhBatch* batch, void* stream);
+
+/* Kernel launches one pbh_batch_forward_adjoint call makes. */
+int pbh_batch_launch_count(const PbhBatch* batch);
+
+void pbh_batch_destroy(PbhBatch* batch);
+
+/* ---------------------------------------------------------------------------
  * Gamma index (global, integer-voxel DTA offsets). gamma.py:81-160
  * ------------------------------------------------------------------------- */
 size_t pbh_gamma_workspace_bytes(const PbhGrid* g);

@@ -82,3 +82,67 @@ def test_product_refuses_cpu_device():
     prob = model.PBProblem(g, lat, np.ones((1, 4)), np.ones((1, 4)))
     with pytest.raises(ValueError, match="GPU only"):
         engines.torch_separable(prob, "cpu")
+
+
+def _batch_items(n=3):
+    """Host descriptors with stand-in pointers (never dereferenced: no kernel runs)."""
+    arr = (_lib.PbhBatchItem * n)()
+    for k in range(n):
+        it = arr[k]
+        it.grid = _lib.PbhGrid(64, 48, 32, 0, 0.2, 0.2, 0.2, -6.3, -4.7, 0.0)
+        it.lattice = _lib.PbhLattice(9 + k, 7, 2 + k, 0, 0.5, 0.5, 0.3 * k, 0.0)
+        it.w = it.idd = it.sigma = it.dose = 256
+        it.presc = 256 if k != 1 else None
+        it.grad_w = 256
+    return arr
+
+
+def test_batch_workspace_query_and_argument_checks():
+    L = _lib.lib()
+    items = _batch_items()
+    t = _lib.PbhBatchTuning(1, 0, 1, 1)
+    n = L.pbh_batch_workspace_bytes(items, 3, 3, C.byref(t))
+    # at least: the residual and per-slice partials of the two items with a prescription
+    need = sum(64 * 48 * 32 * 4 + 32 * (2 + k) * 7 * (9 + k) * 4 for k in (0, 2))
+    assert n >= need
+    assert L.pbh_batch_workspace_bytes(items, 3, 3, None) > 0          # default tuning
+    assert L.pbh_batch_workspace_bytes(items, 0, 3, None) == 0          # empty batch
+    assert L.pbh_batch_workspace_bytes(items, 3, 99, None) == 0         # bad precision
+    bad = _lib.PbhBatchTuning(3, 0, 0, 0)                               # split 3
+    assert L.pbh_batch_workspace_bytes(items, 3, 3, C.byref(bad)) == 0
+    # a forward split needs room for its partial tiles
+    t8 = _lib.PbhBatchTuning(8, 0, 1, 1)
+    assert L.pbh_batch_workspace_bytes(items, 3, 3, C.byref(t8)) > n
+    items[1].dose = None
+    assert L.pbh_batch_workspace_bytes(items, 3, 3, None) == 0
+    items[1].dose = 256
+    items[0].grad_w = None                                              # objective without a gradient
+    assert L.pbh_batch_workspace_bytes(items, 3, 3, None) == 0
+
+
+def test_batch_create_rejects_before_touching_the_device():
+    L = _lib.lib()
+    items = _batch_items()
+    out = C.c_void_p()
+    t = _lib.PbhBatchTuning(1, 0, 1, 1)
+    assert L.pbh_batch_create(items, 3, 3, C.byref(t), None, 0, None, C.byref(out)) == -1
+    assert L.pbh_batch_create(items, 3, 3, C.byref(t), C.c_void_p(256), 16, None, C.byref(out)) == -7
+    assert L.pbh_batch_create(items, 3, 3, C.byref(t), C.c_void_p(256), 16, None, None) == -1
+    assert not out.value
+    assert L.pbh_batch_forward_adjoint(None, None) == -1
+    assert L.pbh_batch_launch_count(None) == -1
+    L.pbh_batch_destroy(None)
+
+
+def test_standard_scenarios_and_fields():
+    from pbdose import robust
+    sc = robust.standard_scenarios()
+    assert len(sc) == 21 and len(set(sc)) == 21
+    assert robust.Scenario() in sc
+    assert sorted({s.range_scale for s in sc}) == [0.965, 1.0, 1.035]
+    assert sum(1 for s in sc if (s.dx, s.dy, s.dz) == (0, 0, 0)) == 3
+    assert all(abs(s.dx) + abs(s.dy) + abs(s.dz) in (0.0, 0.3) for s in sc)
+    f = robust.robust_fields()
+    assert [(x.lattice.n_sx, x.lattice.n_sy, x.lattice.n_layers) for x in f] == [(31, 31, 25), (41, 41, 40), (27, 35, 45)]
+    assert sum(x.lattice.n_spots for x in f) == 24025 + 67240 + 42525
+    assert f[2].lattice.weights.shape == (45, 35, 27)
