@@ -14,7 +14,9 @@ against the FP64 oracle are reported beside it. ``roofline`` is for the dominant
 kernel (the fused forward of the selected form), timed per launch with HIP
 events on the launch stream. ``cpu_baseline`` times the host restatement of the
 reference's PyTorch batched-matmul engine (oracle/torch_cpu.py, FP64, all host
-threads) on full forwards of the same field.
+threads) on full forwards of the same field. ``robust_batch`` times the
+robust-scenario workload (3 fields x 21 scenarios, 63 field-scenarios per graph
+replay), forward only and forward + adjoint, with autotuned launch parameters.
 """
 from __future__ import annotations
 
@@ -112,6 +114,31 @@ def time_adjoint(dp, D, steps, warmup):
     return float(np.mean([s.elapsed_time(e) for s, e in ev]))
 
 
+def time_robust(precision, steps, objective):
+    """Robust-scenario batch (3 fields x 21 scenarios on 128^3), one graph replay
+    per step; launch parameters from the measured autotuner."""
+    from pbdose import robust
+    b = robust.RobustBatch(robust.robust_fields(), robust.standard_scenarios(), precision=precision,
+                           objective=objective, tuning="auto")
+    b.capture()
+    for _ in range(3):
+        b.run()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(steps):
+        b.run()
+    e.record()
+    torch.cuda.synchronize()
+    ms = s.elapsed_time(e) / steps
+    flop = b.flops() * (2.0 if objective else 1.0)
+    out = {"items": len(b.items), "ms": ms, "field_scenarios_per_s": len(b.items) * 1e3 / ms,
+           "algorithmic_tflop_per_s": flop / (ms * 1e-3) / 1e12, "kernel_launches": b.launch_count,
+           "tuning": b.tuning}
+    b.close()
+    return out
+
+
 def cpu_baseline(prob, budget_s=20.0, max_reps=15):
     from oracle.torch_cpu import torch_separable_cpu
     nthreads = os.cpu_count() or 1
@@ -140,6 +167,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--precision", default="bf16x3")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-robust", action="store_true", help="skip the robust-scenario batch leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build runs the "
@@ -197,6 +225,14 @@ def main():
                      "frac": achieved / peak, "traffic": None,
                      "algorithmic_flop_per_launch": flops, "launch_ms": launch_ms},
     }
+    if not args.no_robust:
+        rs = max(5, args.steps // 10)
+        line["robust_batch"] = {
+            "workload": "fields 31x31x25, 41x41x40, 27x35x45 (nsx x nsy x layers) x 21 scenarios "
+                        "(nominal, +-3 mm x/y/z, range x {1, 0.965, 1.035}) on water 128^3 @ 2 mm; "
+                        "one HIP graph replay per step",
+            "forward": time_robust(args.precision, rs, objective=False),
+            "forward_adjoint": time_robust(args.precision, rs, objective=True)}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(prob)
     print(json.dumps(line), flush=True)

@@ -82,8 +82,9 @@ SIGNATURES = {
     "pbh_batch_launch_count": (C.c_int, [_P]),
     "pbh_batch_destroy": (None, [_P]),
     "pbh_gamma_workspace_bytes": (C.c_size_t, [_G]),
-    "pbh_gamma_index": (C.c_int, [_G, _P, _P, C.c_double, C.c_double, C.c_double, C.c_double,
-                                  _P, _P, _P, C.c_size_t, _P]),
+    "pbh_gamma_count_offsets": (C.c_int, [_G, C.c_double]),
+    "pbh_gamma_index": (C.c_int, [_G, _P, C.c_int, _P, C.c_int, C.c_double, C.c_double, C.c_double,
+                                  C.c_double, C.c_int, _P, _P, _P, _P, C.c_size_t, _P]),
 }
 
 _lib = None

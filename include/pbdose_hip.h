@@ -27,7 +27,8 @@
  *   physics.py:328-387  idd_pristine(normalize=False)
  *   physics.py:221-286  fermi_eyges_moments(...)["sigma_x"]      -> pbh_build_lut
  *   model.py:225-233    per-layer LUT loop of the field builders  -> pbh_build_lut
- *   gamma.py:81-160     gamma_index (global)                     -> pbh_gamma_index
+ *   gamma.py:41-160     dta_offsets / gamma_index                -> pbh_gamma_index
+ *   gamma.py:213-261    gamma_index_torch                        -> pbh_gamma_index
  *   pencil_beam.cu:647-699 PB_ERR_* codes and pb_strerror         -> PBH_ERR_*, pbh_strerror
  *   pencil_beam.cu:717-730 pb_problem_validate                    -> pbh_validate
  *   pencil_beam.cu:907-919 pb_problem_bytes (sizing query)        -> *_workspace_bytes
@@ -243,15 +244,31 @@ int pbh_batch_launch_count(const PbhBatch* batch);
 void pbh_batch_destroy(PbhBatch* batch);
 
 /* ---------------------------------------------------------------------------
- * Gamma index (global, integer-voxel DTA offsets). gamma.py:81-160
+ * Gamma index with integer-voxel DTA offsets. gamma.py:41-160 (gamma_index);
+ * also replaces gamma.py:213-261 (gamma_index_torch, which evaluates in FP32:
+ * here the arithmetic is FP64 for both).
  * ------------------------------------------------------------------------- */
+#define PBH_DTYPE_F32 0
+#define PBH_DTYPE_F64 1
+
 size_t pbh_gamma_workspace_bytes(const PbhGrid* g);
-/* counts[0] = voxels evaluated (ref >= thr * ref_max), counts[1] = voxels with
- * gamma <= 1; gamma_out (nz, ny, nx) FP32 may be NULL. ref_max <= 0 means
- * "use max(ref)". */
-int pbh_gamma_index(const PbhGrid* g, const float* ref, const float* eval,
-                    double dose_pct, double dta_mm, double threshold_pct,
-                    double ref_max, float* gamma_out, unsigned long long* counts,
+
+/* Number of DTA offsets for the grid spacing (dz, dy, dx) and dta_mm
+ * (gamma.py:41-60: per axis at most min(floor(DTA/d + 1e-9), 6) voxels, kept
+ * if |offset| <= DTA + 1e-12). Host-only. */
+int pbh_gamma_count_offsets(const PbhGrid* g, double dta_mm);
+
+/* ref, eval: (nz, ny, nx) doses of dtype PBH_DTYPE_F32 / _F64 on the grid g
+ * (only its sizes and spacings are used). dose_pct, dta_mm, threshold_pct as
+ * in gamma.py:81-86; ref_max <= 0 means max(ref) (computed on the device);
+ * local != 0 selects local normalisation (crit = dose% * max(ref, 1e-12)).
+ * Writes (device memory) counts[3] = {n_eval, n_pass, n_dta_offsets} and
+ * stats[3] = {gamma_max, gamma_mean, ref_max used}; gamma_out (nz, ny, nx)
+ * FP64 may be NULL. Evaluated voxels: ref >= threshold% * ref_max, or ref > 0
+ * if there are none (gamma.py:139-141). Deterministic. */
+int pbh_gamma_index(const PbhGrid* g, const void* ref, int ref_dtype, const void* eval, int eval_dtype,
+                    double dose_pct, double dta_mm, double threshold_pct, double ref_max, int local,
+                    double* gamma_out, unsigned long long* counts, double* stats,
                     void* workspace, size_t workspace_bytes, void* stream);
 
 #ifdef __cplusplus

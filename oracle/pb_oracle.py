@@ -416,13 +416,16 @@ def _shifted(a, off):
 
 
 def gamma_index(ref, ev, spacing=(0.2, 0.2, 0.2), dose_pct=3.0, dta_mm=3.0,
-                threshold_pct=10.0):
-    """Global gamma index over integer-voxel DTA offsets. gamma.py:81-160"""
+                threshold_pct=10.0, normalization="global", ref_dose=None, return_map=False):
+    """Gamma index over integer-voxel DTA offsets. gamma.py:81-160"""
     ref = np.asarray(ref, dtype=np.float64)
     ev = np.asarray(ev, dtype=np.float64)
     dta = dta_mm / 10.0
-    rmax = float(ref.max())
-    crit = dose_pct / 100.0 * rmax
+    rmax = float(ref.max()) if ref_dose is None else float(ref_dose)
+    if normalization == "global":
+        crit = dose_pct / 100.0 * rmax
+    else:
+        crit = dose_pct / 100.0 * np.maximum(ref, 1e-12)
     mask = ref >= threshold_pct / 100.0 * rmax
     if mask.sum() == 0:
         mask = ref > 0
@@ -430,11 +433,17 @@ def gamma_index(ref, ev, spacing=(0.2, 0.2, 0.2), dose_pct=3.0, dta_mm=3.0,
     g2 = np.full(ref.shape, np.inf)
     for off, r in zip(offs, dist):
         dd = (_shifted(ev, tuple(off)) - ref) / crit
-        np.minimum(g2, (r / dta) ** 2 + dd * dd, out=g2)
-    g = np.sqrt(g2)[mask]
-    return dict(pass_rate=float((g <= 1.0).mean() * 100.0), gamma_max=float(g.max()),
-                gamma_mean=float(g.mean()), n_eval=int(g.size),
-                n_pass=int((g <= 1.0).sum()), n_dta_offsets=len(offs))
+        dr = r / dta
+        np.minimum(g2, dr * dr + dd * dd, out=g2)
+    gm = np.sqrt(g2)
+    g = gm[mask]
+    out = dict(pass_rate=float((g <= 1.0).mean() * 100.0), gamma_max=float(g.max()),
+               gamma_mean=float(g.mean()), n_eval=int(g.size),
+               n_pass=int((g <= 1.0).sum()), n_dta_offsets=len(offs), ref_max=rmax,
+               gamma_p95=float(np.percentile(g, 95)), gamma_p99=float(np.percentile(g, 99)))
+    if return_map:
+        out["gamma"], out["mask"] = gm, mask
+    return out
 
 
 # ---------------------------------------------------------------------------

@@ -146,3 +146,25 @@ def test_standard_scenarios_and_fields():
     assert [(x.lattice.n_sx, x.lattice.n_sy, x.lattice.n_layers) for x in f] == [(31, 31, 25), (41, 41, 40), (27, 35, 45)]
     assert sum(x.lattice.n_spots for x in f) == 24025 + 67240 + 42525
     assert f[2].lattice.weights.shape == (45, 35, 27)
+
+
+@pytest.mark.parametrize("spacing,dta_mm", [((0.2, 0.2, 0.2), 2.0), ((0.2, 0.2, 0.2), 3.0),
+                                            ((0.25, 0.2, 0.15), 3.0), ((0.1, 0.1, 0.1), 10.0),
+                                            ((0.3, 0.3, 0.3), 2.0), ((0.2, 0.2, 0.2), 1.0)])
+def test_gamma_offset_count_matches_oracle(spacing, dta_mm):
+    from oracle import pb_oracle as O
+    dz, dy, dx = spacing
+    g = _lib.PbhGrid(8, 8, 8, 0, dx, dy, dz, 0.0, 0.0, 0.0)
+    assert _lib.lib().pbh_gamma_count_offsets(g, dta_mm) == len(O.dta_offsets(spacing, dta_mm / 10.0)[0])
+
+
+def test_gamma_argument_checks():
+    L = _lib.lib()
+    g = _lib.PbhGrid(8, 8, 8, 0, 0.2, 0.2, 0.2, 0.0, 0.0, 0.0)
+    assert L.pbh_gamma_workspace_bytes(g) > 0
+    nul = C.c_void_p(0)
+    one = C.c_void_p(256)
+    assert L.pbh_gamma_index(g, nul, 0, one, 0, 2.0, 2.0, 10.0, 0.0, 0, nul, one, one, one, 1 << 20, nul) == -1
+    assert L.pbh_gamma_index(g, one, 2, one, 0, 2.0, 2.0, 10.0, 0.0, 0, nul, one, one, one, 1 << 20, nul) == -5
+    assert L.pbh_gamma_index(g, one, 0, one, 0, 0.0, 2.0, 10.0, 0.0, 0, nul, one, one, one, 1 << 20, nul) == -2
+    assert L.pbh_gamma_index(g, one, 0, one, 0, 2.0, 2.0, 10.0, 0.0, 0, nul, one, one, one, 16, nul) == -7

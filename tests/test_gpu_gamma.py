@@ -1,0 +1,101 @@
+"""Gamma index on the GPU (pbh_gamma_index) vs the FP64 oracle (gamma.py
+restatement), and the 2 %/2 mm acceptance check of every arithmetic form of the
+forward dose against the FP64 reference dose.
+
+Bars: n_eval, n_pass and the DTA offset count equal; the gamma map and
+gamma_max equal to 1e-12 relative (FP64 in the same operation order); the mean
+to 1e-12 (summation order differs).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import pb_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def smooth_pair(shape, seed, noise=0.02, shift=0):
+    """A smooth dose-like field and a perturbed, shifted copy."""
+    rng = np.random.default_rng(seed)
+    nz, ny, nx = shape
+    z, y, x = np.meshgrid(np.linspace(-1, 1, nz), np.linspace(-1, 1, ny), np.linspace(-1, 1, nx), indexing="ij")
+    ref = np.exp(-(x * x + y * y) * 3.0) * (1.0 + 0.5 * z) + 0.05 * rng.random(shape)
+    ev = np.roll(ref, shift, axis=2) * (1.0 + noise * rng.standard_normal(shape))
+    return ref, ev
+
+
+@pytest.mark.parametrize("case", [
+    dict(shape=(12, 14, 16), spacing=(0.2, 0.2, 0.2), dose=2.0, dta=2.0, thr=10.0, norm="global"),
+    dict(shape=(10, 11, 13), spacing=(0.25, 0.2, 0.15), dose=3.0, dta=3.0, thr=10.0, norm="global"),
+    dict(shape=(9, 12, 10), spacing=(0.2, 0.2, 0.2), dose=2.0, dta=2.0, thr=10.0, norm="local"),
+    dict(shape=(8, 9, 10), spacing=(0.1, 0.1, 0.1), dose=1.0, dta=4.0, thr=30.0, norm="global", ref_dose=2.5),
+    dict(shape=(8, 9, 10), spacing=(0.2, 0.2, 0.2), dose=2.0, dta=2.0, thr=150.0, norm="global"),
+])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_gamma_matches_oracle(case, dtype):
+    from pbdose import gamma
+    print(f"case gamma {case} {dtype}", flush=True)
+    ref, ev = smooth_pair(case["shape"], seed=len(case["shape"]) + int(case["dose"]), shift=1)
+    ref_t = torch.as_tensor(ref, dtype=dtype, device="cuda")
+    ev_t = torch.as_tensor(ev, dtype=dtype, device="cuda")
+    r64, e64 = ref_t.double().cpu().numpy(), ev_t.double().cpu().numpy()
+    kw = dict(dose_pct=case["dose"], dta_mm=case["dta"], threshold_pct=case["thr"],
+              normalization=case["norm"], ref_dose=case.get("ref_dose"))
+    want = O.gamma_index(r64, e64, case["spacing"], return_map=True, **kw)
+    got = gamma.gamma_index(ref_t, ev_t, case["spacing"], case["dose"], case["dta"], case["thr"],
+                            case["norm"], case.get("ref_dose"))
+    assert got["n_eval"] == want["n_eval"] and got["n_pass"] == want["n_pass"]
+    assert got["n_dta_offsets"] == want["n_dta_offsets"]
+    gm, wm = got["gamma"], want["gamma"]
+    fin = np.isfinite(wm)
+    assert np.array_equal(fin, np.isfinite(gm))
+    assert np.abs(gm[fin] - wm[fin]).max() <= 1e-12 * max(1.0, np.abs(wm[fin]).max())
+    assert abs(got["gamma_max"] - want["gamma_max"]) <= 1e-12 * want["gamma_max"]
+    assert abs(got["gamma_mean"] - want["gamma_mean"]) <= 1e-12 * want["gamma_mean"]
+    assert abs(got["gamma_p95"] - want["gamma_p95"]) <= 1e-12 * want["gamma_p95"]
+    assert np.array_equal(got["mask"], want["mask"])
+    print(f"  pass {got['pass_rate']:.3f}% of {got['n_eval']}, offsets {got['n_dta_offsets']}", flush=True)
+
+
+def test_gamma_identity_and_device_entry_points():
+    from pbdose import gamma
+    ref, _ = smooth_pair((16, 16, 16), 5)
+    t = torch.as_tensor(ref, dtype=torch.float32, device="cuda")
+    r = gamma.gamma_index_torch(t, t, dose_crit_pct=2.0, dta_mm=2.0)
+    assert r["pass_rate"] == 100.0 and r["gamma_max"] == 0.0 and r["n_dta_offsets"] == 7
+    pr, n, gmax = gamma.pass_rate(t, t * 1.05)
+    want = O.gamma_index(t.double().cpu().numpy(), (t * 1.05).double().cpu().numpy(), dose_pct=2.0, dta_mm=2.0)
+    assert n == want["n_eval"] and abs(pr - want["pass_rate"]) < 1e-12
+    rows = gamma.gamma_pass_rate_vs_criteria(t, t * 1.02)
+    assert [(x["dose_pct"], x["dta_mm"]) for x in rows] == [(3.0, 3.0), (2.0, 2.0), (1.0, 1.0), (3.0, 2.0), (2.0, 3.0)]
+    with pytest.raises(ValueError):
+        gamma.gamma_index(t * 0.0, t)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "fp16", "bf16x3", "fp16x3", "bf16x3_peak", "fp16x3_peak"])
+def test_every_form_passes_2pct_2mm_against_fp64(precision):
+    """128^3 / 41x41 / 40-layer field: gamma 2 %/2 mm, 10 % threshold, global,
+    of each arithmetic form against the FP64 dose of the batched-matmul
+    restatement (itself equal to the separable oracle to 1e-14)."""
+    from oracle.torch_cpu import torch_separable_cpu
+    from pbdose import engines, gamma, model, physics
+    print(f"case gamma acceptance {precision} on 128^3", flush=True)
+    grid = model.DoseGrid.centered(128, 0.2)
+    E = np.linspace(70.0, 180.0, 40)
+    w = np.random.default_rng(seed=0).lognormal(0.0, 0.5, (40, 41, 41))
+    idd, sig, _ = physics.build_luts(grid.nz, grid.oz, grid.dz, E)
+    lat = model.SpotLattice(41, 41, 40, 0.5, 0.5, energies=E, weights=w)
+    prob = model.PBProblem(grid, lat, idd.cpu().numpy(), sig.cpu().numpy())
+    global _REF
+    if "_REF" not in globals():
+        torch.set_num_threads(min(16, torch.get_num_threads()))
+        w32 = w.astype(np.float32).astype(np.float64)
+        _REF = torch_separable_cpu(grid.x, grid.y, lat.spot_x, lat.spot_y, w32, prob.idd, prob.sigma).numpy()
+    D = engines.DeviceProblem(prob, idd=idd, sigma=sig).forward(precision)
+    ref = torch.as_tensor(_REF, device="cuda")
+    r = gamma.gamma_index(ref, D, (0.2, 0.2, 0.2), 2.0, 2.0, 10.0, return_maps=False)
+    e = O.relative_error(D.double().cpu().numpy(), _REF)
+    print(f"  pass {r['pass_rate']:.4f}% of {r['n_eval']} voxels, gamma_max {r['gamma_max']:.3e}, "
+          f"max rel err {e['max_rel_err_global']:.2e}", flush=True)
+    assert r["pass_rate"] == 100.0

@@ -151,6 +151,22 @@ def test_separable_equals_brute_force():
     assert np.abs(D - B).max() / np.abs(B).max() < 1e-14
 
 
+def test_torch_cpu_restatement_equals_separable_and_brute_force():
+    """The host restatement of the batched-matmul engine (the CPU baseline and the
+    autograd reference) computes the same dose, with and without layer chunks."""
+    import torch
+    from oracle.torch_cpu import torch_separable_cpu
+    x, y, sx, sy, w, idd, sig = _tiny()
+    D = O.separable_dose(x, y, sx, sy, w, idd, sig)
+    B = O.brute_force_dose(x, y, idd.shape[1], sx, sy, w, idd, sig)
+    for chunk in (None, 1, 2):
+        T = torch_separable_cpu(x, y, sx, sy, w, idd, sig, layer_chunk=chunk).numpy()
+        assert np.abs(T - D).max() / np.abs(D).max() < 1e-14
+        assert np.abs(T - B).max() / np.abs(B).max() < 1e-14
+    T32 = torch_separable_cpu(x, y, sx, sy, w, idd, sig, dtype=torch.float32).double().numpy()
+    assert np.abs(T32 - D).max() / np.abs(D).max() < 1e-6
+
+
 def test_single_spot_is_analytic_gaussian():
     x = O.axis(41, 0.1, -2.0)
     y = O.axis(31, 0.1, -1.5)
@@ -205,6 +221,32 @@ def test_gamma_identity_and_shift():
     ev = ref * 1.0 + 0.03 * ref.max()
     g2 = O.gamma_index(ref, ev, dose_pct=2.0, dta_mm=2.0)
     assert g2["pass_rate"] < 100.0
+
+
+def test_gamma_hand_cases():
+    """Hand-computed gamma on a 1 x 1 x 5 row (2 mm voxels, 2 %/2 mm): in-row
+    neighbours at distance 1 DTA, out-of-grid shifts read 0."""
+    ref = np.ones((1, 1, 5))
+    ev = np.array([[[1.0, 1.01, 1.03, 1.01, 1.0]]])
+    r = O.gamma_index(ref, ev, dose_pct=2.0, dta_mm=2.0, return_map=True)
+    # voxel 2: min(0 DTA + (0.03/0.02)^2, 1 + (0.01/0.02)^2) = 1.25
+    # voxel 1: min((0.01/0.02)^2, 1 + 0, 1 + (0.03/0.02)^2) = 0.25
+    np.testing.assert_allclose(r["gamma"][0, 0], np.sqrt([0.0, 0.25, 1.25, 0.25, 0.0]), rtol=1e-12)
+    assert r["n_pass"] == 4 and r["n_eval"] == 5 and r["n_dta_offsets"] == 7
+    # local normalisation: crit = 2 % of the local reference dose
+    ref = np.array([[[1.0, 1.0, 0.5, 1.0, 1.0]]])
+    ev = np.array([[[1.0, 1.0, 0.51, 1.0, 1.0]]])
+    g = O.gamma_index(ref, ev, dose_pct=2.0, dta_mm=2.0, return_map=True)["gamma"][0, 0, 2]
+    gl = O.gamma_index(ref, ev, dose_pct=2.0, dta_mm=2.0, normalization="local", return_map=True)["gamma"][0, 0, 2]
+    assert abs(g - 0.5) < 1e-12 and abs(gl - 1.0) < 1e-12
+    # threshold above every voxel: evaluated set falls back to ref > 0
+    assert O.gamma_index(ref, ev, threshold_pct=150.0)["n_eval"] == 5
+    # DTA offset counts: 3 mm at 2 mm voxels -> |o| <= 1 voxel with |o| <= 0.3 cm: 19
+    assert len(O.dta_offsets((0.2, 0.2, 0.2), 0.3)[0]) == 19
+    assert len(O.dta_offsets((0.2, 0.2, 0.2), 0.2)[0]) == 7
+    assert len(O.dta_offsets((0.1, 0.1, 0.1), 0.1)[0]) == 7
+    assert len(O.dta_offsets((0.1, 0.1, 0.1), 1.0, max_vox=6)[0]) == sum(
+        1 for a in range(-6, 7) for b in range(-6, 7) for c in range(-6, 7) if a * a + b * b + c * c <= 100)
 
 
 def test_adjoint_dot_product():
