@@ -37,6 +37,12 @@ for _p in (ROOT, os.path.join(ROOT, "gpu-proton-dose-engine_amd")):
 import numpy as np  # noqa: E402
 import torch  # noqa: E402
 
+# HBM bytes per launch of the forward kernels on this workload, from rocprofv3 FETCH_SIZE
+# and WRITE_SIZE (KiB per dispatch) scaled by the same counters on a 256 MiB copy
+# (fetch x 2.000, write x 1.000): profiles/counters_forward_adjoint_128cube_67240spots.md
+TRAFFIC_KIB = {"fp32": (1492.3, 8192.0), "bf16x3": (1509.9, 8192.0), "bf16": (1494.0, 8192.0)}
+FETCH_SCALE, WRITE_SCALE = 2.000, 1.000
+
 # published dense MFMA peaks of MI355X (TFLOP/s) for the arithmetic each form runs on
 PEAK = {"fp32": 157.3, "bf16": 2500.0, "fp16": 2500.0}
 DTYPE = {"fp32": "f32", "bf16": "bf16", "fp16": "f16"}
@@ -51,6 +57,14 @@ def field_flops(nx, ny, nz, nsx, nsy, L):
     a = nsy * nsx * ny + nx * nsx * ny
     b = nsy * nsx * nx + ny * nsy * nx
     return 2.0 * L * nz * min(a, b)
+
+
+def traffic_bytes(precision):
+    """Profiled HBM bytes per forward launch (None for forms not profiled)."""
+    if precision not in TRAFFIC_KIB:
+        return None
+    f, w = TRAFFIC_KIB[precision]
+    return (f * FETCH_SCALE + w * WRITE_SCALE) * 1024.0
 
 
 def build_field(device):
@@ -182,6 +196,33 @@ def cpu_leg(prob, budget_s=20.0, max_reps=15):
     return base, ref.numpy()
 
 
+def profile_kernels(forms, reps=5):
+    """Short run for rocprofv3: forwards of the given forms, objective + adjoint,
+    and three 256 MiB device copies (known bytes, to calibrate HBM counters)."""
+    from pbdose import planning
+    dev = torch.device("cuda", 0)
+    prob, dp = build_field(dev)
+    out = torch.empty(dp.shape, dtype=torch.float32, device=dev)
+    for f in forms:
+        for _ in range(reps):
+            dp.forward(f, out=out)
+        torch.cuda.synchronize()
+        print(f"forward {f}: {reps} launches", flush=True)
+    Dp, wv = planning.central_cube(prob.grid, device=dev)
+    D = dp.forward("fp32")
+    for _ in range(reps):
+        _, R = planning.objective(dp, D, Dp, wv)
+        planning.adjoint(dp, R)
+    torch.cuda.synchronize()
+    print(f"objective + adjoint: {reps} calls", flush=True)
+    a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print("copy 256 MiB: 3 calls", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -190,6 +231,8 @@ def main():
     ap.add_argument("--precision", default="bf16x3")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-robust", action="store_true", help="skip the robust-scenario batch leg")
+    ap.add_argument("--profile-kernels", metavar="FORMS", default=None,
+                    help="comma-separated forms: run a short kernel sequence for rocprofv3 and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build runs the "
@@ -200,6 +243,9 @@ def main():
         sys.exit(1)
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.profile_kernels:
+        profile_kernels(args.profile_kernels.split(","))
+        return
     prob, dp = build_field(dev)
     g, lat = prob.grid, prob.lattice
     flops = field_flops(g.nx, g.ny, g.nz, lat.n_sx, lat.n_sy, lat.n_layers)
@@ -241,7 +287,7 @@ def main():
         "roofline": {"bound": "mfma",
                      "kernel": "fwd_fp32_kernel" if args.precision == "fp32" else "fwd_mixed_kernel",
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak, "traffic": traffic_bytes(args.precision),
                      "algorithmic_flop_per_launch": flops, "launch_ms": launch_ms},
     }
     if not args.no_robust:
