@@ -9,7 +9,8 @@ Workload (one step = one forward dose of the whole field, inputs resident in HBM
 Prints ONE JSON line. ``value`` = fields/s of the selected arithmetic form
 (``--precision``, default the BF16 split-product form); the FP32 form, the
 forward + adjoint time (adjoint of the weighted-quadratic objective: dose
-residual kernel + FP32 adjoint + depth reduction) and the error of each form
+residual kernel + adjoint + depth reduction; FP32 adjoint for the FP32 form,
+BF16 or BF16x3 adjoint for the 16-bit forms) and the error of each form
 against the FP64 oracle are reported beside it. ``roofline`` is for the dominant
 kernel (the fused forward of the selected form), timed per launch with HIP
 events on the launch stream. ``cpu_baseline`` times the host restatement of the
@@ -124,23 +125,31 @@ def time_torch_separable(prob, precision, reps):
     return float(np.median(ts)) * 1e3
 
 
-def time_adjoint(dp, D, steps, warmup):
-    """Objective residual + adjoint (dJ/dw) per call, events on the launch stream."""
+def adjoint_form(precision):
+    """Arithmetic of the adjoint that goes with a forward form."""
+    if precision == "fp32":
+        return "fp32"
+    return "bf16" if precision in ("bf16", "fp16") else "bf16x3"
+
+
+def time_adjoint(dp, D, steps, warmup, precision="fp32"):
+    """Objective residual + adjoint (dJ/dw) per call, events on the launch stream.
+    Returns (ms per call, dJ/dw of the last call)."""
     from pbdose import planning
     Dp, wv = planning.central_cube(dp.problem.grid, device=dp.dev)
     for _ in range(warmup):
         _, R = planning.objective(dp, D, Dp, wv)
-        planning.adjoint(dp, R)
+        planning.adjoint(dp, R, precision=precision)
     torch.cuda.synchronize()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
           for _ in range(steps)]
     for s, e in ev:
         s.record()
         _, R = planning.objective(dp, D, Dp, wv)
-        planning.adjoint(dp, R)
+        g = planning.adjoint(dp, R, precision=precision)
         e.record()
     torch.cuda.synchronize()
-    return float(np.mean([s.elapsed_time(e) for s, e in ev]))
+    return float(np.mean([s.elapsed_time(e) for s, e in ev])), g
 
 
 def time_robust(precision, steps, objective):
@@ -255,7 +264,13 @@ def main():
         D32, ms32, launch32 = time_forward(dp, "fp32", args.steps, args.warmup)
     else:
         D32, ms32, launch32 = D, ms_step, launch_ms
-    adj_ms = time_adjoint(dp, D32, max(5, args.steps // 4), 3)
+    adj32_ms, g32 = time_adjoint(dp, D32, max(5, args.steps // 4), 3)
+    aform = adjoint_form(args.precision)
+    if aform != "fp32":
+        adj_ms, gmix = time_adjoint(dp, D32, max(5, args.steps // 4), 3, aform)
+        adj_err = float((gmix.double() - g32.double()).abs().max() / g32.double().abs().max())
+    else:
+        adj_ms, adj_err = adj32_ms, 0.0
     pcie_ms = time_torch_separable(prob, args.precision, max(5, args.steps // 10))
     bt = base_type(args.precision)
     peak = PEAK[bt]
@@ -280,9 +295,13 @@ def main():
         "fp32": {"fields_per_s": 1e3 / ms32, "ms": ms32, "launch_ms": launch32,
                  "accuracy_vs_fp64": None},
         "speedup_vs_fp32": ms32 / ms_step,
+        "adjoint_form": aform,
         "adjoint_ms": adj_ms,
+        "adjoint_ms_fp32": adj32_ms,
+        "adjoint_max_rel_diff_vs_fp32_adjoint": adj_err,
         "fwd_adj_ms": ms_step + adj_ms,
-        "fwd_adj_ms_fp32": ms32 + adj_ms,
+        "fwd_adj_ms_fp32": ms32 + adj32_ms,
+        "fwd_adj_speedup_vs_fp32": (ms32 + adj32_ms) / (ms_step + adj_ms),
         "torch_separable_ms_incl_host_copies": pcie_ms,
         "roofline": {"bound": "mfma",
                      "kernel": "fwd_fp32_kernel" if args.precision == "fp32" else "fwd_mixed_kernel",

@@ -81,6 +81,16 @@ __device__ __forceinline__ void adj_fp32_body(const AdjArgs& A, int blk)
                         for (int q = 0; q < nl; ++q) {
                             const int l = lb + q * A.split;
                             const double iddv = a.idd[(size_t)l * a.nz + z];
+                            if (iddv == 0.0 && !lut_grads) {
+                                // no dose from this layer at this depth: its dJ/dw partial is zero
+                                // (uniform in the workgroup: one address for all threads)
+                                if (first_tile) {
+                                    float* pz = A.part + ((size_t)z * a.L + l) * a.nsy * a.nsx;
+                                    for (int e = threadIdx.x; e < ni * nj; e += AJ_THREADS)
+                                        pz[(size_t)(j0 + e / ni) * a.nsx + i0 + e % ni] = 0.0f;
+                                }
+                                continue;
+                            }
                             const double sg = a.sigma[(size_t)l * a.nz + z];
                             const double sd = sg > kSigmaFloor ? sg : kSigmaFloor;
                             const double k2d = kLog2e / (2.0 * sd * sd);
@@ -374,7 +384,7 @@ extern "C" int pbh_validate(const PbhGrid* g, const PbhLattice* lat);
 int pbh_adj_default_split(const PbhGrid* g, const PbhLattice* lat)
 {
     int s = 1;
-    while (s < 16 && (long long)g->nz * s < 512 && s * 2 <= lat->n_layers) s *= 2;
+    while (s < 16 && (long long)g->nz * s < 1024 && s * 2 <= lat->n_layers) s *= 2;
     return s;
 }
 
@@ -425,20 +435,31 @@ int pbh_make_adj_args(const PbhGrid* g, const PbhLattice* lat, const float* w, c
     return PBH_OK;
 }
 
-extern "C" int pbh_adjoint(const PbhGrid* g, const PbhLattice* lat, const float* w, const double* idd,
-                           const double* sigma, const float* R, float* grad_w, double* grad_idd,
-                           double* grad_sigma, int precision, void* workspace, size_t workspace_bytes,
-                           void* stream)
+extern "C" int pbh_adjoint_tuned(const PbhGrid* g, const PbhLattice* lat, const float* w, const double* idd,
+                                 const double* sigma, const float* R, float* grad_w, double* grad_idd,
+                                 double* grad_sigma, int precision, int split, void* workspace,
+                                 size_t workspace_bytes, void* stream)
 {
     int rc = pbh_validate(g, lat);
     if (rc != PBH_OK) return rc;
     if (!w || !idd || !sigma || !R || !grad_w || !workspace) return PBH_ERR_NULL_ARG;
-    if (precision != PBH_PREC_FP32) return PBH_ERR_UNSUPPORTED;
+    const bool mixed = precision == PBH_PREC_BF16 || precision == PBH_PREC_BF16X3;
+    if (precision != PBH_PREC_FP32 && !mixed) return PBH_ERR_UNSUPPORTED;
+    if (mixed && (grad_idd || grad_sigma)) return PBH_ERR_UNSUPPORTED;   // table gradients: FP32 only
+    if (split < 0) return PBH_ERR_BAD_DIM;
     if (workspace_bytes < pbh_adjoint_workspace_bytes(g, lat, precision)) return PBH_ERR_WORKSPACE;
     hipStream_t s = (hipStream_t)stream;
     AdjArgs A;
-    rc = pbh_make_adj_args(g, lat, w, idd, sigma, R, (float*)workspace, grad_idd, grad_sigma, 0, &A);
+    rc = pbh_make_adj_args(g, lat, w, idd, sigma, R, (float*)workspace, grad_idd, grad_sigma, split, &A);
     if (rc != PBH_OK) return rc;
+    const size_t n = (size_t)lat->n_layers * lat->n_sy * lat->n_sx;
+    if (mixed) {
+        if (!A.tab) return PBH_ERR_UNSUPPORTED;   // the BF16 adjoint gathers every factor from tables
+        if ((rc = pbh_launch_adj_mixed(A, precision, g->nz, s)) != PBH_OK) return rc;
+        hipLaunchKernelGGL(adj_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                           (const float*)A.part, g->nz, n, grad_w);
+        return launch_rc("adj_reduce_kernel");
+    }
     const dim3 grid((unsigned)(g->nz * A.split)), block(AJ_THREADS);
     const bool sig = grad_sigma != nullptr;
     if (A.tab) {
@@ -449,10 +470,18 @@ extern "C" int pbh_adjoint(const PbhGrid* g, const PbhLattice* lat, const float*
         else hipLaunchKernelGGL((adj_fp32_kernel<false, false>), grid, block, 0, s, A);
     }
     if ((rc = launch_rc("adj_fp32_kernel")) != PBH_OK) return rc;
-    const size_t n = (size_t)lat->n_layers * lat->n_sy * lat->n_sx;
     hipLaunchKernelGGL(adj_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
                        (const float*)A.part, g->nz, n, grad_w);
     return launch_rc("adj_reduce_kernel");
+}
+
+extern "C" int pbh_adjoint(const PbhGrid* g, const PbhLattice* lat, const float* w, const double* idd,
+                           const double* sigma, const float* R, float* grad_w, double* grad_idd,
+                           double* grad_sigma, int precision, void* workspace, size_t workspace_bytes,
+                           void* stream)
+{
+    return pbh_adjoint_tuned(g, lat, w, idd, sigma, R, grad_w, grad_idd, grad_sigma, precision, 0, workspace,
+                             workspace_bytes, stream);
 }
 
 static constexpr int OBJ_BLOCKS = 1024;

@@ -52,7 +52,14 @@ int pbh_adj_default_split(const PbhGrid* g, const PbhLattice* lat);
 int pbh_launch_fwd_fp32_batch(const pbh::BatchFwdItem* items, const int* wg0, int n, int total, int scp,
                               bool tab, hipStream_t s);
 int pbh_launch_fwd_mixed_batch(const pbh::BatchFwdItem* items, const int* wg0, int n, int total,
-                               int precision, int kb, bool tab, hipStream_t s);
+                               int precision, int kb, bool tab, bool q5, hipStream_t s);
+int pbh_launch_wsplit_batch(const pbh::BatchFwdItem* items, int n, int max_chunks, int precision, hipStream_t s);
+int pbh_mixed_slot_mode(const pbh::FwdPlan& pl);
+bool pbh_adj_mixed_q5(const pbh::AdjArgs& A);
+int pbh_launch_adj_mixed(const pbh::AdjArgs& A, int precision, int nz, hipStream_t s);
+int pbh_launch_adj_mixed_batch(const pbh::AdjArgs* items, const int* wg0, int n, int total, int precision, bool q5,
+                               hipStream_t s);
+int pbh_launch_combine_batch(const pbh::BatchFwdItem* items, const int* wg0, int n, int total, hipStream_t s);
 int pbh_launch_peak_batch(const pbh::BatchPeakItem* items, int n, hipStream_t s);
 int pbh_launch_adj_batch(const pbh::AdjArgs* items, const int* wg0, int n, int total, bool tab, bool sig,
                          hipStream_t s);

@@ -34,7 +34,7 @@ struct Launch {
     int wg0_off;    // offset of the group's prefix table (count + 1 entries)
     int total;      // workgroups
     int key;        // FP32: padded chunk width; mixed: stage B K steps
-    bool tab;
+    int mode;       // bit 0: table factors; bit 1: compile-time table slots (mixed)
 };
 
 bool is_mixed(int precision) { return precision != PBH_PREC_FP32; }
@@ -48,7 +48,6 @@ PbhBatchTuning tuning_or_default(const PbhBatchTuning* t)
 {
     PbhBatchTuning r{0, 0, 0, 0};
     if (t) r = *t;
-    if (r.fwd_split == 0) r.fwd_split = 1;
     return r;
 }
 
@@ -72,7 +71,7 @@ double fwd_work(const PbhBatchItem& it)
 
 struct Layout {
     // offsets in the workspace
-    size_t fwd_items, fwd_wg0, peak_items, obj_items, adj_items, adj_wg0, red_items, red_wg0;
+    size_t fwd_items, fwd_wg0, cmb_wg0, peak_items, obj_items, adj_items, adj_wg0, red_items, red_wg0;
     size_t table_end;
     std::vector<size_t> fwd_ws, R, adj_part, obj_part;
     std::vector<size_t> fwd_ws_bytes;
@@ -96,7 +95,7 @@ int check_items(const PbhBatchItem* items, int n, int precision)
 
 int check_tuning(const PbhBatchTuning& t)
 {
-    if (t.fwd_split != 1 && t.fwd_split != 2 && t.fwd_split != 4 && t.fwd_split != 8) return PBH_ERR_UNSUPPORTED;
+    if (t.fwd_chunk < 0) return PBH_ERR_UNSUPPORTED;
     if (t.adj_split != 0 && t.adj_split != 1 && t.adj_split != 2 && t.adj_split != 4 && t.adj_split != 8 &&
         t.adj_split != 16)
         return PBH_ERR_UNSUPPORTED;
@@ -115,6 +114,7 @@ Layout make_layout(const PbhBatchItem* items, int n, const PbhBatchTuning& t)
     // there are at most n groups per stage
     L.fwd_items = off; off = align256(off + (size_t)n * sizeof(BatchFwdItem));
     L.fwd_wg0 = off; off = align256(off + (size_t)2 * n * sizeof(int) + sizeof(int));
+    L.cmb_wg0 = off; off = align256(off + (size_t)(n + 1) * sizeof(int));
     L.peak_items = off; off = align256(off + (size_t)n * sizeof(BatchPeakItem));
     L.obj_items = off; off = align256(off + (size_t)std::max(n_obj, 1) * sizeof(BatchObjItem));
     L.adj_items = off; off = align256(off + (size_t)std::max(n_obj, 1) * sizeof(AdjArgs));
@@ -130,7 +130,7 @@ Layout make_layout(const PbhBatchItem* items, int n, const PbhBatchTuning& t)
     for (int k = 0; k < n; ++k) {
         const PbhBatchItem& it = items[k];
         const FwdArgs a = make_fwd_args(&it.grid, &it.lattice, nullptr, nullptr, nullptr, nullptr);
-        const size_t fb = pbh_forward_ws_bytes(a, t.fwd_split);
+        const size_t fb = pbh_forward_ws_bytes(a, t.fwd_chunk);
         L.fwd_ws[k] = off;
         L.fwd_ws_bytes[k] = fb;
         off = align256(off + fb);
@@ -151,8 +151,12 @@ Layout make_layout(const PbhBatchItem* items, int n, const PbhBatchTuning& t)
 struct PbhBatch {
     int n = 0, precision = 0, n_obj = 0;
     bool peaks = false, sig = false;
+    int max_wchunks = 0;     // mixed forms: largest (layer, spot chunk) count of an item
+    int adj_prec = PBH_PREC_FP32;   // arithmetic of the adjoint kernels
     BatchFwdItem* d_fwd = nullptr;
     int* d_fwd_wg0 = nullptr;
+    int* d_cmb_wg0 = nullptr;    // combine: prefix of the tiles of items with several layer chunks
+    int cmb_total = 0;
     BatchPeakItem* d_peak = nullptr;
     BatchObjItem* d_obj = nullptr;
     AdjArgs* d_adj = nullptr;
@@ -163,14 +167,14 @@ struct PbhBatch {
     std::vector<Launch> fwd, adj;
 };
 
-// Splits items (already in launch order) into groups of equal key and table mode.
-static std::vector<Launch> make_groups(const std::vector<int>& keys, const std::vector<char>& tabs)
+// Splits items (already in launch order) into groups of equal key and mode.
+static std::vector<Launch> make_groups(const std::vector<int>& keys, const std::vector<char>& modes)
 {
     std::vector<Launch> out;
     const int n = (int)keys.size();
     for (int k = 0; k < n; ++k) {
-        if (out.empty() || out.back().key != keys[k] || out.back().tab != (bool)tabs[k])
-            out.push_back(Launch{k, 0, 0, 0, keys[k], (bool)tabs[k]});
+        if (out.empty() || out.back().key != keys[k] || out.back().mode != (int)modes[k])
+            out.push_back(Launch{k, 0, 0, 0, keys[k], (int)modes[k]});
         out.back().count++;
     }
     int off = 0;
@@ -230,15 +234,15 @@ extern "C" int pbh_batch_create(const PbhBatchItem* items, int n, int precision,
         f.a = make_fwd_args(&it.grid, &it.lattice, it.w, it.idd, it.sigma, it.dose);
         const int kx = mixed ? 16 * key[k] : key[k];
         const int ky = mixed ? 64 : key[k];
-        rc = pbh_plan_forward(f.a, kx, ky, ws + L.fwd_ws[k], L.fwd_ws_bytes[k], t.fwd_split, &f.pl);
+        rc = pbh_plan_forward(f.a, kx, ky, ws + L.fwd_ws[k], L.fwd_ws_bytes[k], t.fwd_chunk, &f.pl);
         if (rc != PBH_OK) return rc;
         f.oz = it.grid.oz;
         f.dz = it.grid.dz;
         f.r_peak = is_peak(precision) ? pbh_forward_ws_peaks(f.a, ws + L.fwd_ws[k]) : nullptr;
         hp[p] = BatchPeakItem{it.idd, it.lattice.n_layers, it.grid.nz, it.grid.oz, it.grid.dz, (float*)f.r_peak};
         keys[p] = key[k];
-        tabs[p] = (char)f.pl.table;
-        const long long w = (long long)f.a.tiles_x * f.a.tiles_y * f.a.nz * f.pl.split;
+        tabs[p] = (char)(mixed ? pbh_mixed_slot_mode(f.pl) : f.pl.table);
+        const long long w = (long long)f.a.tiles_x * f.a.tiles_y * f.a.nz * f.pl.nchunk;
         if (w > 0x3fffffffLL) return PBH_ERR_UNSUPPORTED;
         wgs[p] = (int)w;
     }
@@ -247,7 +251,23 @@ extern "C" int pbh_batch_create(const PbhBatchItem* items, int n, int precision,
     b->n = n;
     b->precision = precision;
     b->peaks = is_peak(precision);
+    // the adjoint follows the forward's arithmetic: FP32, BF16 (plain forms) or BF16 x3
+    b->adj_prec = precision == PBH_PREC_FP32 ? PBH_PREC_FP32
+                  : (precision == PBH_PREC_BF16 || precision == PBH_PREC_FP16) ? PBH_PREC_BF16 : PBH_PREC_BF16X3;
     b->fwd = make_groups(keys, tabs);
+    for (int k = 0; k < n; ++k)
+        b->max_wchunks = std::max(b->max_wchunks, items[k].lattice.n_layers * chunk_count(items[k].lattice.n_sy) *
+                                                      chunk_count(items[k].lattice.n_sx));
+    std::vector<int> cmb_wg0(n + 1, 0);
+    {
+        long long acc = 0;
+        for (int p = 0; p < n; ++p) {
+            cmb_wg0[p] = (int)acc;
+            if (hf[p].pl.nchunk > 1) acc += (long long)hf[p].a.tiles_x * hf[p].a.tiles_y * hf[p].a.nz;
+        }
+        cmb_wg0[n] = (int)acc;
+        b->cmb_total = (int)acc;
+    }
     std::vector<int> fwd_wg0(2 * n + 1, 0);
     for (auto& g : b->fwd) {
         long long acc = 0;
@@ -288,7 +308,9 @@ extern "C" int pbh_batch_create(const PbhBatchItem* items, int n, int precision,
         ha.push_back(A);
         b->sig = b->sig || it.grad_sigma != nullptr;
         akeys.push_back(0);
-        atabs.push_back((char)A.tab);
+        // adjoint mode: bit 0 table factors, bit 1 BF16 kernel, bit 2 P = 2, Q = 5 slots
+        const bool mix = b->adj_prec != PBH_PREC_FP32 && A.tab && !it.grad_idd && !it.grad_sigma;
+        atabs.push_back((char)(mix ? (3 | (pbh_adj_mixed_q5(A) ? 4 : 0)) : (A.tab ? 1 : 0)));
         aw.push_back(it.grid.nz * A.split);
         const size_t ne = (size_t)it.lattice.n_layers * it.lattice.n_sy * it.lattice.n_sx;
         hr.push_back(BatchRedItem{part, it.grad_w, ne, it.grid.nz});
@@ -337,6 +359,7 @@ extern "C" int pbh_batch_create(const PbhBatchItem* items, int n, int precision,
     // ---- copy the tables to the device ----
     b->d_fwd = (BatchFwdItem*)(ws + L.fwd_items);
     b->d_fwd_wg0 = (int*)(ws + L.fwd_wg0);
+    b->d_cmb_wg0 = (int*)(ws + L.cmb_wg0);
     b->d_peak = (BatchPeakItem*)(ws + L.peak_items);
     b->d_obj = (BatchObjItem*)(ws + L.obj_items);
     b->d_adj = (AdjArgs*)(ws + L.adj_items);
@@ -346,6 +369,7 @@ extern "C" int pbh_batch_create(const PbhBatchItem* items, int n, int precision,
     std::vector<char> host(L.table_end, 0);
     std::memcpy(host.data() + L.fwd_items, hf.data(), hf.size() * sizeof(BatchFwdItem));
     std::memcpy(host.data() + L.fwd_wg0, fwd_wg0.data(), fwd_wg0.size() * sizeof(int));
+    std::memcpy(host.data() + L.cmb_wg0, cmb_wg0.data(), cmb_wg0.size() * sizeof(int));
     std::memcpy(host.data() + L.peak_items, hp.data(), hp.size() * sizeof(BatchPeakItem));
     if (b->n_obj) {
         std::memcpy(host.data() + L.obj_items, ho.data(), ho.size() * sizeof(BatchObjItem));
@@ -370,19 +394,28 @@ extern "C" int pbh_batch_forward_adjoint(const PbhBatch* b, void* stream)
     hipStream_t s = (hipStream_t)stream;
     int rc;
     if (b->peaks && (rc = pbh_launch_peak_batch(b->d_peak, b->n, s)) != PBH_OK) return rc;
+    if (is_mixed(b->precision) && (rc = pbh_launch_wsplit_batch(b->d_fwd, b->n, b->max_wchunks, b->precision, s)) != PBH_OK)
+        return rc;
     for (const Launch& g : b->fwd) {
         if (b->precision == PBH_PREC_FP32)
             rc = pbh_launch_fwd_fp32_batch(b->d_fwd + g.first, b->d_fwd_wg0 + g.wg0_off, g.count, g.total, g.key,
-                                           g.tab, s);
+                                           g.mode & 1, s);
         else
             rc = pbh_launch_fwd_mixed_batch(b->d_fwd + g.first, b->d_fwd_wg0 + g.wg0_off, g.count, g.total,
-                                            b->precision, g.key, g.tab, s);
+                                            b->precision, g.key, g.mode & 1, (g.mode & 2) != 0, s);
         if (rc != PBH_OK) return rc;
     }
+    if (b->cmb_total > 0 && (rc = pbh_launch_combine_batch(b->d_fwd, b->d_cmb_wg0, b->n, b->cmb_total, s)) != PBH_OK)
+        return rc;
     if (b->n_obj == 0) return PBH_OK;
     if ((rc = pbh_launch_obj_batch(b->d_obj, b->n_obj, s)) != PBH_OK) return rc;
     for (const Launch& g : b->adj) {
-        rc = pbh_launch_adj_batch(b->d_adj + g.first, b->d_adj_wg0 + g.wg0_off, g.count, g.total, g.tab, b->sig, s);
+        if (g.mode & 2)
+            rc = pbh_launch_adj_mixed_batch(b->d_adj + g.first, b->d_adj_wg0 + g.wg0_off, g.count, g.total,
+                                            b->adj_prec, (g.mode & 4) != 0, s);
+        else
+            rc = pbh_launch_adj_batch(b->d_adj + g.first, b->d_adj_wg0 + g.wg0_off, g.count, g.total, g.mode & 1,
+                                      b->sig, s);
         if (rc != PBH_OK) return rc;
     }
     return pbh_launch_reduce_batch(b->d_red, b->d_red_wg0, b->n_obj, b->red_total, s);
@@ -391,7 +424,7 @@ extern "C" int pbh_batch_forward_adjoint(const PbhBatch* b, void* stream)
 extern "C" int pbh_batch_launch_count(const PbhBatch* b)
 {
     if (!b) return PBH_ERR_NULL_ARG;
-    int n = (b->peaks ? 1 : 0) + (int)b->fwd.size();
+    int n = (b->peaks ? 1 : 0) + (is_mixed(b->precision) ? 1 : 0) + (int)b->fwd.size() + (b->cmb_total > 0 ? 1 : 0);
     if (b->n_obj) n += 2 + (int)b->adj.size() + 1;
     return n;
 }

@@ -1,16 +1,18 @@
 // pbh_forward.hip -- fused forward dose contraction, FP32 MFMA (gfx950), and the
 // host-side planning shared with the mixed-precision forms.
 //
-// For one depth slice z, a dose tile D[y0:y0+64, x0:x0+128] is
+// For one depth slice z, a dose tile D[y0:y0+32, x0:x0+128] is
 //     D = sum_l sum_chunks (c_lz * Gy_lz[:, J] W_l[J, I]) Gx_lz[:, I]^T
-// with c_lz = IDD_l(z) / (2 pi s^2). A workgroup (4 waves) owns one tile and a
-// share of its layers (layer split S: layers s, s+S, ...), keeping D in MFMA
+// with c_lz = IDD_l(z) / (2 pi s^2). A workgroup (4 waves, two per CU) owns one
+// tile and one chunk of the slice's active layers, keeping D in MFMA
 // accumulators, so no per-layer intermediate reaches HBM (the reference
-// materialises Gx, Gy, A, T and Dl per layer: engines.py:261-274).
+// materialises Gx, Gy, A, T and Dl per layer: engines.py:261-274). Chunk 0
+// writes the dose; later chunks write partial tiles that combine_kernel adds in
+// chunk order.
 //
-//   stage A  U[64 x 48]  = Gy[64 x J] W[J x I]      v_mfma_f32_16x16x4f32
-//            wave w: rows 16w..16w+15; Gy is the A operand, W (LDS) the B operand.
-//   stage B  D[64 x 128] += (c U)[64 x I] Gx^T       v_mfma_f32_32x32x2f32
+//   stage A  U[32 x 48]  = Gy[32 x J] W[J x I]      v_mfma_f32_16x16x4f32
+//            Gy is the A operand, W (LDS) the B operand.
+//   stage B  D[32 x 128] += (c U)[32 x I] Gx^T       v_mfma_f32_32x32x2f32
 //            wave w: columns 32w..32w+31; (c U) from LDS, Gx the B operand.
 //
 // Gaussian factors. On regular grids whose voxel and spot pitches have a small
@@ -23,7 +25,7 @@
 //
 // Each layer's contribution is summed in fresh accumulators and added to the
 // running dose once (rounding chain per voxel: K steps + layers). Tables, W and
-// U are double-buffered in LDS: two barriers per layer.
+// U are double-buffered in LDS: one barrier per layer.
 #include "pbh_fwd_core.h"
 
 namespace pbh {
@@ -36,7 +38,7 @@ namespace pbh {
 // block (w % 2, w / 2) and, when NCB = 3, block (w % 2, 2) for w < 2.
 // Stage B: wave w owns the 32 x 32 dose block at columns 32 w.
 //
-// blk: the workgroup's index in the problem's own grid (tile * split + share).
+// blk: the workgroup's index in the problem's own grid (chunk * tiles + tile).
 template <bool TAB, int SCP>
 __device__ __forceinline__ void fwd_fp32_body(const FwdArgs& a, const FwdPlan& pl, int blk)
 {
@@ -55,8 +57,9 @@ __device__ __forceinline__ void fwd_fp32_body(const FwdArgs& a, const FwdPlan& p
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int s = blk % pl.split;
-    const int tile = blk / pl.split;
+    const int tiles = a.tiles_x * a.tiles_y * a.nz;
+    const int ch = blk / tiles;                     // layer chunk of this workgroup
+    const int tile = blk % tiles;
     const int tx = tile % a.tiles_x;
     const int ty = (tile / a.tiles_x) % a.tiles_y;
     const int z = tile / (a.tiles_x * a.tiles_y);
@@ -82,9 +85,13 @@ __device__ __forceinline__ void fwd_fp32_body(const FwdArgs& a, const FwdPlan& p
     v16f acc = {};
     float wreg[NW];
 
-    for (int lb = s; lb < a.L; lb += FT_LB * pl.split) {
-        const int nact = collect_layers(a, lb, pl.split, z, 0.0f, nullptr, ll);
-        if (nact == 0) continue;
+    const int lo = ch * pl.chunk, hi = lo + pl.chunk;   // this workgroup's active layers
+    int base = 0;                                       // active layers of earlier passes
+    for (int lb = 0; lb < a.L && base < hi; lb += FT_LB) {
+        const int nact = collect_layers(a, lb, z, 0.0f, nullptr, ll);
+        const int m0 = max(0, lo - base), m1 = min(nact, hi - base);
+        base += nact;
+        if (m1 <= m0) continue;
         for (int cy = 0; cy < ncy; ++cy) {
             const int j0 = chunk_begin(a.nsy, ncy, cy), nj = chunk_begin(a.nsy, ncy, cy + 1) - j0;
             const double Cy = (a.oy + (double)y0 * a.dy) - (a.sy0 + (double)j0 * a.dsy);
@@ -179,27 +186,27 @@ __device__ __forceinline__ void fwd_fp32_body(const FwdArgs& a, const FwdPlan& p
                     acc += p;
                 };
 
-                // prologue: layer 0 staged, U(0) computed, Tx(0), W(1), Ty(1) staged
-                load_w(0);
-                store_w(0);
-                build_ty(0);
-                if (nact > 1) load_w(1);
+                // prologue: layer m0 staged, U(m0) computed, Tx(m0), W(m0+1), Ty(m0+1) staged
+                load_w(m0);
+                store_w(m0 & 1);
+                build_ty(m0);
+                if (m0 + 1 < m1) load_w(m0 + 1);
                 __syncthreads();
-                stage_a(0);
-                build_tx(0);
-                if (nact > 1) {
-                    store_w(1);
-                    build_ty(1);
+                stage_a(m0);
+                build_tx(m0);
+                if (m0 + 1 < m1) {
+                    store_w((m0 + 1) & 1);
+                    build_ty(m0 + 1);
                 }
                 __syncthreads();
-                for (int m = 0; m < nact; ++m) {
-                    if (m + 2 < nact) load_w(m + 2);
+                for (int m = m0; m < m1; ++m) {
+                    if (m + 2 < m1) load_w(m + 2);
                     stage_b(m);
-                    if (m + 1 < nact) {
+                    if (m + 1 < m1) {
                         stage_a(m + 1);
                         build_tx(m + 1);
                     }
-                    if (m + 2 < nact) {
+                    if (m + 2 < m1) {
                         store_w(m & 1);
                         build_ty(m + 2);
                     }
@@ -208,7 +215,8 @@ __device__ __forceinline__ void fwd_fp32_body(const FwdArgs& a, const FwdPlan& p
             }
         }
     }
-    store_block(a, pl, tile, s, z, y0, x0, 0, cnB, acc, (size_t)a.tiles_x * a.tiles_y * a.nz);
+    if (ch > 0 && base <= lo) return;                   // no layers in this chunk: nothing to add
+    store_block(a, pl, tile, ch, z, y0, x0, 0, cnB, acc, (size_t)tiles);
 }
 
 template <bool TAB, int SCP>
@@ -227,6 +235,20 @@ void fwd_fp32_batch_kernel(const BatchFwdItem* __restrict__ items, const int* __
     const FwdArgs a = items[k].a;
     const FwdPlan pl = items[k].pl;
     fwd_fp32_body<TAB, SCP>(a, pl, (int)blockIdx.x - wg0[k]);
+}
+
+__global__ __launch_bounds__(256) void combine_kernel(FwdArgs a, FwdPlan pl)
+{
+    combine_body(a, pl, blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void combine_batch_kernel(const BatchFwdItem* __restrict__ items,
+                                                            const int* __restrict__ wg0, int n)
+{
+    const int k = batch_item(wg0, n, blockIdx.x);
+    const FwdArgs a = items[k].a;
+    const FwdPlan pl = items[k].pl;
+    combine_body(a, pl, (int)blockIdx.x - wg0[k]);
 }
 
 }  // namespace pbh
@@ -304,38 +326,53 @@ static int device_cus()
 
 static long long tile_count(const FwdArgs& a) { return (long long)a.tiles_x * a.tiles_y * a.nz; }
 
-// Smallest split (1, 2, 4, 8) that gives at least one workgroup per CU.
-int pbh_auto_split(const FwdArgs& a)
+constexpr int FT_MIN_CHUNK = 4;   // smallest automatic layer chunk
+
+// Layer chunk for about four workgroups per resident slot (two per CU): the
+// chunks of shallow slices (many active layers) then spread over more
+// workgroups than those of deep slices, and the dispatcher balances them.
+int pbh_auto_chunk(const FwdArgs& a)
 {
-    const long long target = (long long)device_cus();
-    int s = 1;
-    while (s < 8 && tile_count(a) * s < target && s * 2 <= a.L) s *= 2;
-    return s;
+    const long long slots = 2LL * device_cus();
+    const long long per_tile = (4 * slots + tile_count(a) - 1) / tile_count(a);
+    if (per_tile <= 1) return a.L;
+    int chunk = (int)((a.L + per_tile - 1) / per_tile);
+    return chunk < FT_MIN_CHUNK ? (a.L < FT_MIN_CHUNK ? a.L : FT_MIN_CHUNK) : chunk;
 }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Workspace layout: [counters][layer peaks (mixed "_peak" forms)][partial tiles].
-size_t pbh_forward_ws_bytes(const FwdArgs& a, int split)
+static int nchunks(const FwdArgs& a, int chunk) { return (a.L + chunk - 1) / chunk; }
+
+// Workspace layout: [layer peaks (mixed "_peak" forms)][split 16-bit weights
+// (mixed forms)][partial tiles of layer chunks 1 .. nchunk - 1].
+static size_t ws_peaks(const FwdArgs& a) { return align256((size_t)a.L * sizeof(float)); }
+static size_t ws_wsplit(const FwdArgs& a)
 {
-    size_t n = align256((size_t)tile_count(a) * sizeof(unsigned));
-    n += align256((size_t)a.L * sizeof(float));
-    if (split > 1) n += (size_t)split * tile_count(a) * FT_TY * FT_TX * sizeof(float);
+    return align256((size_t)a.L * chunk_count(a.nsy) * chunk_count(a.nsx) * FT_WSPLIT * sizeof(unsigned short));
+}
+
+size_t pbh_forward_ws_bytes(const FwdArgs& a, int layer_chunk)
+{
+    const int chunk = layer_chunk > 0 ? layer_chunk : pbh_auto_chunk(a);
+    size_t n = ws_peaks(a) + ws_wsplit(a);
+    n += (size_t)(nchunks(a, chunk) - 1) * tile_count(a) * FT_TY * FT_TX * sizeof(float);
     return n;
 }
 
 float* pbh_forward_ws_peaks(const FwdArgs& a, void* ws)
 {
-    return (float*)((char*)ws + align256((size_t)tile_count(a) * sizeof(unsigned)));
+    return (float*)ws;
 }
 
 int pbh_plan_forward(const FwdArgs& a, int kx_pad, int ky_pad, void* ws, size_t ws_bytes,
-                     int split_override, FwdPlan* plan)
+                     int layer_chunk, FwdPlan* plan)
 {
-    int split = split_override > 0 ? split_override : pbh_auto_split(a);
-    if (split != 1 && split != 2 && split != 4 && split != 8) return PBH_ERR_UNSUPPORTED;
-    if (tile_count(a) * split > 0x7fffffffLL) return PBH_ERR_UNSUPPORTED;
-    plan->split = split;
+    if (layer_chunk < 0) return PBH_ERR_BAD_DIM;
+    const int chunk = layer_chunk > 0 ? (layer_chunk < a.L ? layer_chunk : a.L) : pbh_auto_chunk(a);
+    plan->chunk = chunk;
+    plan->nchunk = nchunks(a, chunk);
+    if (tile_count(a) * plan->nchunk > 0x7fffffffLL) return PBH_ERR_UNSUPPORTED;
     plan->table = axis_table(a.dx, a.dsx, FT_TX, kx_pad, &plan->tx) &&
                   axis_table(a.dy, a.dsy, FT_TY, ky_pad, &plan->ty);
     if (!plan->table) {
@@ -343,46 +380,68 @@ int pbh_plan_forward(const FwdArgs& a, int kx_pad, int ky_pad, void* ws, size_t 
         plan->ty = AxisTable{0, 0, 0.0, 0};
     }
     plan->part = nullptr;
-    plan->ctr = nullptr;
-    if (split > 1) {
-        if (!ws) return PBH_ERR_NULL_ARG;
-        if (ws_bytes < pbh_forward_ws_bytes(a, split)) return PBH_ERR_WORKSPACE;
+    plan->wsplit = nullptr;
+    const size_t need = pbh_forward_ws_bytes(a, chunk);
+    if (ws && ws_bytes >= need) {
         char* p = (char*)ws;
-        plan->ctr = (unsigned*)p;
-        plan->part = (float*)(p + align256((size_t)tile_count(a) * sizeof(unsigned))
-                              + align256((size_t)a.L * sizeof(float)));
+        plan->wsplit = (unsigned short*)(p + ws_peaks(a));
+        if (plan->nchunk > 1) plan->part = (float*)(p + ws_peaks(a) + ws_wsplit(a));
+    }
+    if (plan->nchunk > 1) {
+        if (!ws) return PBH_ERR_NULL_ARG;
+        if (ws_bytes < need) return PBH_ERR_WORKSPACE;
     }
     return PBH_OK;
 }
 
-int pbh_forward_mixed(const FwdArgs& a, double oz, double dz, int precision, int split,
+int pbh_launch_combine(const FwdArgs& a, const FwdPlan& pl, hipStream_t s)
+{
+    if (pl.nchunk <= 1) return PBH_OK;
+    hipLaunchKernelGGL(combine_kernel, dim3((unsigned)tile_count(a)), dim3(256), 0, s, a, pl);
+    return launch_rc("combine_kernel");
+}
+
+int pbh_launch_combine_batch(const BatchFwdItem* items, const int* wg0, int n, int total, hipStream_t s)
+{
+    hipLaunchKernelGGL(combine_batch_kernel, dim3((unsigned)total), dim3(256), 0, s, items, wg0, n);
+    return launch_rc("combine_batch_kernel");
+}
+
+int pbh_forward_mixed(const FwdArgs& a, double oz, double dz, int precision, int layer_chunk,
                       void* ws, size_t ws_bytes, hipStream_t s);
+
+extern "C" size_t pbh_forward_workspace_bytes_tuned(const PbhGrid* g, const PbhLattice* lat, int precision,
+                                                   int layer_chunk)
+{
+    if (validate_problem(g, lat) != PBH_OK || layer_chunk < 0) return 0;
+    (void)precision;
+    const FwdArgs a = make_fwd_args(g, lat, nullptr, nullptr, nullptr, nullptr);
+    return pbh_forward_ws_bytes(a, layer_chunk > 0 && layer_chunk < a.L ? layer_chunk : (layer_chunk ? a.L : 0));
+}
 
 extern "C" size_t pbh_forward_workspace_bytes(const PbhGrid* g, const PbhLattice* lat, int precision)
 {
-    if (validate_problem(g, lat) != PBH_OK) return 0;
-    (void)precision;
-    const FwdArgs a = make_fwd_args(g, lat, nullptr, nullptr, nullptr, nullptr);
-    return pbh_forward_ws_bytes(a, 8);   // enough for any layer split
+    return pbh_forward_workspace_bytes_tuned(g, lat, precision, 0);
 }
 
 extern "C" int pbh_forward_tuned(const PbhGrid* g, const PbhLattice* lat, const float* w,
                                  const double* idd, const double* sigma, float* dose, int precision,
-                                 int split, void* workspace, size_t workspace_bytes, void* stream)
+                                 int layer_chunk, void* workspace, size_t workspace_bytes, void* stream)
 {
     int rc = validate_problem(g, lat);
     if (rc != PBH_OK) return rc;
     if (!w || !idd || !sigma || !dose) return PBH_ERR_NULL_ARG;
     if (precision < PBH_PREC_FP32 || precision > PBH_PREC_FP16X3_PEAK) return PBH_ERR_UNSUPPORTED;
-    if (split < 0) return PBH_ERR_BAD_DIM;
+    if (layer_chunk < 0) return PBH_ERR_BAD_DIM;
     const FwdArgs a = make_fwd_args(g, lat, w, idd, sigma, dose);
     hipStream_t s = (hipStream_t)stream;
     if (precision != PBH_PREC_FP32)
-        return pbh_forward_mixed(a, g->oz, g->dz, precision, split, workspace, workspace_bytes, s);
+        return pbh_forward_mixed(a, g->oz, g->dz, precision, layer_chunk, workspace, workspace_bytes, s);
     FwdPlan pl;
-    rc = pbh_plan_forward(a, chunk_pad(a.nsx, a.nsy), chunk_pad(a.nsx, a.nsy), workspace, workspace_bytes, split, &pl);
+    rc = pbh_plan_forward(a, chunk_pad(a.nsx, a.nsy), chunk_pad(a.nsx, a.nsy), workspace, workspace_bytes,
+                          layer_chunk, &pl);
     if (rc != PBH_OK) return rc;
-    const dim3 grid((unsigned)(tile_count(a) * pl.split));
+    const dim3 grid((unsigned)(tile_count(a) * pl.nchunk));
     switch (chunk_pad(a.nsx, a.nsy)) {
 #define PBH_FP32_CASE(N) \
     case N: \
@@ -395,7 +454,8 @@ extern "C" int pbh_forward_tuned(const PbhGrid* g, const PbhLattice* lat, const 
 #undef PBH_FP32_CASE
     default: return PBH_ERR_UNSUPPORTED;
     }
-    return launch_rc("fwd_fp32_kernel");
+    if ((rc = launch_rc("fwd_fp32_kernel")) != PBH_OK) return rc;
+    return pbh_launch_combine(a, pl, s);
 }
 
 extern "C" int pbh_forward(const PbhGrid* g, const PbhLattice* lat, const float* w,

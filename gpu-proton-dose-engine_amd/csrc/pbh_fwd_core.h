@@ -11,6 +11,9 @@ constexpr int FT_TY = 32;        // dose tile rows (y) per workgroup
 constexpr int FT_TX = 128;       // dose tile columns (x) per workgroup
 constexpr int FT_THREADS = 256;  // 4 waves; two workgroups share a CU
 constexpr int FT_TAB = 1024;     // entries per axis table
+constexpr int FT_KYP = 64;       // mixed forms: stage A K (spots in y) padded to 2 steps of 32
+// mixed forms: 16-bit (hi, lo) weights of one (layer, spot chunk), [hi|lo][i < 48][j < 64]
+constexpr int FT_WSPLIT = 2 * PBH_SPOT_CHUNK * FT_KYP;
 
 // Regular-grid geometry of one axis: voxel a (in a tile) and spot i (in a chunk)
 // are d = C + (P a - Q i) u apart, so the Gaussian of a (layer, depth) takes
@@ -30,12 +33,21 @@ __device__ __forceinline__ int table_slot(const AxisTable& g, int t)
 }
 
 // Host-side launch plan of a forward call.
+//
+// Layer chunks: the active layers of a depth slice (IDD != 0, in layer order)
+// are cut into chunks of at most `chunk` layers, and chunk c of a dose tile is
+// computed by its own workgroup (blocks c * tiles + tile). Chunk 0 writes the
+// dose; chunk c >= 1 writes a partial tile, and combine_kernel adds the
+// partials to the dose in chunk order, so results are deterministic and no
+// workgroup waits for another. Chunks keep every workgroup to a bounded amount
+// of work although shallow slices have many more active layers than deep ones.
 struct FwdPlan {
-    int split;          // workgroups per dose tile sharing its layers (1, 2, 4 or 8)
+    int chunk;          // at most this many active layers per workgroup (>= 1)
+    int nchunk;         // workgroups per dose tile: ceil(L / chunk)
     int table;          // 1: factors gathered from per-(layer, depth) tables
     AxisTable tx, ty;
-    float* part;        // [split][tiles][FT_TY * FT_TX] partial tiles (split > 1)
-    unsigned* ctr;      // [tiles] arrival counters, multiples of 8 between calls
+    float* part;        // [nchunk - 1][tiles][FT_TY * FT_TX] partial tiles (nchunk > 1)
+    unsigned short* wsplit;   // mixed forms: [L][chunk y][chunk x][FT_WSPLIT] split weights
 };
 
 // Table entry t of an axis (t = Q i - P a + P (T - 1)) for base offset C:
@@ -46,50 +58,22 @@ __device__ __forceinline__ double table_distance(const AxisTable& g, double C, i
 }
 
 // Writes a 32 x 32 block of the dose tile held in the C layout of a 32x32 MFMA
-// (row 8 (r/4) + 4 (lane/32) + r%4, column lane%32) at (rm, cn) in the tile.
-// With a layer split, every workgroup stores its partial tile, and the last of
-// the `split` workgroups of the tile to arrive (atomic counter) adds all
-// partials in split order 0..S-1 and writes the dose, so the result does not
-// depend on arrival order. Counters are never reset: each arrival adds 8/S
-// (S in {2, 4, 8}), so a counter is a multiple of 8 between calls whatever S
-// was used, and the last arrival is the one that brings it to a multiple of 8.
-__device__ inline void store_block(const FwdArgs& a, const FwdPlan& pl, int tile, int s,
+// (row 8 (r/4) + 4 (lane/32) + r%4, column lane%32) at (rm, cn) in the tile:
+// chunk 0 to the dose (rows and columns inside the grid), chunk c >= 1 to its
+// partial tile (all 32 x 128 entries).
+__device__ inline void store_block(const FwdArgs& a, const FwdPlan& pl, int tile, int c,
                                    int z, int y0, int x0, int rm, int cn, v16f acc, size_t tiles)
 {
-    __shared__ int last_flag;
     const int lane = threadIdx.x & 63;
     const int xl = cn + (lane & 31);
-    if (pl.split > 1) {
-        float* mine = pl.part + ((size_t)s * tiles + tile) * (FT_TY * FT_TX);
+    if (c > 0) {
+        float* mine = pl.part + ((size_t)(c - 1) * tiles + tile) * (FT_TY * FT_TX);
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int row = rm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
             mine[row * FT_TX + xl] = acc[r];
         }
-        __threadfence();
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const unsigned inc = 8u / (unsigned)pl.split;
-            const unsigned old = atomicAdd(&pl.ctr[tile], inc);
-            last_flag = ((old + inc) & 7u) == 0u;
-        }
-        __syncthreads();
-        if (!last_flag) return;
-        __threadfence();
-        v16f sum = {};
-        for (int q = 0; q < pl.split; ++q) {
-            if (q == s) {
-                sum += acc;
-            } else {
-                const float* p = pl.part + ((size_t)q * tiles + tile) * (FT_TY * FT_TX);
-#pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const int row = rm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                    sum[r] += p[row * FT_TX + xl];
-                }
-            }
-        }
-        acc = sum;
+        return;
     }
     const int xB = x0 + xl;
     if (xB < a.nx) {
@@ -102,7 +86,42 @@ __device__ inline void store_block(const FwdArgs& a, const FwdPlan& pl, int tile
     }
 }
 
-constexpr int FT_LB = 256;   // layers gathered per pass
+// Active layers (IDD != 0) of depth slice z, counted by one workgroup.
+__device__ inline int count_active(const FwdArgs& a, int z, int* scratch)
+{
+    int n = 0;
+    for (int l = threadIdx.x; l < a.L; l += blockDim.x) n += a.idd[(size_t)l * a.nz + z] != 0.0;
+    for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o, 64);
+    if ((threadIdx.x & 63) == 0) scratch[threadIdx.x >> 6] = n;
+    __syncthreads();
+    int t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += scratch[w];
+    return t;
+}
+
+// dose tile += partial tiles of chunks 1 .. nchunk(z) - 1, in chunk order.
+// One workgroup of 256 threads per dose tile.
+__device__ inline void combine_body(const FwdArgs& a, const FwdPlan& pl, int tile)
+{
+    __shared__ int cnt[4];
+    const size_t tiles = (size_t)a.tiles_x * a.tiles_y * a.nz;
+    const int tx = tile % a.tiles_x;
+    const int ty = (tile / a.tiles_x) % a.tiles_y;
+    const int z = tile / (a.tiles_x * a.tiles_y);
+    const int nact = count_active(a, z, cnt);
+    const int nch = nact > pl.chunk ? (nact + pl.chunk - 1) / pl.chunk : 1;
+    if (nch <= 1) return;
+    for (int e = threadIdx.x; e < FT_TY * FT_TX; e += blockDim.x) {
+        const int y = ty * FT_TY + e / FT_TX, x = tx * FT_TX + e % FT_TX;
+        if (y >= a.ny || x >= a.nx) continue;
+        float* d = a.dose + ((size_t)z * a.ny + y) * a.nx + x;
+        float v = *d;
+        for (int c = 1; c < nch; ++c) v += pl.part[((size_t)(c - 1) * tiles + tile) * (FT_TY * FT_TX) + e];
+        *d = v;
+    }
+}
+
+constexpr int FT_LB = 64;    // layers gathered per pass
 
 // Active layers of one depth slice, in layer order, with their FP64-derived
 // scalars: c = IDD / (2 pi s^2), k2 = log2(e) / (2 s^2) (FP32 and FP64) and the
@@ -116,15 +135,15 @@ struct LayerList {
     int wcount[FT_THREADS / 64];
 };
 
-// Gathers layers l = lb + t * step (t < FT_LB, l < L) whose IDD at depth z is
-// non-zero into ll (stable order, wave ballots + prefix); returns their count.
+// Gathers layers l = lb + t (t < FT_LB, l < L) whose IDD at depth z is non-zero
+// into ll (stable order, wave ballots + prefix); returns their count.
 // Contains two barriers: every thread of the workgroup must call it.
-__device__ inline int collect_layers(const FwdArgs& a, int lb, int step, int z, float zc,
+__device__ inline int collect_layers(const FwdArgs& a, int lb, int z, float zc,
                                      const float* r_peak, LayerList& ll)
 {
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
-    const int l = lb + t * step;
+    const int l = lb + t;
     double iddv = 0.0, sg = 0.0;
     bool act = false;
     if (t < FT_LB && l < a.L) {
@@ -203,6 +222,7 @@ struct BatchPeakItem {
 
 // Host helpers (defined in pbh_forward.hip).
 int pbh_plan_forward(const pbh::FwdArgs& a, int kx_pad, int ky_pad, void* ws, size_t ws_bytes,
-                     int split_override, pbh::FwdPlan* plan);
-size_t pbh_forward_ws_bytes(const pbh::FwdArgs& a, int split);
-int pbh_auto_split(const pbh::FwdArgs& a);
+                     int layer_chunk, pbh::FwdPlan* plan);
+size_t pbh_forward_ws_bytes(const pbh::FwdArgs& a, int layer_chunk);
+int pbh_auto_chunk(const pbh::FwdArgs& a);
+int pbh_launch_combine(const pbh::FwdArgs& a, const pbh::FwdPlan& pl, hipStream_t s);

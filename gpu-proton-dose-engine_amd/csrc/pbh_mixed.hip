@@ -3,76 +3,39 @@
 //
 // Same tiling, layer split, Gaussian tables and double buffering as the FP32
 // kernel (pbh_forward.hip):
-//   stage A  U[64 x 48]  = Gy[64 x J] W[J x I]      v_mfma_f32_16x16x32_{bf16,f16}
-//   stage B  D[64 x 128] += (c U) Gx^T              v_mfma_f32_32x32x16_{bf16,f16}
-// Factors are formed in FP32 (table: FP64 argument), then rounded to the 16-bit
-// type. In the split ("x3") forms every operand a is stored as hi = T(a),
-// lo = T(a - hi) and a product is hi*hi + hi*lo + lo*hi (lo*lo dropped), which
-// restores ~2^-16 (BF16) or ~2^-21 (FP16) relative accuracy per product at
-// three MFMAs instead of one. Table entries hold the (hi, lo) pair packed in 32
-// bits; operands are assembled with v_perm. The "_peak" forms use split products
-// only for (layer, depth) pairs within 1 cm of the layer's Bragg peak (depth of
-// its IDD maximum) and single products elsewhere.
+//   stage A  U^T[48 x 32] = W^T[48 x J] Gy^T[J x 32]   v_mfma_f32_16x16x32_{bf16,f16}
+//   stage B  D[32 x 128] += (c U) Gx^T                 v_mfma_f32_32x32x16_{bf16,f16}
+// Stage A computes U transposed so that each lane's four results are four
+// consecutive spots of one row of U, stored with one 8-byte LDS write per
+// (hi, lo) half. Factors are formed in FP32 (table: FP64 argument), then rounded
+// to the 16-bit type. In the split ("x3") forms every operand a is stored as
+// hi = T(a), lo = T(a - hi) and a product is hi*hi + hi*lo + lo*hi (lo*lo
+// dropped), which restores ~2^-16 (BF16) or ~2^-21 (FP16) relative accuracy per
+// product at three MFMAs instead of one. Table entries hold the (hi, lo) pair
+// packed in 32 bits; operands are assembled with v_perm. The weights are split
+// once per call by wsplit_kernel into the layout the kernel copies to LDS with
+// 16-byte loads and stores. The "_peak" forms use split products only for
+// (layer, depth) pairs within 1 cm of the layer's Bragg peak (depth of its IDD
+// maximum) and single products elsewhere.
+//
+// On the 2 mm voxel / 5 mm spot geometry (P = 2, Q = 5) the table slots of a
+// lane's operands are a base register plus compile-time offsets (QC = 5); other
+// geometries compute the slots at run time (QC = 0).
 //
 // Each layer's contribution is summed in fresh FP32 accumulators and added to
 // the running dose once. FP16 has a narrow exponent range, so U is scaled per
-// wave by a power of two (largest magnitude into [1, 2)) and the inverse scale
-// is applied to the layer's partial when it joins the running dose. BF16
+// row block by a power of two (largest magnitude into [1, 2)) and the inverse
+// scale is applied to the layer's partial when it joins the running dose. BF16
 // shares FP32's exponent range and needs no scaling.
-#include "pbh_fwd_core.h"
+#include "pbh_mixed_core.h"
 
 namespace pbh {
 
 constexpr int SCH = PBH_SPOT_CHUNK;       // 48 spots per chunk
-constexpr int KYP = 64;                   // stage A K (j) padded to 2 steps of 32
+constexpr int KYP = FT_KYP;               // stage A K (j) padded to 2 steps of 32
 constexpr int WT_STRIDE = KYP + 8;        // W^T rows in halves
 constexpr int U_STRIDE = SCH + 8;         // U rows in halves
-
-enum { MODE_PLAIN = 0, MODE_X3 = 1, MODE_PEAK = 2 };
-
-typedef unsigned u4v __attribute__((ext_vector_type(4)));
-
-template <typename T> struct Vec8;
-template <> struct Vec8<__bf16> { typedef v8bf type; };
-template <> struct Vec8<_Float16> { typedef v8h type; };
-
-__device__ __forceinline__ v4f mfma16(v8bf a, v8bf b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
-__device__ __forceinline__ v4f mfma16(v8h a, v8h b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
-__device__ __forceinline__ v16f mfma32(v8bf a, v8bf b, v16f c) { return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0); }
-__device__ __forceinline__ v16f mfma32(v8h a, v8h b, v16f c) { return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0); }
-
-template <typename T>
-__device__ __forceinline__ void split(float x, T& hi, T& lo)
-{
-    hi = (T)x;
-    lo = (T)(x - (float)hi);
-}
-
-template <typename T>
-__device__ __forceinline__ unsigned short bits16(T v) { return __builtin_bit_cast(unsigned short, v); }
-
-// (hi, lo) of x packed as hi | lo << 16
-template <typename T>
-__device__ __forceinline__ unsigned pack_split(float x)
-{
-    T h, l;
-    split<T>(x, h, l);
-    return (unsigned)bits16(h) | ((unsigned)bits16(l) << 16);
-}
-
-// Eight packed entries -> hi and lo operand vectors.
-template <typename V8>
-__device__ __forceinline__ void unpack8(const unsigned* t, V8& hi, V8& lo)
-{
-    u4v h, l;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        h[q] = __builtin_amdgcn_perm(t[2 * q + 1], t[2 * q], 0x05040100u);
-        l[q] = __builtin_amdgcn_perm(t[2 * q + 1], t[2 * q], 0x07060302u);
-    }
-    hi = __builtin_bit_cast(V8, h);
-    lo = __builtin_bit_cast(V8, l);
-}
+constexpr int WV = SCH * KYP / 8;         // 16-byte vectors per split half of a weight chunk
 
 // r_peak[l] = depth of the first IDD maximum of layer l (np.argmax semantics).
 __global__ void peak_depth_f32_kernel(const double* idd, int L, int nz, double oz, double dz, float* r_peak)
@@ -86,14 +49,50 @@ __global__ void peak_depth_f32_kernel(const double* idd, int L, int nz, double o
     r_peak[l] = (float)(oz + (double)best * dz);
 }
 
+// Split 16-bit weights of one (layer, spot chunk): hi and lo at [i][j] (i < 48,
+// j < 64), zero outside the chunk. blk = (layer * chunks_y + chunk_y) * chunks_x + chunk_x.
+template <typename T>
+__device__ __forceinline__ void wsplit_body(const FwdArgs& a, unsigned short* dst, int blk)
+{
+    const int ncx = chunk_count(a.nsx), ncy = chunk_count(a.nsy);
+    const int cx = blk % ncx, cy = (blk / ncx) % ncy, l = blk / (ncx * ncy);
+    if (l >= a.L) return;
+    const int i0 = chunk_begin(a.nsx, ncx, cx), ni = chunk_begin(a.nsx, ncx, cx + 1) - i0;
+    const int j0 = chunk_begin(a.nsy, ncy, cy), nj = chunk_begin(a.nsy, ncy, cy + 1) - j0;
+    const float* wl = a.w + (size_t)l * a.nsy * a.nsx;
+    unsigned short* hi = dst + (size_t)blk * FT_WSPLIT;
+    unsigned short* lo = hi + SCH * KYP;
+    for (int e = threadIdx.x; e < SCH * KYP; e += blockDim.x) {
+        const int i = e % SCH, j = e / SCH;
+        const float v = (i < ni && j < nj) ? wl[(size_t)(j0 + j) * a.nsx + i0 + i] : 0.0f;
+        T h, r;
+        split<T>(v, h, r);
+        hi[i * KYP + j] = bits16(h);
+        lo[i * KYP + j] = bits16(r);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void wsplit_kernel(FwdArgs a, unsigned short* dst)
+{
+    wsplit_body<T>(a, dst, blockIdx.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void wsplit_batch_kernel(const BatchFwdItem* __restrict__ items)
+{
+    const FwdArgs a = items[blockIdx.y].a;
+    wsplit_body<T>(a, items[blockIdx.y].pl.wsplit, blockIdx.x);
+}
+
 // 4 waves (two workgroups per CU), one-barrier software pipeline over the
 // active layers of each spot chunk, as in the FP32 kernel. Stage A: 16x16 blocks
-// of U (2 row blocks x 3 column blocks); wave w takes block (w % 2, w / 2) and,
-// for w < 2, block (w % 2, 2). Stage B: wave w owns the 32 x 32 dose block at
-// columns 32 w; KB K steps of 16 spots.
+// of U^T (3 spot blocks x 2 row blocks); wave w takes spot block w / 2 of row
+// block w % 2 and, for w < 2, spot block 2 of the same row block. Stage B: wave
+// w owns the 32 x 32 dose block at columns 32 w; KB K steps of 16 spots.
 //
-// blk: the workgroup's index in the problem's own grid (tile * split + share).
-template <typename T, int MODE, bool TAB, int KB>
+// blk: the workgroup's index in the problem's own grid (chunk * tiles + tile).
+template <typename T, int MODE, bool TAB, int KB, int QC>
 __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& pl, double oz, double dz,
                                                const float* r_peak, int blk)
 {
@@ -101,51 +100,61 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
     constexpr bool F16 = !__is_same(T, __bf16);
     constexpr bool SPLIT = MODE != MODE_PLAIN;
     constexpr int KX = 16 * KB;                 // stage B K (spots in x), padded
-    constexpr int NW = (SCH * KYP + FT_THREADS - 1) / FT_THREADS;   // W elements per thread
+    constexpr int NWV = SPLIT ? 2 * WV : WV;    // 16-byte weight vectors per chunk and layer
+    constexpr int NV = (NWV + FT_THREADS - 1) / FT_THREADS;
 
     __shared__ __attribute__((aligned(16))) T Wh[2][SCH * WT_STRIDE];      // W^T hi [i][j]
     __shared__ __attribute__((aligned(16))) T Wl[2][SPLIT ? SCH * WT_STRIDE : 8];
     __shared__ __attribute__((aligned(16))) T Uh[2][FT_TY * U_STRIDE];     // c U hi [y][i]
     __shared__ __attribute__((aligned(16))) T Ul[2][SPLIT ? FT_TY * U_STRIDE : 8];
-    __shared__ unsigned Tx[TAB ? 2 : 1][TAB ? FT_TAB : 1];
-    __shared__ unsigned Ty[TAB ? 2 : 1][TAB ? FT_TAB : 1];
+    // table slots: exact interleaved size on the P = 2, Q = QC geometry, FT_TAB otherwise
+    constexpr int TXN = !TAB ? 1 : QC ? 2 * ((2 * (FT_TX - 1) + QC * (KX - 1) + 2) / 2) + 2 : FT_TAB;
+    constexpr int TYN = !TAB ? 1 : QC ? 2 * ((2 * (FT_TY - 1) + QC * (KYP - 1) + 2) / 2) + 2 : FT_TAB;
+    __shared__ unsigned Tx[TAB ? 2 : 1][TXN];
+    __shared__ unsigned Ty[TAB ? 2 : 1][TYN];
     __shared__ float uscale[2][2];
     __shared__ float bmax[2][4];
     __shared__ LayerList ll;
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int s = blk % pl.split;
-    const int tile = blk / pl.split;
+    const int tiles = a.tiles_x * a.tiles_y * a.nz;
+    const int ch = blk / tiles;                     // layer chunk of this workgroup
+    const int tile = blk % tiles;
     const int tx = tile % a.tiles_x;
     const int ty = (tile / a.tiles_x) % a.tiles_y;
     const int z = tile / (a.tiles_x * a.tiles_y);
     const int x0 = tx * FT_TX, y0 = ty * FT_TY;
     const float zc = (float)(oz + (double)z * dz);
 
-    const int rbA = wave & 1;                       // stage A row block
-    const int bA = 16 * rbA + (lane & 15);          // stage A row in the tile
+    const int rbA = wave & 1;                       // stage A row block (y)
+    const int yA = 16 * rbA + (lane & 15);          // stage A row of the tile: B-operand column, output row
     const int gA = lane >> 4;                       // stage A k group (8 values)
-    const int cbA0 = wave >> 1;                     // stage A column blocks: cbA0 and, w < 2, 2
+    const int cbA0 = wave >> 1;                     // stage A spot blocks: cbA0 and, w < 2, 2
     const bool twoA = wave < 2;
     const int cnB = 32 * wave;                      // stage B columns
     const int aB = cnB + (lane & 31);
     const int gB = lane >> 5;                       // stage B k group (8 values)
-    const double yAc = a.oy + (double)(y0 + bA) * a.dy;
+    const double yAc = a.oy + (double)(y0 + yA) * a.dy;
     const double xBc = a.ox + (double)(x0 + aB) * a.dx;
-    const int tA0 = 8 * pl.ty.Q * gA - pl.ty.P * bA + pl.ty.P * (FT_TY - 1);
-    const int tB0 = 8 * pl.tx.Q * gB - pl.tx.P * aB + pl.tx.P * (FT_TX - 1);
+    TabStream<QC> sA, sB;
+    sA.init(pl.ty, 8 * pl.ty.Q * gA - pl.ty.P * yA + pl.ty.P * (FT_TY - 1));
+    sB.init(pl.tx, 8 * pl.tx.Q * gB - pl.tx.P * aB + pl.tx.P * (FT_TX - 1));
     const int ntY = pl.ty.P * (FT_TY - 1) + pl.ty.Q * (KYP - 1) + 1;
     const int ntX = pl.tx.P * (FT_TX - 1) + pl.tx.Q * (KX - 1) + 1;
 
     const int ncx = chunk_count(a.nsx), ncy = chunk_count(a.nsy);
 
     v16f acc = {};
-    float wreg[NW];
+    u4v wreg[NV];
 
-    for (int lb = s; lb < a.L; lb += FT_LB * pl.split) {
-        const int nact = collect_layers(a, lb, pl.split, z, zc, r_peak, ll);
-        if (nact == 0) continue;
+    const int lo = ch * pl.chunk, hi = lo + pl.chunk;   // this workgroup's active layers
+    int base = 0;                                       // active layers of earlier passes
+    for (int lb = 0; lb < a.L && base < hi; lb += FT_LB) {
+        const int nact = collect_layers(a, lb, z, zc, r_peak, ll);
+        const int m0 = max(0, lo - base), m1 = min(nact, hi - base);
+        base += nact;
+        if (m1 <= m0) continue;
         for (int cy = 0; cy < ncy; ++cy) {
             const int j0 = chunk_begin(a.nsy, ncy, cy), nj = chunk_begin(a.nsy, ncy, cy + 1) - j0;
             const int nkA = TAB ? 2 : ((nj + 31) >> 5);
@@ -153,6 +162,8 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
             for (int cx = 0; cx < ncx; ++cx) {
                 const int i0 = chunk_begin(a.nsx, ncx, cx), ni = chunk_begin(a.nsx, ncx, cx + 1) - i0;
                 const double Cx = (a.ox + (double)x0 * a.dx) - (a.sx0 + (double)i0 * a.dsx);
+                const unsigned short* wchunk = pl.wsplit + ((size_t)cy * ncx + cx) * FT_WSPLIT;
+                const size_t wlayer = (size_t)ncy * ncx * FT_WSPLIT;
 
                 auto is_x3 = [&](int m) {
                     if (MODE == MODE_X3) return true;
@@ -160,24 +171,21 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                     return ll.zpk[m] <= 1.0f;
                 };
                 auto load_w = [&](int m) {
-                    const float* wl = a.w + (size_t)ll.lay[m] * a.nsy * a.nsx + (size_t)j0 * a.nsx + i0;
+                    const u4v* src = (const u4v*)(wchunk + (size_t)ll.lay[m] * wlayer);
 #pragma unroll
-                    for (int q = 0; q < NW; ++q) {
-                        const int e = threadIdx.x + q * FT_THREADS;
-                        const int i = e / KYP, j = e % KYP;
-                        wreg[q] = (e < SCH * KYP && i < ni && j < nj) ? wl[(size_t)j * a.nsx + i] : 0.0f;
+                    for (int q = 0; q < NV; ++q) {
+                        const int v = threadIdx.x + q * FT_THREADS;
+                        if (v < NWV) wreg[q] = src[v];
                     }
                 };
                 auto store_w = [&](int b) {
 #pragma unroll
-                    for (int q = 0; q < NW; ++q) {
-                        const int e = threadIdx.x + q * FT_THREADS;
-                        if (e < SCH * KYP) {
-                            T h, lo;
-                            split<T>(wreg[q], h, lo);
-                            const int off = (e / KYP) * WT_STRIDE + (e % KYP);
-                            Wh[b][off] = h;
-                            if (SPLIT) Wl[b][off] = lo;
+                    for (int q = 0; q < NV; ++q) {
+                        const int v = threadIdx.x + q * FT_THREADS;
+                        if (v < NWV) {
+                            const int e8 = v % WV;
+                            T* dst = (v < WV ? Wh[b] : Wl[b]) + (e8 >> 3) * WT_STRIDE + 8 * (e8 & 7);
+                            *(u4v*)dst = wreg[q];
                         }
                     }
                 };
@@ -199,7 +207,7 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                             pack_split<T>(__builtin_amdgcn_exp2f((float)(-(d * d) * k2d)));
                     }
                 };
-                // stage A of layer m: U[m&1] = c Gy W, split into (hi, lo)
+                // stage A of layer m: U[m&1] = c Gy W (computed as W^T Gy^T), split into (hi, lo)
                 auto stage_a = [&](int m) {
                     const int b = m & 1;
                     const float c = ll.c[m], k2 = ll.k2[m];
@@ -208,13 +216,12 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
 #pragma unroll
                     for (int ks = 0; ks < 2; ++ks) {
                         if (ks < nkA) {
-                            V8 ah, al;
+                            V8 gh, gl;      // Gy^T operand: j = 32 ks + 8 gA + e, y = yA
                             if (TAB) {
                                 unsigned t8[8];
 #pragma unroll
-                                for (int e = 0; e < 8; ++e)
-                                    t8[e] = Ty[b][table_slot(pl.ty, tA0 + pl.ty.Q * (32 * ks + e))];
-                                unpack8(t8, ah, al);
+                                for (int e = 0; e < 8; ++e) t8[e] = Ty[b][sA.slot(pl.ty, 32 * ks + e)];
+                                unpack8(t8, gh, gl);
                             } else {
 #pragma unroll
                                 for (int e = 0; e < 8; ++e) {
@@ -223,25 +230,25 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                                     const float g = (j < nj) ? gauss2((float)(d * d), k2) : 0.0f;
                                     T h, lo;
                                     split<T>(g, h, lo);
-                                    ah[e] = h;
-                                    al[e] = lo;
+                                    gh[e] = h;
+                                    gl[e] = lo;
                                 }
                             }
                             const int kofs = 32 * ks + 8 * gA;
                             const int off0 = (16 * cbA0 + (lane & 15)) * WT_STRIDE + kofs;
                             const int off1 = (32 + (lane & 15)) * WT_STRIDE + kofs;
-                            const V8 b0h = *(const V8*)&Wh[b][off0];
-                            u0 = mfma16(ah, b0h, u0);
+                            const V8 w0h = *(const V8*)&Wh[b][off0];
+                            u0 = mfma16(w0h, gh, u0);
                             if (SPLIT && x3) {
-                                u0 = mfma16(ah, *(const V8*)&Wl[b][off0], u0);
-                                u0 = mfma16(al, b0h, u0);
+                                u0 = mfma16(w0h, gl, u0);
+                                u0 = mfma16(*(const V8*)&Wl[b][off0], gh, u0);
                             }
                             if (twoA) {
-                                const V8 b1h = *(const V8*)&Wh[b][off1];
-                                u1 = mfma16(ah, b1h, u1);
+                                const V8 w1h = *(const V8*)&Wh[b][off1];
+                                u1 = mfma16(w1h, gh, u1);
                                 if (SPLIT && x3) {
-                                    u1 = mfma16(ah, *(const V8*)&Wl[b][off1], u1);
-                                    u1 = mfma16(al, b1h, u1);
+                                    u1 = mfma16(w1h, gl, u1);
+                                    u1 = mfma16(*(const V8*)&Wl[b][off1], gh, u1);
                                 }
                             }
                         }
@@ -267,20 +274,24 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                         }
                         if (lane == 0 && wave < 2) uscale[b][wave] = scale;
                     }
-                    const int rbase = 16 * rbA + 4 * gA;
+                    // C layout of U^T (16x16): spots 16 cb + 4 gA + r (r < 4), row yA:
+                    // four consecutive spots of U's row yA, one 8-byte store per half
+                    auto put = [&](v4f u, int cb) {
+                        u2v hv, lv;
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int row = (rbase + r) * U_STRIDE + (lane & 15);
-                        T h, lo;
-                        split<T>(c * u0[r] * inv_scale, h, lo);
-                        Uh[b][row + 16 * cbA0] = h;
-                        if (SPLIT) Ul[b][row + 16 * cbA0] = lo;
-                        if (twoA) {
-                            split<T>(c * u1[r] * inv_scale, h, lo);
-                            Uh[b][row + 32] = h;
-                            if (SPLIT) Ul[b][row + 32] = lo;
+                        for (int q = 0; q < 2; ++q) {
+                            T h0, l0, h1, l1;
+                            split<T>(c * u[2 * q] * inv_scale, h0, l0);
+                            split<T>(c * u[2 * q + 1] * inv_scale, h1, l1);
+                            hv[q] = (unsigned)bits16(h0) | ((unsigned)bits16(h1) << 16);
+                            lv[q] = (unsigned)bits16(l0) | ((unsigned)bits16(l1) << 16);
                         }
-                    }
+                        const int off = yA * U_STRIDE + 16 * cb + 4 * gA;
+                        *(u2v*)&Uh[b][off] = hv;
+                        if (SPLIT) *(u2v*)&Ul[b][off] = lv;
+                    };
+                    put(u0, cbA0);
+                    if (twoA) put(u1, 2);
                 };
                 // stage B of layer m: acc += (c U) Gx^T
                 auto stage_b = [&](int m) {
@@ -293,8 +304,7 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                         if (TAB) {
                             unsigned t8[8];
 #pragma unroll
-                            for (int e = 0; e < 8; ++e)
-                                t8[e] = Tx[b][table_slot(pl.tx, tB0 + pl.tx.Q * (16 * ks + e))];
+                            for (int e = 0; e < 8; ++e) t8[e] = Tx[b][sB.slot(pl.tx, 16 * ks + e)];
                             unpack8(t8, bh[ks], bl[ks]);
                         } else {
 #pragma unroll
@@ -330,26 +340,26 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                     }
                 };
 
-                load_w(0);
-                store_w(0);
-                build_ty(0);
-                if (nact > 1) load_w(1);
+                load_w(m0);
+                store_w(m0 & 1);
+                build_ty(m0);
+                if (m0 + 1 < m1) load_w(m0 + 1);
                 __syncthreads();
-                stage_a(0);
-                build_tx(0);
-                if (nact > 1) {
-                    store_w(1);
-                    build_ty(1);
+                stage_a(m0);
+                build_tx(m0);
+                if (m0 + 1 < m1) {
+                    store_w((m0 + 1) & 1);
+                    build_ty(m0 + 1);
                 }
                 __syncthreads();
-                for (int m = 0; m < nact; ++m) {
-                    if (m + 2 < nact) load_w(m + 2);
+                for (int m = m0; m < m1; ++m) {
+                    if (m + 2 < m1) load_w(m + 2);
                     stage_b(m);
-                    if (m + 1 < nact) {
+                    if (m + 1 < m1) {
                         stage_a(m + 1);
                         build_tx(m + 1);
                     }
-                    if (m + 2 < nact) {
+                    if (m + 2 < m1) {
                         store_w(m & 1);
                         build_ty(m + 2);
                     }
@@ -358,26 +368,31 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
             }
         }
     }
-    store_block(a, pl, tile, s, z, y0, x0, 0, cnB, acc, (size_t)a.tiles_x * a.tiles_y * a.nz);
+    if (ch > 0 && base <= lo) return;                   // no layers in this chunk: nothing to add
+    store_block(a, pl, tile, ch, z, y0, x0, 0, cnB, acc, (size_t)tiles);
 }
 
-template <typename T, int MODE, bool TAB, int KB>
-__global__ __launch_bounds__(FT_THREADS, 2)
+// Workgroups per CU: three on the P = 2, Q = 5 geometry (exact-size tables keep
+// LDS near 51 KB), two otherwise.
+template <int QC> struct MixedOcc { static constexpr int value = QC ? 3 : 2; };
+
+template <typename T, int MODE, bool TAB, int KB, int QC>
+__global__ __launch_bounds__(FT_THREADS, MixedOcc<QC>::value)
 void fwd_mixed_kernel(FwdArgs a, FwdPlan pl, double oz, double dz, const float* r_peak)
 {
-    fwd_mixed_body<T, MODE, TAB, KB>(a, pl, oz, dz, r_peak, blockIdx.x);
+    fwd_mixed_body<T, MODE, TAB, KB, QC>(a, pl, oz, dz, r_peak, blockIdx.x);
 }
 
 // One launch over a batch of problems sharing the template parameters.
-template <typename T, int MODE, bool TAB, int KB>
-__global__ __launch_bounds__(FT_THREADS, 2)
+template <typename T, int MODE, bool TAB, int KB, int QC>
+__global__ __launch_bounds__(FT_THREADS, MixedOcc<QC>::value)
 void fwd_mixed_batch_kernel(const BatchFwdItem* __restrict__ items, const int* __restrict__ wg0, int n)
 {
     const int k = batch_item(wg0, n, blockIdx.x);
     const FwdArgs a = items[k].a;
     const FwdPlan pl = items[k].pl;
-    fwd_mixed_body<T, MODE, TAB, KB>(a, pl, items[k].oz, items[k].dz, items[k].r_peak,
-                                     (int)blockIdx.x - wg0[k]);
+    fwd_mixed_body<T, MODE, TAB, KB, QC>(a, pl, items[k].oz, items[k].dz, items[k].r_peak,
+                                         (int)blockIdx.x - wg0[k]);
 }
 
 // r_peak of every layer of every item (one workgroup per item)
@@ -393,39 +408,88 @@ __global__ void peak_depth_batch_kernel(const BatchPeakItem* __restrict__ items)
     }
 }
 
-template <typename T, int MODE>
-static void launch_mixed_batch(const BatchFwdItem* items, const int* wg0, int n, dim3 grid, int kb,
-                               bool tab, hipStream_t s)
+// Template choice shared by the single-problem and batched launches.
+struct MixedKey {
+    int kb;       // stage B K steps (1..3)
+    bool tab;     // table factors
+    bool q5;      // P = 2, Q = 5 on both axes: compile-time table slots
+};
+
+static MixedKey mixed_key(const FwdPlan& pl, int kb)
 {
-    const dim3 block(FT_THREADS);
-    if (!tab) {
-        hipLaunchKernelGGL((fwd_mixed_batch_kernel<T, MODE, false, 3>), grid, block, 0, s, items, wg0, n);
-        return;
+    MixedKey k{kb, pl.table != 0, false};
+    k.q5 = k.tab && pl.tx.P == 2 && pl.tx.Q == 5 && pl.ty.P == 2 && pl.ty.Q == 5;
+    return k;
+}
+
+// Calls L.template go<T, MODE, TAB, KB, QC>() for the key's instantiation.
+template <typename T, int MODE, typename Launch>
+static int dispatch_mixed(const MixedKey& k, Launch&& L)
+{
+    if (!k.tab) return L.template go<T, MODE, false, 3, 0>();
+    if (k.q5) {
+        switch (k.kb) {
+        case 1: return L.template go<T, MODE, true, 1, 5>();
+        case 2: return L.template go<T, MODE, true, 2, 5>();
+        case 3: return L.template go<T, MODE, true, 3, 5>();
+        default: return PBH_ERR_UNSUPPORTED;
+        }
     }
-    switch (kb) {
-    case 1: hipLaunchKernelGGL((fwd_mixed_batch_kernel<T, MODE, true, 1>), grid, block, 0, s, items, wg0, n); break;
-    case 2: hipLaunchKernelGGL((fwd_mixed_batch_kernel<T, MODE, true, 2>), grid, block, 0, s, items, wg0, n); break;
-    default: hipLaunchKernelGGL((fwd_mixed_batch_kernel<T, MODE, true, 3>), grid, block, 0, s, items, wg0, n); break;
+    switch (k.kb) {
+    case 1: return L.template go<T, MODE, true, 1, 0>();
+    case 2: return L.template go<T, MODE, true, 2, 0>();
+    case 3: return L.template go<T, MODE, true, 3, 0>();
+    default: return PBH_ERR_UNSUPPORTED;
     }
 }
 
-template <typename T, int MODE>
-static int launch_mixed(const FwdArgs& a, const FwdPlan& pl, int kb, double oz, double dz,
-                        const float* rp, hipStream_t s)
+template <typename Launch>
+static int dispatch_precision(int precision, const MixedKey& k, Launch&& L)
 {
-    const dim3 grid((unsigned)((long long)a.tiles_x * a.tiles_y * a.nz * pl.split)), block(FT_THREADS);
-    if (!pl.table) {
-        hipLaunchKernelGGL((fwd_mixed_kernel<T, MODE, false, 3>), grid, block, 0, s, a, pl, oz, dz, rp);
-        return PBH_OK;
-    }
-    switch (kb) {
-    case 1: hipLaunchKernelGGL((fwd_mixed_kernel<T, MODE, true, 1>), grid, block, 0, s, a, pl, oz, dz, rp); break;
-    case 2: hipLaunchKernelGGL((fwd_mixed_kernel<T, MODE, true, 2>), grid, block, 0, s, a, pl, oz, dz, rp); break;
-    case 3: hipLaunchKernelGGL((fwd_mixed_kernel<T, MODE, true, 3>), grid, block, 0, s, a, pl, oz, dz, rp); break;
+    switch (precision) {
+    case PBH_PREC_BF16: return dispatch_mixed<__bf16, MODE_PLAIN>(k, L);
+    case PBH_PREC_FP16: return dispatch_mixed<_Float16, MODE_PLAIN>(k, L);
+    case PBH_PREC_BF16X3: return dispatch_mixed<__bf16, MODE_X3>(k, L);
+    case PBH_PREC_FP16X3: return dispatch_mixed<_Float16, MODE_X3>(k, L);
+    case PBH_PREC_BF16X3_PEAK: return dispatch_mixed<__bf16, MODE_PEAK>(k, L);
+    case PBH_PREC_FP16X3_PEAK: return dispatch_mixed<_Float16, MODE_PEAK>(k, L);
     default: return PBH_ERR_UNSUPPORTED;
     }
-    return PBH_OK;
 }
+
+static bool is_f16(int precision)
+{
+    return precision == PBH_PREC_FP16 || precision == PBH_PREC_FP16X3 || precision == PBH_PREC_FP16X3_PEAK;
+}
+
+struct SingleLaunch {
+    const FwdArgs& a;
+    const FwdPlan& pl;
+    double oz, dz;
+    const float* rp;
+    hipStream_t s;
+    template <typename T, int MODE, bool TAB, int KB, int QC>
+    int go() const
+    {
+        const dim3 grid((unsigned)((long long)a.tiles_x * a.tiles_y * a.nz * pl.nchunk)), block(FT_THREADS);
+        hipLaunchKernelGGL((fwd_mixed_kernel<T, MODE, TAB, KB, QC>), grid, block, 0, s, a, pl, oz, dz, rp);
+        return launch_rc("fwd_mixed_kernel");
+    }
+};
+
+struct BatchLaunch {
+    const BatchFwdItem* items;
+    const int* wg0;
+    int n, total;
+    hipStream_t s;
+    template <typename T, int MODE, bool TAB, int KB, int QC>
+    int go() const
+    {
+        hipLaunchKernelGGL((fwd_mixed_batch_kernel<T, MODE, TAB, KB, QC>), dim3((unsigned)total), dim3(FT_THREADS),
+                           0, s, items, wg0, n);
+        return launch_rc("fwd_mixed_batch_kernel");
+    }
+};
 
 }  // namespace pbh
 
@@ -433,33 +497,31 @@ using namespace pbh;
 
 float* pbh_forward_ws_peaks(const FwdArgs& a, void* ws);
 
-int pbh_forward_mixed(const FwdArgs& a, double oz, double dz, int precision, int split, void* ws,
+int pbh_forward_mixed(const FwdArgs& a, double oz, double dz, int precision, int layer_chunk, void* ws,
                       size_t ws_bytes, hipStream_t s)
 {
     const int kb = (chunk_pad(a.nsx, 1) + 15) / 16;   // stage B K steps of 16 spots
+    if (!ws) return PBH_ERR_NULL_ARG;
+    if (ws_bytes < pbh_forward_ws_bytes(a, a.L)) return PBH_ERR_WORKSPACE;
     FwdPlan pl;
-    int rc = pbh_plan_forward(a, 16 * kb, KYP, ws, ws_bytes, split, &pl);
+    int rc = pbh_plan_forward(a, 16 * kb, KYP, ws, ws_bytes, layer_chunk, &pl);
     if (rc != PBH_OK) return rc;
     float* r_peak = nullptr;
     if (precision == PBH_PREC_BF16X3_PEAK || precision == PBH_PREC_FP16X3_PEAK) {
-        if (!ws) return PBH_ERR_NULL_ARG;
-        if (ws_bytes < pbh_forward_ws_bytes(a, 1)) return PBH_ERR_WORKSPACE;
         r_peak = pbh_forward_ws_peaks(a, ws);
         hipLaunchKernelGGL(peak_depth_f32_kernel, dim3(cdiv(a.L, 64)), dim3(64), 0, s,
                            a.idd, a.L, a.nz, oz, dz, r_peak);
         if ((rc = launch_rc("peak_depth_f32_kernel")) != PBH_OK) return rc;
     }
-    switch (precision) {
-    case PBH_PREC_BF16: rc = launch_mixed<__bf16, MODE_PLAIN>(a, pl, kb, oz, dz, r_peak, s); break;
-    case PBH_PREC_FP16: rc = launch_mixed<_Float16, MODE_PLAIN>(a, pl, kb, oz, dz, r_peak, s); break;
-    case PBH_PREC_BF16X3: rc = launch_mixed<__bf16, MODE_X3>(a, pl, kb, oz, dz, r_peak, s); break;
-    case PBH_PREC_FP16X3: rc = launch_mixed<_Float16, MODE_X3>(a, pl, kb, oz, dz, r_peak, s); break;
-    case PBH_PREC_BF16X3_PEAK: rc = launch_mixed<__bf16, MODE_PEAK>(a, pl, kb, oz, dz, r_peak, s); break;
-    case PBH_PREC_FP16X3_PEAK: rc = launch_mixed<_Float16, MODE_PEAK>(a, pl, kb, oz, dz, r_peak, s); break;
-    default: return PBH_ERR_UNSUPPORTED;
-    }
-    if (rc != PBH_OK) return rc;
-    return launch_rc("fwd_mixed_kernel");
+    const dim3 wgrid((unsigned)(a.L * chunk_count(a.nsy) * chunk_count(a.nsx)));
+    if (is_f16(precision))
+        hipLaunchKernelGGL(wsplit_kernel<_Float16>, wgrid, dim3(256), 0, s, a, pl.wsplit);
+    else
+        hipLaunchKernelGGL(wsplit_kernel<__bf16>, wgrid, dim3(256), 0, s, a, pl.wsplit);
+    if ((rc = launch_rc("wsplit_kernel")) != PBH_OK) return rc;
+    if ((rc = dispatch_precision(precision, mixed_key(pl, kb), SingleLaunch{a, pl, oz, dz, r_peak, s})) != PBH_OK)
+        return rc;
+    return pbh_launch_combine(a, pl, s);
 }
 
 int pbh_launch_peak_batch(const BatchPeakItem* items, int n, hipStream_t s)
@@ -468,21 +530,29 @@ int pbh_launch_peak_batch(const BatchPeakItem* items, int n, hipStream_t s)
     return launch_rc("peak_depth_batch_kernel");
 }
 
-// Batched mixed-precision forward: items[0..n) share kb (stage B K steps) and
-// the table mode; total = wg0[n].
+// Split weights of every item of a batch (grid: max chunks x items).
+int pbh_launch_wsplit_batch(const BatchFwdItem* items, int n, int max_chunks, int precision, hipStream_t s)
+{
+    const dim3 grid((unsigned)max_chunks, (unsigned)n);
+    if (is_f16(precision))
+        hipLaunchKernelGGL(wsplit_batch_kernel<_Float16>, grid, dim3(256), 0, s, items);
+    else
+        hipLaunchKernelGGL(wsplit_batch_kernel<__bf16>, grid, dim3(256), 0, s, items);
+    return launch_rc("wsplit_batch_kernel");
+}
+
+// Batched mixed-precision forward: items[0..n) share kb (stage B K steps), the
+// table mode and the slot mode (q5); total = wg0[n].
 int pbh_launch_fwd_mixed_batch(const BatchFwdItem* items, const int* wg0, int n, int total, int precision,
-                               int kb, bool tab, hipStream_t s)
+                               int kb, bool tab, bool q5, hipStream_t s)
 {
     if (kb < 1 || kb > 3) return PBH_ERR_UNSUPPORTED;
-    const dim3 grid((unsigned)total);
-    switch (precision) {
-    case PBH_PREC_BF16: launch_mixed_batch<__bf16, MODE_PLAIN>(items, wg0, n, grid, kb, tab, s); break;
-    case PBH_PREC_FP16: launch_mixed_batch<_Float16, MODE_PLAIN>(items, wg0, n, grid, kb, tab, s); break;
-    case PBH_PREC_BF16X3: launch_mixed_batch<__bf16, MODE_X3>(items, wg0, n, grid, kb, tab, s); break;
-    case PBH_PREC_FP16X3: launch_mixed_batch<_Float16, MODE_X3>(items, wg0, n, grid, kb, tab, s); break;
-    case PBH_PREC_BF16X3_PEAK: launch_mixed_batch<__bf16, MODE_PEAK>(items, wg0, n, grid, kb, tab, s); break;
-    case PBH_PREC_FP16X3_PEAK: launch_mixed_batch<_Float16, MODE_PEAK>(items, wg0, n, grid, kb, tab, s); break;
-    default: return PBH_ERR_UNSUPPORTED;
-    }
-    return launch_rc("fwd_mixed_batch_kernel");
+    return dispatch_precision(precision, MixedKey{kb, tab, q5 && tab}, BatchLaunch{items, wg0, n, total, s});
+}
+
+// Template key of a planned item, for grouping a batch: bit 0 = table, bit 1 = q5.
+int pbh_mixed_slot_mode(const FwdPlan& pl)
+{
+    const MixedKey k = mixed_key(pl, 1);
+    return (k.tab ? 1 : 0) | (k.q5 ? 2 : 0);
 }

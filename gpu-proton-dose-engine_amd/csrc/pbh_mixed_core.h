@@ -1,0 +1,87 @@
+// pbh_mixed_core.h -- 16-bit operand helpers shared by the mixed-precision
+// forward and adjoint kernels: MFMA wrappers, (hi, lo) splitting and packing,
+// and compile-time table-slot streams.
+#pragma once
+
+#include "pbh_fwd_core.h"
+
+namespace pbh {
+
+enum { MODE_PLAIN = 0, MODE_X3 = 1, MODE_PEAK = 2 };
+
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
+typedef unsigned u2v __attribute__((ext_vector_type(2)));
+
+template <typename T> struct Vec8;
+template <> struct Vec8<__bf16> { typedef v8bf type; };
+template <> struct Vec8<_Float16> { typedef v8h type; };
+
+__device__ __forceinline__ v4f mfma16(v8bf a, v8bf b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+__device__ __forceinline__ v4f mfma16(v8h a, v8h b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+__device__ __forceinline__ v16f mfma32(v8bf a, v8bf b, v16f c) { return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0); }
+__device__ __forceinline__ v16f mfma32(v8h a, v8h b, v16f c) { return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0); }
+
+// Table slots of one lane's operand stream: entries t0 + Q k for k = 0, 1, ...
+// QC > 0: Q = QC at compile time and the table parity-interleaved (P == 2):
+// slot(t0 + c) = (c even ? bE : bO) + (c >> 1), offsets known at compile time.
+// QC == 0: slots computed from the run-time geometry.
+template <int QC>
+struct TabStream {
+    int bE, bO;
+    __device__ __forceinline__ void init(const AxisTable& g, int t0)
+    {
+        if (QC > 0) {
+            const int p0 = t0 & 1, h0 = t0 >> 1;
+            bE = (p0 ? g.half : 0) + h0;
+            bO = (p0 ? 0 : g.half) + h0 + p0;
+        } else {
+            bE = t0;
+            bO = 0;
+        }
+    }
+    // QC > 0: slot of entry t0 + c for any integer offset c (compile-time after unrolling)
+    __device__ __forceinline__ int at(int c) const { return ((c & 1) ? bO : bE) + (c >> 1); }
+    __device__ __forceinline__ int slot(const AxisTable& g, int k) const
+    {
+        if (QC > 0) {
+            const int c = QC * k;
+            return ((c & 1) ? bO : bE) + (c >> 1);
+        }
+        return table_slot(g, bE + g.Q * k);
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ void split(float x, T& hi, T& lo)
+{
+    hi = (T)x;
+    lo = (T)(x - (float)hi);
+}
+
+template <typename T>
+__device__ __forceinline__ unsigned short bits16(T v) { return __builtin_bit_cast(unsigned short, v); }
+
+// (hi, lo) of x packed as hi | lo << 16
+template <typename T>
+__device__ __forceinline__ unsigned pack_split(float x)
+{
+    T h, l;
+    split<T>(x, h, l);
+    return (unsigned)bits16(h) | ((unsigned)bits16(l) << 16);
+}
+
+// Eight packed entries -> hi and lo operand vectors.
+template <typename V8>
+__device__ __forceinline__ void unpack8(const unsigned* t, V8& hi, V8& lo)
+{
+    u4v h, l;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        h[q] = __builtin_amdgcn_perm(t[2 * q + 1], t[2 * q], 0x05040100u);
+        l[q] = __builtin_amdgcn_perm(t[2 * q + 1], t[2 * q], 0x07060302u);
+    }
+    hi = __builtin_bit_cast(V8, h);
+    lo = __builtin_bit_cast(V8, l);
+}
+
+}  // namespace pbh

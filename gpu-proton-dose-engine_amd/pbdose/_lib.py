@@ -46,7 +46,7 @@ class PbhBatchItem(C.Structure):
 
 
 class PbhBatchTuning(C.Structure):
-    _fields_ = [("fwd_split", C.c_int32), ("adj_split", C.c_int32), ("group", C.c_int32),
+    _fields_ = [("fwd_chunk", C.c_int32), ("adj_split", C.c_int32), ("group", C.c_int32),
                 ("order", C.c_int32)]
 
 
@@ -67,10 +67,12 @@ SIGNATURES = {
     "pbh_build_lut": (C.c_int, [_M, C.c_int, C.c_double, C.c_double, C.c_int, _P, _P, _P, _P,
                                 _P, C.c_size_t, _P]),
     "pbh_forward_workspace_bytes": (C.c_size_t, [_G, _L, C.c_int]),
+    "pbh_forward_workspace_bytes_tuned": (C.c_size_t, [_G, _L, C.c_int, C.c_int]),
     "pbh_forward": (C.c_int, [_G, _L, _P, _P, _P, _P, C.c_int, _P, C.c_size_t, _P]),
     "pbh_forward_tuned": (C.c_int, [_G, _L, _P, _P, _P, _P, C.c_int, C.c_int, _P, C.c_size_t, _P]),
     "pbh_adjoint_workspace_bytes": (C.c_size_t, [_G, _L, C.c_int]),
     "pbh_adjoint": (C.c_int, [_G, _L, _P, _P, _P, _P, _P, _P, _P, C.c_int, _P, C.c_size_t, _P]),
+    "pbh_adjoint_tuned": (C.c_int, [_G, _L, _P, _P, _P, _P, _P, _P, _P, C.c_int, C.c_int, _P, C.c_size_t, _P]),
     "pbh_objective_workspace_bytes": (C.c_size_t, [_G]),
     "pbh_objective": (C.c_int, [_G, _P, _P, _P, _P, _P, _P, C.c_size_t, _P]),
     "pbh_lut_chain": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_int, _P, _P]),

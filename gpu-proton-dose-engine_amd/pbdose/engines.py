@@ -49,18 +49,22 @@ class DeviceProblem:
         _lib.check(_lib.lib().pbh_validate(self.grid_s, self.lat_s), "pbh_validate")
         self._ws = {}
 
-    def workspace(self, kind: str, precision: int) -> torch.Tensor:
-        key = (kind, precision)
+    def workspace(self, kind: str, precision: int, layer_chunk: int = 0) -> torch.Tensor:
+        key = (kind, precision, layer_chunk)
         if key not in self._ws:
             L = _lib.lib()
-            q = {"fwd": L.pbh_forward_workspace_bytes, "adj": L.pbh_adjoint_workspace_bytes}[kind]
-            self._ws[key] = _lib.workspace(q(self.grid_s, self.lat_s, precision), self.dev)
+            if kind == "fwd":
+                n = L.pbh_forward_workspace_bytes_tuned(self.grid_s, self.lat_s, precision, layer_chunk)
+            else:
+                n = L.pbh_adjoint_workspace_bytes(self.grid_s, self.lat_s, precision)
+            self._ws[key] = _lib.workspace(n, self.dev)
         return self._ws[key]
 
     def forward(self, precision: str = "fp32", out: Optional[torch.Tensor] = None,
-                w: Optional[torch.Tensor] = None, split: int = 0) -> torch.Tensor:
+                w: Optional[torch.Tensor] = None, layer_chunk: int = 0) -> torch.Tensor:
         """Dose (nz, ny, nx) float32, asynchronous on the current stream.
-        ``split``: workgroups per dose tile sharing its layers (0 = automatic)."""
+        ``layer_chunk``: at most this many active layers of a depth slice per
+        workgroup (0 = automatic, >= L = one workgroup per dose tile)."""
         p = _lib.PREC[precision]
         if out is None:
             out = torch.empty(self.shape, dtype=torch.float32, device=self.dev)
@@ -69,10 +73,10 @@ class DeviceProblem:
         wt = self.w if w is None else w
         if wt.dtype != torch.float32 or tuple(wt.shape) != tuple(self.w.shape) or not wt.is_contiguous():
             raise ValueError("weights must be a contiguous float32 (L, nsy, nsx) tensor")
-        ws = self.workspace("fwd", p)
+        ws = self.workspace("fwd", p, int(layer_chunk))
         _lib.check(_lib.lib().pbh_forward_tuned(self.grid_s, self.lat_s, _lib.ptr(wt),
                                                 _lib.ptr(self.idd), _lib.ptr(self.sigma),
-                                                _lib.ptr(out), p, int(split), _lib.ptr(ws),
+                                                _lib.ptr(out), p, int(layer_chunk), _lib.ptr(ws),
                                                 ws.numel(), _lib.stream_of(self.dev)),
                    "pbh_forward_tuned")
         return out

@@ -98,9 +98,15 @@ def objective(dp: DeviceProblem, dose: torch.Tensor, presc: torch.Tensor,
 
 
 def adjoint(dp: DeviceProblem, R: torch.Tensor, w: Optional[torch.Tensor] = None,
-            lut_grads: bool = False):
+            lut_grads: bool = False, split: int = 0, precision: str = "fp32"):
     """dJ/dw (L, nsy, nsx) FP32 for the dose cotangent R; with ``lut_grads`` also
-    dJ/dIDD and dJ/dsigma (L, nz) FP64."""
+    dJ/dIDD and dJ/dsigma (L, nz) FP64 (FP32 arithmetic only). ``split``:
+    workgroups per depth slice sharing its layers (0 = automatic).
+    ``precision``: 'fp32', or 'bf16' / 'bf16x3' (BF16 MFMA, dJ/dw only)."""
+    if precision not in ("fp32", "bf16", "bf16x3"):
+        raise ValueError(f"adjoint precision {precision!r}: 'fp32', 'bf16' or 'bf16x3'")
+    if lut_grads and precision != "fp32":
+        raise ValueError("table gradients (lut_grads) are computed in FP32 only")
     L = _lib.lib()
     if R.dtype != torch.float32 or tuple(R.shape) != tuple(dp.shape):
         raise ValueError("R must be float32 with the dose shape")
@@ -110,9 +116,10 @@ def adjoint(dp: DeviceProblem, R: torch.Tensor, w: Optional[torch.Tensor] = None
     gi = torch.empty_like(dp.idd) if lut_grads else None
     gs = torch.empty_like(dp.sigma) if lut_grads else None
     ws = dp.workspace("adj", 0)
-    _lib.check(L.pbh_adjoint(dp.grid_s, dp.lat_s, _lib.ptr(wt), _lib.ptr(dp.idd), _lib.ptr(dp.sigma),
-                             _lib.ptr(R), _lib.ptr(gw), _lib.ptr(gi), _lib.ptr(gs), 0, _lib.ptr(ws),
-                             ws.numel(), _lib.stream_of(dp.dev)), "pbh_adjoint")
+    _lib.check(L.pbh_adjoint_tuned(dp.grid_s, dp.lat_s, _lib.ptr(wt), _lib.ptr(dp.idd), _lib.ptr(dp.sigma),
+                                   _lib.ptr(R), _lib.ptr(gw), _lib.ptr(gi), _lib.ptr(gs), _lib.PREC[precision],
+                                   int(split),
+                                   _lib.ptr(ws), ws.numel(), _lib.stream_of(dp.dev)), "pbh_adjoint_tuned")
     return (gw, gi, gs) if lut_grads else gw
 
 

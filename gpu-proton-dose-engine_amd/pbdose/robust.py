@@ -14,8 +14,8 @@ is an ordinary PBProblem:
 The prescription and voxel weights stay attached to the patient grid.
 The standard set has 21 scenarios: {nominal, +-dx, +-dy, +-dz} x {1, 1 -+ r}.
 
-Launch parameters (forward and adjoint layer splits, one forward launch per
-spot-chunk width or one at the widest, item order) are chosen by ``autotune``:
+Launch parameters (forward layer chunks, adjoint layer split, one forward
+launch per spot-chunk width or one at the widest, item order) are chosen by ``autotune``:
 every candidate is built and timed on the device, and the winner is cached per
 batch signature in a JSON file.
 """
@@ -106,10 +106,12 @@ def central_cube(grid: DoseGrid, half: float = 3.0, dose_value: float = 2.0, w_i
 # ---------------------------------------------------------------------------
 # tuning
 # ---------------------------------------------------------------------------
-DEFAULT_TUNING = {"fwd_split": 1, "adj_split": 0, "group": 1, "order": 1}
+DEFAULT_TUNING = {"fwd_chunk": 0, "adj_split": 0, "group": 1, "order": 1}
 
-CANDIDATES = [{"fwd_split": f, "adj_split": a, "group": g, "order": o}
-              for f in (1, 2) for a in (1, 2, 4) for g in (0, 1) for o in (0, 1)]
+# forward: automatic layer chunks per item, or one chunk (all layers of a tile
+# in one workgroup); adjoint: workgroups per depth slice
+CANDIDATES = [{"fwd_chunk": f, "adj_split": a, "group": g, "order": o}
+              for f in (0, 4096) for a in (2, 4, 8) for g in (0, 1) for o in (0, 1)]
 
 
 def _cache_path() -> str:
@@ -141,7 +143,7 @@ def _save_cache(c: dict) -> None:
 
 
 def tuning_struct(t: dict) -> _lib.PbhBatchTuning:
-    return _lib.PbhBatchTuning(int(t["fwd_split"]), int(t["adj_split"]), int(t["group"]), int(t["order"]))
+    return _lib.PbhBatchTuning(int(t["fwd_chunk"]), int(t["adj_split"]), int(t["group"]), int(t["order"]))
 
 
 # ---------------------------------------------------------------------------

@@ -135,7 +135,11 @@ int pbh_build_lut(const PbhBeamModel* m, int nz, double z0, double dz,
 /* ---------------------------------------------------------------------------
  * Forward dose.
  * ------------------------------------------------------------------------- */
+/* Workspace of pbh_forward (automatic layer chunk) and of pbh_forward_tuned
+ * with a given layer_chunk (0 = automatic). */
 size_t pbh_forward_workspace_bytes(const PbhGrid* g, const PbhLattice* lat, int precision);
+size_t pbh_forward_workspace_bytes_tuned(const PbhGrid* g, const PbhLattice* lat, int precision,
+                                         int layer_chunk);
 
 /* dose (nz, ny, nx) FP32 := sum_l IDD_l Gy_l W_l Gx_l^T for weights w
  * (L, nsy, nsx) FP32 and FP64 tables idd, sigma (L, nz). The dose array is
@@ -145,15 +149,16 @@ int pbh_forward(const PbhGrid* g, const PbhLattice* lat,
                 float* dose, int precision,
                 void* workspace, size_t workspace_bytes, void* stream);
 
-/* pbh_forward with an explicit layer split: `split` workgroups share the
- * layers of each dose tile (1, 2, 4 or 8; 0 = automatic). Results do not depend
- * on the split beyond FP32 summation order. The workspace of a forward call
- * (pbh_forward_workspace_bytes, valid for every split) must be zero-filled
- * before its first use; calls leave it ready for the next call. Calls that
- * share one workspace must not overlap. */
+/* pbh_forward with an explicit layer chunk: the active layers (IDD != 0) of
+ * each depth slice are cut into chunks of at most `layer_chunk` layers (0 =
+ * automatic; >= L: one chunk), each computed by its own workgroup; a second
+ * kernel adds the chunks' partial tiles to the dose in chunk order, so results
+ * are deterministic and differ between chunk sizes only by FP32 summation
+ * order. The workspace (pbh_forward_workspace_bytes_tuned) needs no
+ * initialisation. Calls that share one workspace must not overlap. */
 int pbh_forward_tuned(const PbhGrid* g, const PbhLattice* lat,
                       const float* w, const double* idd, const double* sigma,
-                      float* dose, int precision, int split,
+                      float* dose, int precision, int layer_chunk,
                       void* workspace, size_t workspace_bytes, void* stream);
 
 /* ---------------------------------------------------------------------------
@@ -162,7 +167,11 @@ int pbh_forward_tuned(const PbhGrid* g, const PbhLattice* lat,
 size_t pbh_adjoint_workspace_bytes(const PbhGrid* g, const PbhLattice* lat, int precision);
 
 /* grad_w (L, nsy, nsx) FP32 := sum_z IDD_l(z) Gy_l(z)^T R_z Gx_l(z) for a dose
- * cotangent R (nz, ny, nx). Deterministic (no atomics).
+ * cotangent R (nz, ny, nx). Deterministic (no atomics). precision:
+ * PBH_PREC_FP32, or PBH_PREC_BF16 / PBH_PREC_BF16X3 (BF16 MFMA, plain or split
+ * products, FP32 accumulation) when no table gradient is requested and the
+ * voxel and spot pitches have a small integer ratio (table geometry); other
+ * combinations return PBH_ERR_UNSUPPORTED.
  * If grad_sigma != NULL it also writes grad_sigma (L, nz) FP64:
  *     d<R, dose>/d sigma_l(z)  and, if grad_idd != NULL, grad_idd (L, nz) FP64:
  *     d<R, dose>/d IDD_l(z). */
@@ -170,6 +179,14 @@ int pbh_adjoint(const PbhGrid* g, const PbhLattice* lat,
                 const float* w, const double* idd, const double* sigma,
                 const float* R, float* grad_w, double* grad_idd, double* grad_sigma,
                 int precision, void* workspace, size_t workspace_bytes, void* stream);
+
+/* pbh_adjoint with an explicit layer split: `split` workgroups share the layers
+ * of each depth slice (1, 2, 4, 8 or 16; 0 = automatic). Results do not depend
+ * on the split. */
+int pbh_adjoint_tuned(const PbhGrid* g, const PbhLattice* lat,
+                      const float* w, const double* idd, const double* sigma,
+                      const float* R, float* grad_w, double* grad_idd, double* grad_sigma,
+                      int precision, int split, void* workspace, size_t workspace_bytes, void* stream);
 
 /* R = 2 wv (dose - presc) and J = sum wv (dose - presc)^2 (J written to
  * j_out[0] as FP64). Fused residual of the weighted-quadratic objective. */
@@ -210,7 +227,8 @@ typedef struct PbhBatchItem {
 } PbhBatchItem;
 
 typedef struct PbhBatchTuning {
-    int32_t fwd_split;       /* forward layer split 1, 2, 4 or 8 (0 = 1) */
+    int32_t fwd_chunk;       /* forward layer chunk: at most this many active layers per
+                                workgroup (0 = automatic per item) */
     int32_t adj_split;       /* adjoint layer split 1, 2, 4, 8 or 16 (0 = per-item default) */
     int32_t group;           /* 0: one forward launch at the widest spot chunk;
                                 1: one forward launch per chunk width */
@@ -231,6 +249,10 @@ size_t pbh_batch_workspace_bytes(const PbhBatchItem* items, int n_items, int pre
 int pbh_batch_create(const PbhBatchItem* items, int n_items, int precision,
                      const PbhBatchTuning* tuning, void* workspace, size_t workspace_bytes,
                      void* stream, PbhBatch** out);
+
+/* The adjoint of a batch follows `precision`: FP32 for PBH_PREC_FP32, BF16 for
+ * the plain 16-bit forms, BF16 split products for the x3 forms; items that
+ * request table gradients or have no table geometry use the FP32 adjoint. */
 
 /* Runs the whole batch on `stream`: kernel launches only (no copies, memsets
  * or allocations), so it can be captured into a HIP graph and replayed.

@@ -63,6 +63,19 @@ def test_forward_rejects_null_and_bad_precision_without_launching():
     assert L.pbh_forward(g, lat, one, one, one, one, 99, nul, 0, nul) == -5
 
 
+def test_forward_workspace_query():
+    L = _lib.lib()
+    g = _lib.PbhGrid(128, 128, 128, 0, 0.2, 0.2, 0.2, -12.7, -12.7, 0.0)
+    lat = _lib.PbhLattice(41, 41, 40, 0, 0.5, 0.5, 0.0, 0.0)
+    one = L.pbh_forward_workspace_bytes_tuned(g, lat, 3, 40)
+    assert one >= 40 * 2 * 48 * 64 * 2                       # split 16-bit weights
+    # chunks of 10 layers: 3 partial tiles of 32 x 128 FP32 per dose tile
+    assert L.pbh_forward_workspace_bytes_tuned(g, lat, 3, 10) - one >= 3 * 512 * 32 * 128 * 4
+    assert L.pbh_forward_workspace_bytes_tuned(g, lat, 3, 4096) == one
+    assert L.pbh_forward_workspace_bytes_tuned(g, lat, 3, -1) == 0
+    assert L.pbh_forward_workspace_bytes(g, lat, 3) >= one
+
+
 def test_lut_workspace_query():
     from pbdose import physics
     m = physics.beam_model()
@@ -100,7 +113,7 @@ def _batch_items(n=3):
 def test_batch_workspace_query_and_argument_checks():
     L = _lib.lib()
     items = _batch_items()
-    t = _lib.PbhBatchTuning(1, 0, 1, 1)
+    t = _lib.PbhBatchTuning(4096, 0, 1, 1)                              # one layer chunk
     n = L.pbh_batch_workspace_bytes(items, 3, 3, C.byref(t))
     # at least: the residual and per-slice partials of the two items with a prescription
     need = sum(64 * 48 * 32 * 4 + 32 * (2 + k) * 7 * (9 + k) * 4 for k in (0, 2))
@@ -108,11 +121,13 @@ def test_batch_workspace_query_and_argument_checks():
     assert L.pbh_batch_workspace_bytes(items, 3, 3, None) > 0          # default tuning
     assert L.pbh_batch_workspace_bytes(items, 0, 3, None) == 0          # empty batch
     assert L.pbh_batch_workspace_bytes(items, 3, 99, None) == 0         # bad precision
-    bad = _lib.PbhBatchTuning(3, 0, 0, 0)                               # split 3
+    bad = _lib.PbhBatchTuning(0, 3, 0, 0)                               # adjoint split 3
     assert L.pbh_batch_workspace_bytes(items, 3, 3, C.byref(bad)) == 0
-    # a forward split needs room for its partial tiles
-    t8 = _lib.PbhBatchTuning(8, 0, 1, 1)
-    assert L.pbh_batch_workspace_bytes(items, 3, 3, C.byref(t8)) > n
+    bad = _lib.PbhBatchTuning(-1, 0, 0, 0)                              # negative layer chunk
+    assert L.pbh_batch_workspace_bytes(items, 3, 3, C.byref(bad)) == 0
+    # layer chunks of one layer need room for the partial tiles of later chunks
+    t1 = _lib.PbhBatchTuning(1, 0, 1, 1)
+    assert L.pbh_batch_workspace_bytes(items, 3, 3, C.byref(t1)) > n
     items[1].dose = None
     assert L.pbh_batch_workspace_bytes(items, 3, 3, None) == 0
     items[1].dose = 256
@@ -124,7 +139,7 @@ def test_batch_create_rejects_before_touching_the_device():
     L = _lib.lib()
     items = _batch_items()
     out = C.c_void_p()
-    t = _lib.PbhBatchTuning(1, 0, 1, 1)
+    t = _lib.PbhBatchTuning(4096, 0, 1, 1)
     assert L.pbh_batch_create(items, 3, 3, C.byref(t), None, 0, None, C.byref(out)) == -1
     assert L.pbh_batch_create(items, 3, 3, C.byref(t), C.c_void_p(256), 16, None, C.byref(out)) == -7
     assert L.pbh_batch_create(items, 3, 3, C.byref(t), C.c_void_p(256), 16, None, None) == -1

@@ -174,3 +174,56 @@ def test_autograd_matches_torch_restatement_of_reference_engine():
     e = rel(gw.cpu(), wl.grad.numpy())
     print(f"  hand-written adjoint vs torch autograd (FP64): {e:.2e}", flush=True)
     assert e <= 2e-5
+
+
+MIXED_ADJ_TOL = {"bf16x3": 3e-5, "bf16": 1e-2}
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "bf16"])
+@pytest.mark.parametrize("geom", [(40, 36, 24, 11, 9, 0.2, 0.5), (70, 150, 6, 60, 50, 0.25, 0.3),
+                                  (130, 65, 5, 7, 13, 0.2, 0.5)])
+def test_mixed_adjoint_matches_oracle(precision, geom):
+    """BF16 MFMA adjoint (dJ/dw): 2 mm / 5 mm pitch (compile-time table slots),
+    2.5 mm / 3 mm (run-time slots), ragged tiles and multi-chunk lattices."""
+    from pbdose import engines, planning
+    nx, ny, nz, nsx, nsy, d, ds = geom
+    print(f"case mixed adjoint {precision} {geom}", flush=True)
+    p = small_problem(nx=nx, ny=ny, nz=nz, d=d, nsx=nsx, nsy=nsy, ds=ds)
+    dp = engines.DeviceProblem(p)
+    R = np.random.default_rng(4).standard_normal(p.grid.shape)
+    Rt = torch.as_tensor(R, dtype=torch.float32, device=dp.dev)
+    gw = planning.adjoint(dp, Rt, precision=precision)
+    ref = O.adjoint_w(p.grid.x, p.grid.y, p.lattice.spot_x, p.lattice.spot_y, p.idd, p.sigma,
+                      R.astype(np.float32).astype(np.float64))
+    e = rel(gw.cpu(), ref)
+    g2 = planning.adjoint(dp, Rt, precision=precision).clone()
+    assert torch.equal(gw, g2), "mixed adjoint not deterministic"
+    print(f"  dJ/dw vs oracle {e:.2e}", flush=True)
+    assert e <= MIXED_ADJ_TOL[precision]
+
+
+def test_mixed_adjoint_clinical_field_and_unsupported_cases():
+    """128^3 / 41x41 / 40 layers, R of the weighted-quadratic objective: the BF16x3
+    adjoint against the FP32 adjoint (itself within 2e-7 of the oracle); table
+    gradients and direct-factor geometries refuse the BF16 form."""
+    from pbdose import engines, model, physics, planning
+    print("case mixed adjoint on the 128^3 field", flush=True)
+    grid = model.DoseGrid.centered(128, 0.2)
+    E = np.linspace(70.0, 180.0, 40)
+    w = np.random.default_rng(seed=0).lognormal(0.0, 0.5, (40, 41, 41))
+    idd, sig, _ = physics.build_luts(grid.nz, grid.oz, grid.dz, E)
+    lat = model.SpotLattice(41, 41, 40, 0.5, 0.5, energies=E, weights=w)
+    dp = engines.DeviceProblem(model.PBProblem(grid, lat, idd.cpu().numpy(), sig.cpu().numpy()), idd=idd, sigma=sig)
+    Dp, wv = planning.central_cube(grid)
+    _, R = planning.objective(dp, dp.forward("bf16x3"), Dp, wv)
+    g32 = planning.adjoint(dp, R)
+    for prec in ("bf16x3", "bf16"):
+        e = rel(planning.adjoint(dp, R, precision=prec).cpu(), g32.cpu())
+        print(f"  {prec} vs FP32 adjoint: {e:.2e}", flush=True)
+        assert e <= MIXED_ADJ_TOL[prec]
+    with pytest.raises(ValueError):
+        planning.adjoint(dp, R, lut_grads=True, precision="bf16x3")
+    p = small_problem(nsx=13, nsy=7, ds=0.37)
+    dq = engines.DeviceProblem(p)
+    with pytest.raises(RuntimeError, match="unsupported"):
+        planning.adjoint(dq, torch.zeros(dq.shape, dtype=torch.float32, device=dq.dev), precision="bf16x3")

@@ -132,19 +132,21 @@ def test_mixed_ragged_and_chunked(precision):
 
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16x3", "fp16x3"])
-def test_layer_split_parity_and_determinism(precision):
+def test_layer_chunk_parity_and_determinism(precision):
+    """Active layers cut into chunks of 1, 3, 5, 7 and 16 (one chunk) per workgroup,
+    and the automatic chunk: every result matches the oracle and repeats bit for bit."""
     from pbdose import engines
-    print(f"case layer split sweep {precision}", flush=True)
+    print(f"case layer chunk sweep {precision}", flush=True)
     p = make_problem(128, 96, 12, 0.2, 41, 33, 0.5, np.linspace(80, 170, 16), seed=21)
     dp = engines.DeviceProblem(p)
     ref = oracle_dose(p)
     tol = TOL_FP32 if precision == "fp32" else MIXED_TOL[precision]
-    for split in (1, 2, 4, 8):
-        a = dp.forward(precision, split=split)
-        b = dp.forward(precision, split=split)
-        assert torch.equal(a, b), f"split {split} not deterministic"
+    for chunk in (0, 1, 3, 5, 7, 16):
+        a = dp.forward(precision, layer_chunk=chunk).clone()
+        b = dp.forward(precision, layer_chunk=chunk)
+        assert torch.equal(a, b), f"layer chunk {chunk} not deterministic"
         r = O.relative_error(a.double().cpu().numpy(), ref)["max_rel_err_global"]
-        print(f"  split {split}: {r:.3e}", flush=True)
+        print(f"  layer chunk {chunk}: {r:.3e}", flush=True)
         assert r <= tol
 
 

@@ -55,20 +55,32 @@ def oracle_item(field, sc, z_idx=None):
     return D, (sx, sy, idd, sig)
 
 
+def adjoint_form(precision):
+    """The adjoint arithmetic a batch pairs with a forward form."""
+    if precision == "fp32":
+        return "fp32"
+    return "bf16" if precision in ("bf16", "fp16") else "bf16x3"
+
+
 def single_path(b, k, precision):
-    """Dose, J and dJ/dw of item k through pbh_forward_tuned / pbh_objective / pbh_adjoint."""
+    """Dose, J and dJ/dw of item k through pbh_forward_tuned / pbh_objective / pbh_adjoint_tuned."""
     from pbdose import engines, planning
     fi, _ = b.items[k]
     dp = engines.DeviceProblem(b.problems[k], b.dev, weights=b.w[fi], idd=b.idd[k], sigma=b.sigma[k])
-    D = dp.forward(precision, split=1)
+    D = dp.forward(precision, layer_chunk=b.tuning["fwd_chunk"])
     J, R = planning.objective(dp, D, b.Dp[fi], b.wv[fi])
-    return D, J, planning.adjoint(dp, R)
+    try:
+        g = planning.adjoint(dp, R, precision=adjoint_form(precision))
+    except RuntimeError:
+        # no table geometry (direct factors): the batch uses the FP32 adjoint for this item
+        g = planning.adjoint(dp, R)
+    return D, J, g
 
 
 @pytest.mark.parametrize("precision,tuning", [
-    ("fp32", {"fwd_split": 1, "adj_split": 0, "group": 1, "order": 1}),
-    ("fp32", {"fwd_split": 2, "adj_split": 2, "group": 0, "order": 0}),
-    ("bf16x3", {"fwd_split": 1, "adj_split": 1, "group": 0, "order": 1}),
+    ("fp32", {"fwd_chunk": 0, "adj_split": 0, "group": 1, "order": 1}),
+    ("fp32", {"fwd_chunk": 2, "adj_split": 2, "group": 0, "order": 0}),
+    ("bf16x3", {"fwd_chunk": 4096, "adj_split": 1, "group": 0, "order": 1}),
 ])
 def test_small_batch_matches_oracle_and_single_path(precision, tuning):
     from pbdose import robust
@@ -93,7 +105,7 @@ def test_small_batch_matches_oracle_and_single_path(precision, tuning):
         e_sg = rel(b.grad_w[k].cpu(), g1.cpu())
         worst = [max(x, y) for x, y in zip(worst, (e_d, e_g, e_j, e_s, e_sg))]
         assert e_d <= tol, (k, e_d)
-        assert e_g <= 2e-5, (k, e_g)
+        assert e_g <= (2e-5 if precision != "bf16" else 1e-2), (k, e_g)
         assert e_j <= 1e-9, (k, e_j)
         assert e_s <= 1e-6 and e_sg <= 1e-6, (k, e_s, e_sg)
         assert abs(float(b.J[k]) - float(J1)) <= 1e-6 * abs(float(J1))
@@ -155,8 +167,8 @@ def test_autotune_picks_a_measured_candidate(tmp_path, monkeypatch):
     from pbdose import robust
     monkeypatch.setenv("PBDOSE_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
     fields, scen = small_fields(), small_scenarios()[:2]
-    cands = [{"fwd_split": 1, "adj_split": 1, "group": 1, "order": 0},
-             {"fwd_split": 2, "adj_split": 4, "group": 0, "order": 1}]
+    cands = [{"fwd_chunk": 0, "adj_split": 1, "group": 1, "order": 0},
+             {"fwd_chunk": 1, "adj_split": 4, "group": 0, "order": 1}]
     b = robust.RobustBatch(fields, scen, precision="fp32", tuning=robust.DEFAULT_TUNING)
     best = b.autotune(candidates=cands, reps=2)
     assert best in cands
