@@ -219,11 +219,12 @@ def profile_kernels(forms, reps=5):
         print(f"forward {f}: {reps} launches", flush=True)
     Dp, wv = planning.central_cube(prob.grid, device=dev)
     D = dp.forward("fp32")
-    for _ in range(reps):
-        _, R = planning.objective(dp, D, Dp, wv)
-        planning.adjoint(dp, R)
-    torch.cuda.synchronize()
-    print(f"objective + adjoint: {reps} calls", flush=True)
+    for aform in sorted({adjoint_form(f) for f in forms} | {"fp32"}):
+        for _ in range(reps):
+            _, R = planning.objective(dp, D, Dp, wv)
+            planning.adjoint(dp, R, precision=aform)
+        torch.cuda.synchronize()
+        print(f"objective + {aform} adjoint: {reps} calls", flush=True)
     a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
     b = torch.empty_like(a)
     for _ in range(3):

@@ -5,15 +5,22 @@
 //   Q   = R_z Gx_lz          (64 x 48 per tile: rows y, spots i; K = 128 voxels x)
 //   P  += Gy_lz^T Q          (48 x 48: spots j, i; K = 64 voxels y)
 //   dJ/dW_l = sum_z c_lz P_lz,  c_lz = IDD_l(z) / (2 pi s^2)
-// One workgroup (4 waves) per depth slice and share of its layers, as in the
-// FP32 adjoint (pbh_adjoint.hip). Per dose tile (64 x 128) the wave's 16 rows of
-// R are loaded once into registers and split into BF16 (hi, lo). Per layer:
+// Workgroup (z, c) of the `split` workgroups of depth slice z takes the active
+// layers (IDD != 0) of the slice whose active index is c modulo split, and
+// zeroes the partials of the inactive layers l with l % split == c: the
+// workgroups of a slice share its active layers evenly (at most
+// ceil(active / split) each), and those left without one skip the slice's R
+// tiles entirely. The slice is processed in groups of two 64 x 128 tiles
+// (y): the wave's 16 rows of R of both are loaded once into registers and split
+// into BF16 (hi, lo). Per layer (x table once per group, y table per tile):
 //   stage 1  Q = R Gx      v_mfma_f32_16x16x32_bf16, A = R (registers), B = Gx
 //                          gathered from the packed (hi, lo) x table
 //   Q^T is stored split in LDS, four consecutive rows per 8-byte write
 //   stage 2  P = Gy^T Q    v_mfma_f32_16x16x32_bf16, A = Gy^T from the y table,
 //                          B = Q^T rows (16-byte reads)
-// and c P joins the per-slice FP32 partial that adj_reduce_kernel sums over z
+// with P summed over the group's tiles in registers, and c P joins the
+// per-slice FP32 partial (written by the first group, added by later ones) that
+// adj_reduce_kernel sums over z
 // in FP64 (deterministic). Split products: hi*hi + hi*lo + lo*hi of the
 // (hi, lo) parts of both operands; plain: hi*hi. Layers with IDD = 0 at the
 // slice contribute nothing and are skipped. Gradients with respect to the
@@ -25,6 +32,8 @@ namespace pbh {
 
 constexpr int AM_TY = 64, AM_TX = 128, AM_SC = PBH_SPOT_CHUNK;   // 64 x 128 tile, 48-spot chunk
 constexpr int AM_QS = AM_TY + 8;                                  // Q^T row stride (halves)
+constexpr int AM_MAXK = 128;                                      // active layers per workgroup, at most
+constexpr int AM_NT = 2;                                          // 64-row tiles per register group
 
 // Table geometry of an axis for tile width T and chunk width K (as axis_table in
 // the forward): entry t = Q i - P a + P (T - 1) for voxel a, spot i.
@@ -42,137 +51,202 @@ __device__ __forceinline__ void adj_mixed_body(const AdjArgs& A, int blk)
     constexpr int TXN = AmTab<QC, AM_TX, AM_SC>::N, TYN = AmTab<QC, AM_TY, AM_SC>::N;
     const FwdArgs& a = A.f;
 
-    __shared__ unsigned Tx[TXN], Ty[TYN];
-    __shared__ __attribute__((aligned(16))) T Qh[AM_SC * AM_QS];
-    __shared__ __attribute__((aligned(16))) T Ql[SPLIT ? AM_SC * AM_QS : 8];
+    __shared__ unsigned Tx[TXN], Ty[AM_NT][TYN];                   // y tables per tile of a group
+    __shared__ __attribute__((aligned(16))) T Qh[AM_NT][AM_SC * AM_QS];
+    __shared__ __attribute__((aligned(16))) T Ql[AM_NT][SPLIT ? AM_SC * AM_QS : 8];
+
+    __shared__ int lay[AM_MAXK];        // this workgroup's active layers, in layer order,
+    __shared__ double lk2[AM_MAXK];     //   their log2(e) / (2 s^2)
+    __shared__ double lcn[AM_MAXK];     //   and c = IDD / (2 pi s^2), in layer order
+    __shared__ int wcnt[FT_THREADS / 64];
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, c16 = lane & 15;
     const int s = blk % A.split, z = blk / A.split;
+
     const int tiles_x = cdiv(a.nx, AM_TX), tiles_y = cdiv(a.ny, AM_TY);
     const int ncx = chunk_count(a.nsx), ncy = chunk_count(a.nsy);
     const float* Rz = A.R + (size_t)z * a.ny * a.nx;
     const AxisTable& gx = A.tx;
     const AxisTable& gy = A.ty;
     // stage 1 B operand (Gx): x = 32 ks + 8 g + e, i = 16 nb + c16:
-    //   t = Q i - P x + P (AM_TX - 1) = t0 + 16 Q nb - 32 P ks - P e
+    //   t = Q i - P x + P (AM_TX - 1) = t0 + 16 Q nb - 32 P ks - P e.
+    // The streams start at their smallest entry (ks = 3 or 1, e = 7) so that
+    // every offset is non-negative and becomes an LDS immediate.
+    constexpr int D1 = 32 * 2 * 3 + 2 * 7, D2 = 32 * 2 * 1 + 2 * 7;   // P = 2 (QC > 0)
     TabStream<QC> s1, s2;
-    s1.init(gx, gx.Q * c16 - 8 * gx.P * g + gx.P * (AM_TX - 1));
+    s1.init(gx, gx.Q * c16 - 8 * gx.P * g + gx.P * (AM_TX - 1) - (QC ? D1 : 0));
     // stage 2 A operand (Gy^T): y = 32 ks + 8 g + e, j = 16 mb + c16
-    s2.init(gy, gy.Q * c16 - 8 * gy.P * g + gy.P * (AM_TY - 1));
+    s2.init(gy, gy.Q * c16 - 8 * gy.P * g + gy.P * (AM_TY - 1) - (QC ? D2 : 0));
     const int ntX = gx.P * (AM_TX - 1) + gx.Q * (AM_SC - 1) + 1;
     const int ntY = gy.P * (AM_TY - 1) + gy.Q * (AM_SC - 1) + 1;
 
     auto slot1 = [&](int nb, int ks, int e) {
-        if (QC > 0) return s1.at(16 * QC * nb - 64 * ks - 2 * e);     // P = 2
+        if (QC > 0) return s1.at(16 * QC * nb + D1 - 64 * ks - 2 * e);     // P = 2
         return table_slot(gx, s1.bE + 16 * gx.Q * nb - 32 * gx.P * ks - gx.P * e);
     };
     auto slot2 = [&](int mb, int ks, int e) {
-        if (QC > 0) return s2.at(16 * QC * mb - 64 * ks - 2 * e);
+        if (QC > 0) return s2.at(16 * QC * mb + D2 - 64 * ks - 2 * e);
         return table_slot(gy, s2.bE + 16 * gy.Q * mb - 32 * gy.P * ks - gy.P * e);
     };
+
+    // zero the partials of the inactive layers l % split == s (uniform test)
+    for (int l = s; l < a.L; l += A.split) {
+        if (a.idd[(size_t)l * a.nz + z] != 0.0) continue;
+        float* pz = A.part + ((size_t)z * a.L + l) * a.nsy * a.nsx;
+        for (int e = threadIdx.x; e < a.nsy * a.nsx; e += FT_THREADS) pz[e] = 0.0f;
+    }
+    // active layers of the slice (stable order, ballots over passes of 256); keep
+    // those with active index = s (mod split), at most ceil(L / split) <= AM_MAXK
+    int nmine = 0, base = 0;
+    for (int lb = 0; lb < a.L; lb += FT_THREADS) {
+        const int l = lb + threadIdx.x;
+        const bool act = l < a.L && a.idd[(size_t)l * a.nz + z] != 0.0;
+        const unsigned long long m = __ballot(act);
+        if (lane == 0) wcnt[wave] = __popcll(m);
+        __syncthreads();
+        int off = base, tot = 0;
+        for (int w = 0; w < FT_THREADS / 64; ++w) {
+            off += w < wave ? wcnt[w] : 0;
+            tot += wcnt[w];
+        }
+        const int idx = off + __popcll(m & ((1ull << lane) - 1ull));
+        if (act && idx % A.split == s) {
+            const double sg = a.sigma[(size_t)l * a.nz + z];
+            const double sd = sg > kSigmaFloor ? sg : kSigmaFloor;
+            lk2[idx / A.split] = kLog2e / (2.0 * sd * sd);
+            lcn[idx / A.split] = a.idd[(size_t)l * a.nz + z] / (2.0 * kPi * sd * sd);
+            lay[idx / A.split] = l;
+        }
+        base += tot;
+        __syncthreads();
+    }
+    nmine = base > s ? (base - s + A.split - 1) / A.split : 0;
+    if (nmine <= 0) return;             // uniform: nothing for this workgroup
 
     for (int cy = 0; cy < ncy; ++cy) {
         const int j0 = chunk_begin(a.nsy, ncy, cy), nj = chunk_begin(a.nsy, ncy, cy + 1) - j0;
         for (int cx = 0; cx < ncx; ++cx) {
             const int i0 = chunk_begin(a.nsx, ncx, cx), ni = chunk_begin(a.nsx, ncx, cx + 1) - i0;
-            for (int tyi = 0; tyi < tiles_y; ++tyi) {
-                for (int txi = 0; txi < tiles_x; ++txi) {
-                    const int y0 = tyi * AM_TY, x0 = txi * AM_TX;
-                    const bool first_tile = tyi == 0 && txi == 0;
-                    // this wave's 16 rows of R (x = 32 ks + 8 g + e), split into (hi, lo)
-                    V8 rh[4], rl[4];
-                    {
-                        const int y = y0 + 16 * wave + c16;
+            for (int txi = 0; txi < tiles_x; ++txi) {
+                const int x0 = txi * AM_TX;
+                const bool interior_x = x0 + AM_TX <= a.nx && (a.nx & 3) == 0 && ((uintptr_t)A.R & 15) == 0;
+                const double Cx = (a.ox + (double)x0 * a.dx) - (a.sx0 + (double)i0 * a.dsx);
+                // groups of AM_NT consecutive 64-row tiles: R of the group in registers, P of a
+                // layer accumulated over the group and joined to the partial once
+                for (int tg = 0; tg < tiles_y; tg += AM_NT) {
+                    const int nt = min(AM_NT, tiles_y - tg);
+                    const bool first_group = txi == 0 && tg == 0;
+                    V8 rh[AM_NT][4], rl[AM_NT][4];
+#pragma unroll
+                    for (int u = 0; u < AM_NT; ++u) {
+                        const int y = (tg + u) * AM_TY + 16 * wave + c16;
+                        float rv[4][8];
+                        if (u < nt && interior_x && y < a.ny) {
+                            // whole 128-column row segment inside the grid, 16-byte aligned
+                            const float4* src = (const float4*)(Rz + (size_t)y * a.nx + x0 + 8 * g);
+#pragma unroll
+                            for (int ks = 0; ks < 4; ++ks) {
+                                const float4 v0 = src[8 * ks], v1 = src[8 * ks + 1];
+                                rv[ks][0] = v0.x; rv[ks][1] = v0.y; rv[ks][2] = v0.z; rv[ks][3] = v0.w;
+                                rv[ks][4] = v1.x; rv[ks][5] = v1.y; rv[ks][6] = v1.z; rv[ks][7] = v1.w;
+                            }
+                        } else {
+#pragma unroll
+                            for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+                                for (int e = 0; e < 8; ++e) {
+                                    const int x = x0 + 32 * ks + 8 * g + e;
+                                    rv[ks][e] = (u < nt && y < a.ny && x < a.nx) ? Rz[(size_t)y * a.nx + x] : 0.0f;
+                                }
+                        }
 #pragma unroll
                         for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
                             for (int e = 0; e < 8; ++e) {
-                                const int x = x0 + 32 * ks + 8 * g + e;
-                                const float v = (y < a.ny && x < a.nx) ? Rz[(size_t)y * a.nx + x] : 0.0f;
                                 T h, l;
-                                split<T>(v, h, l);
-                                rh[ks][e] = h;
-                                rl[ks][e] = l;
+                                split<T>(rv[ks][e], h, l);
+                                rh[u][ks][e] = h;
+                                rl[u][ks][e] = l;
                             }
                     }
-                    const double Cx = (a.ox + (double)x0 * a.dx) - (a.sx0 + (double)i0 * a.dsx);
-                    const double Cy = (a.oy + (double)y0 * a.dy) - (a.sy0 + (double)j0 * a.dsy);
-                    for (int l = s; l < a.L; l += A.split) {
-                        const double iddv = a.idd[(size_t)l * a.nz + z];
-                        float* pz = A.part + ((size_t)z * a.L + l) * a.nsy * a.nsx;
-                        if (iddv == 0.0) {      // uniform in the workgroup: no dose, zero partial
-                            if (first_tile)
-                                for (int e = threadIdx.x; e < ni * nj; e += FT_THREADS)
-                                    pz[(size_t)(j0 + e / ni) * a.nsx + i0 + e % ni] = 0.0f;
-                            continue;
-                        }
-                        const double sg = a.sigma[(size_t)l * a.nz + z];
-                        const double sd = sg > kSigmaFloor ? sg : kSigmaFloor;
-                        const double k2d = kLog2e / (2.0 * sd * sd);
-                        const double cn = iddv / (2.0 * kPi * sd * sd);
+                    for (int q = 0; q < nmine; ++q) {
+                        const double k2d = lk2[q];
                         __syncthreads();        // previous layer's readers of the tables and Q are done
                         for (int t = threadIdx.x; t < ntX; t += FT_THREADS) {
                             const double d = table_distance(gx, Cx, AM_TX, t);
                             Tx[table_slot(gx, t)] = pack_split<T>(__builtin_amdgcn_exp2f((float)(-(d * d) * k2d)));
                         }
-                        for (int t = threadIdx.x; t < ntY; t += FT_THREADS) {
-                            const double d = table_distance(gy, Cy, AM_TY, t);
-                            Ty[table_slot(gy, t)] = pack_split<T>(__builtin_amdgcn_exp2f((float)(-(d * d) * k2d)));
+                        for (int u = 0; u < nt; ++u) {
+                            const double Cy = (a.oy + (double)((tg + u) * AM_TY) * a.dy) - (a.sy0 + (double)j0 * a.dsy);
+                            for (int t = threadIdx.x; t < ntY; t += FT_THREADS) {
+                                const double d = table_distance(gy, Cy, AM_TY, t);
+                                Ty[u][table_slot(gy, t)] =
+                                    pack_split<T>(__builtin_amdgcn_exp2f((float)(-(d * d) * k2d)));
+                            }
                         }
                         __syncthreads();
-                        // stage 1: Q[y][i] for this wave's 16 rows and the three 16-spot blocks
+                        // stage 1 of every tile of the group: Q[y][i], this wave's 16 rows, 3 spot blocks
+#pragma unroll
+                        for (int u = 0; u < AM_NT; ++u) {
+                            if (u >= nt) break;
 #pragma unroll 1
-                        for (int nb = 0; nb < 3; ++nb) {
-                            v4f q = {};
+                            for (int nb = 0; nb < 3; ++nb) {
+                                v4f acc1 = {};
 #pragma unroll
-                            for (int ks = 0; ks < 4; ++ks) {
-                                unsigned t8[8];
+                                for (int ks = 0; ks < 4; ++ks) {
+                                    unsigned t8[8];
 #pragma unroll
-                                for (int e = 0; e < 8; ++e) t8[e] = Tx[slot1(nb, ks, e)];
-                                V8 bh, bl;
-                                unpack8(t8, bh, bl);
-                                q = mfma16(rh[ks], bh, q);
-                                if (SPLIT) {
-                                    q = mfma16(rh[ks], bl, q);
-                                    q = mfma16(rl[ks], bh, q);
+                                    for (int e = 0; e < 8; ++e) t8[e] = Tx[slot1(nb, ks, e)];
+                                    V8 bh, bl;
+                                    unpack8(t8, bh, bl);
+                                    acc1 = mfma16(rh[u][ks], bh, acc1);
+                                    if (SPLIT) {
+                                        acc1 = mfma16(rh[u][ks], bl, acc1);
+                                        acc1 = mfma16(rl[u][ks], bh, acc1);
+                                    }
                                 }
-                            }
-                            // C layout: rows y = 16 wave + 4 g + r, column i = 16 nb + c16
-                            u2v hv, lv;
+                                // C layout: rows y = 16 wave + 4 g + r, column i = 16 nb + c16
+                                u2v hv, lv;
 #pragma unroll
-                            for (int h2 = 0; h2 < 2; ++h2) {
-                                T a0, b0, a1, b1;
-                                split<T>(q[2 * h2], a0, b0);
-                                split<T>(q[2 * h2 + 1], a1, b1);
-                                hv[h2] = (unsigned)bits16(a0) | ((unsigned)bits16(a1) << 16);
-                                lv[h2] = (unsigned)bits16(b0) | ((unsigned)bits16(b1) << 16);
+                                for (int h2 = 0; h2 < 2; ++h2) {
+                                    T a0, b0, a1, b1;
+                                    split<T>(acc1[2 * h2], a0, b0);
+                                    split<T>(acc1[2 * h2 + 1], a1, b1);
+                                    hv[h2] = (unsigned)bits16(a0) | ((unsigned)bits16(a1) << 16);
+                                    lv[h2] = (unsigned)bits16(b0) | ((unsigned)bits16(b1) << 16);
+                                }
+                                const int off = (16 * nb + c16) * AM_QS + 16 * wave + 4 * g;
+                                *(u2v*)&Qh[u][off] = hv;
+                                if (SPLIT) *(u2v*)&Ql[u][off] = lv;
                             }
-                            const int off = (16 * nb + c16) * AM_QS + 16 * wave + 4 * g;
-                            *(u2v*)&Qh[off] = hv;
-                            if (SPLIT) *(u2v*)&Ql[off] = lv;
                         }
                         __syncthreads();
-                        // stage 2: P[j][i] blocks (mb, nb) = wave + 4 qq, qq = 0..2, of the 3 x 3
+                        // stage 2: P[j][i] blocks (mb, nb) = wave + 4 qq of the 3 x 3, summed over
+                        // the group's tiles, joined to the slice's partial once
+                        float* pz = A.part + ((size_t)z * a.L + lay[q]) * a.nsy * a.nsx;
+                        const double cn = lcn[q];
 #pragma unroll 1
                         for (int qq = 0; qq < 3; ++qq) {
                             const int b2 = wave + 4 * qq;
                             if (b2 >= 9) break;
                             const int mb = b2 / 3, nb = b2 % 3;
                             v4f p = {};
+                            for (int u = 0; u < nt; ++u) {
 #pragma unroll
-                            for (int ks = 0; ks < 2; ++ks) {
-                                unsigned t8[8];
+                                for (int ks = 0; ks < 2; ++ks) {
+                                    unsigned t8[8];
 #pragma unroll
-                                for (int e = 0; e < 8; ++e) t8[e] = Ty[slot2(mb, ks, e)];
-                                V8 ah, al;
-                                unpack8(t8, ah, al);
-                                const int off = (16 * nb + c16) * AM_QS + 32 * ks + 8 * g;
-                                const V8 bh = *(const V8*)&Qh[off];
-                                p = mfma16(ah, bh, p);
-                                if (SPLIT) {
-                                    p = mfma16(ah, *(const V8*)&Ql[off], p);
-                                    p = mfma16(al, bh, p);
+                                    for (int e = 0; e < 8; ++e) t8[e] = Ty[u][slot2(mb, ks, e)];
+                                    V8 ah, al;
+                                    unpack8(t8, ah, al);
+                                    const int off = (16 * nb + c16) * AM_QS + 32 * ks + 8 * g;
+                                    const V8 bh = *(const V8*)&Qh[u][off];
+                                    p = mfma16(ah, bh, p);
+                                    if (SPLIT) {
+                                        p = mfma16(ah, *(const V8*)&Ql[u][off], p);
+                                        p = mfma16(al, bh, p);
+                                    }
                                 }
                             }
                             // C layout: j = 16 mb + 4 g + r, i = 16 nb + c16
@@ -182,7 +256,7 @@ __device__ __forceinline__ void adj_mixed_body(const AdjArgs& A, int blk)
                                 if (j < nj && i < ni) {
                                     const size_t off = (size_t)(j0 + j) * a.nsx + i0 + i;
                                     const float cp = (float)(cn * (double)p[r]);
-                                    pz[off] = first_tile ? cp : pz[off] + cp;
+                                    pz[off] = first_group ? cp : pz[off] + cp;
                                 }
                             }
                         }
@@ -222,8 +296,15 @@ bool pbh_adj_mixed_q5(const AdjArgs& A)
 
 // Launches the mixed adjoint (dJ/dw only) for one problem; A.tab must be set
 // (the mixed adjoint gathers every factor from tables).
+// Layers per workgroup of a split: the mixed adjoint keeps at most AM_MAXK.
+static bool split_fits(const AdjArgs& A)
+{
+    return (A.f.L + A.split - 1) / A.split <= AM_MAXK;
+}
+
 int pbh_launch_adj_mixed(const AdjArgs& A, int precision, int nz, hipStream_t s)
 {
+    if (!split_fits(A)) return PBH_ERR_UNSUPPORTED;
     const dim3 grid((unsigned)(nz * A.split)), block(FT_THREADS);
     const bool x3 = precision != PBH_PREC_BF16;
     const bool q5 = pbh_adj_mixed_q5(A);
@@ -235,6 +316,11 @@ int pbh_launch_adj_mixed(const AdjArgs& A, int precision, int nz, hipStream_t s)
         else hipLaunchKernelGGL((adj_mixed_kernel<MODE_PLAIN, 0>), grid, block, 0, s, A);
     }
     return launch_rc("adj_mixed_kernel");
+}
+
+bool pbh_adj_mixed_fits(const AdjArgs& A)
+{
+    return split_fits(A);
 }
 
 int pbh_launch_adj_mixed_batch(const AdjArgs* items, const int* wg0, int n, int total, int precision, bool q5,

@@ -56,6 +56,7 @@ int pbh_launch_fwd_mixed_batch(const pbh::BatchFwdItem* items, const int* wg0, i
 int pbh_launch_wsplit_batch(const pbh::BatchFwdItem* items, int n, int max_chunks, int precision, hipStream_t s);
 int pbh_mixed_slot_mode(const pbh::FwdPlan& pl);
 bool pbh_adj_mixed_q5(const pbh::AdjArgs& A);
+bool pbh_adj_mixed_fits(const pbh::AdjArgs& A);
 int pbh_launch_adj_mixed(const pbh::AdjArgs& A, int precision, int nz, hipStream_t s);
 int pbh_launch_adj_mixed_batch(const pbh::AdjArgs* items, const int* wg0, int n, int total, int precision, bool q5,
                                hipStream_t s);

@@ -309,7 +309,8 @@ extern "C" int pbh_batch_create(const PbhBatchItem* items, int n, int precision,
         b->sig = b->sig || it.grad_sigma != nullptr;
         akeys.push_back(0);
         // adjoint mode: bit 0 table factors, bit 1 BF16 kernel, bit 2 P = 2, Q = 5 slots
-        const bool mix = b->adj_prec != PBH_PREC_FP32 && A.tab && !it.grad_idd && !it.grad_sigma;
+        const bool mix = b->adj_prec != PBH_PREC_FP32 && A.tab && !it.grad_idd && !it.grad_sigma &&
+                         pbh_adj_mixed_fits(A);
         atabs.push_back((char)(mix ? (3 | (pbh_adj_mixed_q5(A) ? 4 : 0)) : (A.tab ? 1 : 0)));
         aw.push_back(it.grid.nz * A.split);
         const size_t ne = (size_t)it.lattice.n_layers * it.lattice.n_sy * it.lattice.n_sx;

@@ -328,13 +328,15 @@ static long long tile_count(const FwdArgs& a) { return (long long)a.tiles_x * a.
 
 constexpr int FT_MIN_CHUNK = 4;   // smallest automatic layer chunk
 
-// Layer chunk for about four workgroups per resident slot (two per CU): the
-// chunks of shallow slices (many active layers) then spread over more
-// workgroups than those of deep slices, and the dispatcher balances them.
+// Layer chunk for about three workgroups per two resident slots (two per CU):
+// the chunks of shallow slices (many active layers) then spread over more
+// workgroups than those of deep slices, and the dispatcher balances them,
+// while few partial tiles have to be combined (measured on the 128^3 field:
+// two chunks per tile beat one and four).
 int pbh_auto_chunk(const FwdArgs& a)
 {
     const long long slots = 2LL * device_cus();
-    const long long per_tile = (4 * slots + tile_count(a) - 1) / tile_count(a);
+    const long long per_tile = (3 * slots + 2 * tile_count(a) - 1) / (2 * tile_count(a));
     if (per_tile <= 1) return a.L;
     int chunk = (int)((a.L + per_tile - 1) / per_tile);
     return chunk < FT_MIN_CHUNK ? (a.L < FT_MIN_CHUNK ? a.L : FT_MIN_CHUNK) : chunk;
