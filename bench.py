@@ -11,9 +11,9 @@ Prints ONE JSON line. ``value`` = fields/s of the selected arithmetic form
 forward + adjoint time (adjoint of the weighted-quadratic objective: dose
 residual kernel + adjoint + depth reduction; FP32 adjoint for the FP32 form,
 BF16 or BF16x3 adjoint for the 16-bit forms) and the error of each form
-against the FP64 oracle are reported beside it. ``roofline`` is for the dominant
-kernel (the fused forward of the selected form), timed per launch with HIP
-events on the launch stream. ``cpu_baseline`` times the host restatement of the
+against the FP64 oracle are reported beside it. ``roofline`` is for the forward
+call of the selected form (weight split, fused forward, layer-chunk combine),
+timed per call with HIP events on the launch stream. ``cpu_baseline`` times the host restatement of the
 reference's PyTorch batched-matmul engine (oracle/torch_cpu.py, FP64, all host
 threads) on full forwards of the same field; its FP64 output is the reference
 for ``accuracy_vs_fp64`` (errors and the 2 %/2 mm gamma pass rate of the
@@ -38,10 +38,13 @@ for _p in (ROOT, os.path.join(ROOT, "gpu-proton-dose-engine_amd")):
 import numpy as np  # noqa: E402
 import torch  # noqa: E402
 
-# HBM bytes per launch of the forward kernels on this workload, from rocprofv3 FETCH_SIZE
-# and WRITE_SIZE (KiB per dispatch) scaled by the same counters on a 256 MiB copy
-# (fetch x 2.000, write x 1.000): profiles/counters_forward_adjoint_128cube_67240spots.md
-TRAFFIC_KIB = {"fp32": (1492.3, 8192.0), "bf16x3": (1509.9, 8192.0), "bf16": (1494.0, 8192.0)}
+# HBM bytes of one forward call on this workload (the kernels bench times between its
+# events: weight split, forward, layer-chunk combine), from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE (KiB per dispatch, summed over the call's kernels) scaled by the same
+# counters on a 256 MiB copy (fetch x 2.000, write x 1.000):
+# profiles/counters_forward_adjoint_128cube_67240spots.md
+TRAFFIC_KIB = {"fp32": (6152.1, 17152.0), "bf16x3": (7160.9, 17632.0), "bf16": (6188.9, 17632.0)}
+CALL_KERNELS = {"fp32": "fwd_fp32_kernel + combine_kernel"}
 FETCH_SCALE, WRITE_SCALE = 2.000, 1.000
 
 # published dense MFMA peaks of MI355X (TFLOP/s) for the arithmetic each form runs on
@@ -305,7 +308,8 @@ def main():
         "fwd_adj_speedup_vs_fp32": (ms32 + adj32_ms) / (ms_step + adj_ms),
         "torch_separable_ms_incl_host_copies": pcie_ms,
         "roofline": {"bound": "mfma",
-                     "kernel": "fwd_fp32_kernel" if args.precision == "fp32" else "fwd_mixed_kernel",
+                     "kernel": "forward call: " + CALL_KERNELS.get(
+                         args.precision, "wsplit_kernel + fwd_mixed_kernel + combine_kernel"),
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic_bytes(args.precision),
                      "algorithmic_flop_per_launch": flops, "launch_ms": launch_ms},

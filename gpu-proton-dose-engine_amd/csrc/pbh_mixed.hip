@@ -50,9 +50,15 @@ __global__ void peak_depth_f32_kernel(const double* idd, int L, int nz, double o
 }
 
 // Split 16-bit weights of one (layer, spot chunk): hi and lo at [i][j] (i < 48,
-// j < 64), zero outside the chunk. blk = (layer * chunks_y + chunk_y) * chunks_x + chunk_x.
+// j < 64), zero outside the chunk. blk = (layer * chunks_y + chunk_y) *
+// chunks_x + chunk_x; part (0 .. WS_PARTS-1) selects a slice of the chunk's
+// (i, j pair) entries. Each thread writes one pair of consecutive j of hi and of
+// lo as 32-bit words, consecutive threads consecutive words.
+constexpr int WS_PAIRS = SCH * KYP / 2;                  // 1536 (i, j pair) entries per chunk
+constexpr int WS_PARTS = WS_PAIRS / 256;                 // workgroups of 256 threads per chunk
+
 template <typename T>
-__device__ __forceinline__ void wsplit_body(const FwdArgs& a, unsigned short* dst, int blk)
+__device__ __forceinline__ void wsplit_body(const FwdArgs& a, unsigned short* dst, int blk, int part)
 {
     const int ncx = chunk_count(a.nsx), ncy = chunk_count(a.nsy);
     const int cx = blk % ncx, cy = (blk / ncx) % ncy, l = blk / (ncx * ncy);
@@ -60,29 +66,30 @@ __device__ __forceinline__ void wsplit_body(const FwdArgs& a, unsigned short* ds
     const int i0 = chunk_begin(a.nsx, ncx, cx), ni = chunk_begin(a.nsx, ncx, cx + 1) - i0;
     const int j0 = chunk_begin(a.nsy, ncy, cy), nj = chunk_begin(a.nsy, ncy, cy + 1) - j0;
     const float* wl = a.w + (size_t)l * a.nsy * a.nsx;
-    unsigned short* hi = dst + (size_t)blk * FT_WSPLIT;
-    unsigned short* lo = hi + SCH * KYP;
-    for (int e = threadIdx.x; e < SCH * KYP; e += blockDim.x) {
-        const int i = e % SCH, j = e / SCH;
-        const float v = (i < ni && j < nj) ? wl[(size_t)(j0 + j) * a.nsx + i0 + i] : 0.0f;
-        T h, r;
-        split<T>(v, h, r);
-        hi[i * KYP + j] = bits16(h);
-        lo[i * KYP + j] = bits16(r);
-    }
+    unsigned* hi = (unsigned*)(dst + (size_t)blk * FT_WSPLIT);
+    unsigned* lo = hi + WS_PAIRS;
+    const int p = part * 256 + threadIdx.x;               // pair index: i = p / 32, j = 2 (p % 32) + {0, 1}
+    const int i = p / (KYP / 2), j = 2 * (p % (KYP / 2));
+    const float v0 = (i < ni && j < nj) ? wl[(size_t)(j0 + j) * a.nsx + i0 + i] : 0.0f;
+    const float v1 = (i < ni && j + 1 < nj) ? wl[(size_t)(j0 + j + 1) * a.nsx + i0 + i] : 0.0f;
+    T h0, r0, h1, r1;
+    split<T>(v0, h0, r0);
+    split<T>(v1, h1, r1);
+    hi[p] = (unsigned)bits16(h0) | ((unsigned)bits16(h1) << 16);
+    lo[p] = (unsigned)bits16(r0) | ((unsigned)bits16(r1) << 16);
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void wsplit_kernel(FwdArgs a, unsigned short* dst)
 {
-    wsplit_body<T>(a, dst, blockIdx.x);
+    wsplit_body<T>(a, dst, blockIdx.x, blockIdx.y);
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void wsplit_batch_kernel(const BatchFwdItem* __restrict__ items)
 {
-    const FwdArgs a = items[blockIdx.y].a;
-    wsplit_body<T>(a, items[blockIdx.y].pl.wsplit, blockIdx.x);
+    const FwdArgs a = items[blockIdx.z].a;
+    wsplit_body<T>(a, items[blockIdx.z].pl.wsplit, blockIdx.x, blockIdx.y);
 }
 
 // 4 waves (two workgroups per CU), one-barrier software pipeline over the
@@ -513,7 +520,7 @@ int pbh_forward_mixed(const FwdArgs& a, double oz, double dz, int precision, int
                            a.idd, a.L, a.nz, oz, dz, r_peak);
         if ((rc = launch_rc("peak_depth_f32_kernel")) != PBH_OK) return rc;
     }
-    const dim3 wgrid((unsigned)(a.L * chunk_count(a.nsy) * chunk_count(a.nsx)));
+    const dim3 wgrid((unsigned)(a.L * chunk_count(a.nsy) * chunk_count(a.nsx)), WS_PARTS);
     if (is_f16(precision))
         hipLaunchKernelGGL(wsplit_kernel<_Float16>, wgrid, dim3(256), 0, s, a, pl.wsplit);
     else
@@ -533,7 +540,7 @@ int pbh_launch_peak_batch(const BatchPeakItem* items, int n, hipStream_t s)
 // Split weights of every item of a batch (grid: max chunks x items).
 int pbh_launch_wsplit_batch(const BatchFwdItem* items, int n, int max_chunks, int precision, hipStream_t s)
 {
-    const dim3 grid((unsigned)max_chunks, (unsigned)n);
+    const dim3 grid((unsigned)max_chunks, WS_PARTS, (unsigned)n);
     if (is_f16(precision))
         hipLaunchKernelGGL(wsplit_batch_kernel<_Float16>, grid, dim3(256), 0, s, items);
     else
