@@ -83,6 +83,8 @@ SIGNATURES = {
     "pbh_batch_forward_adjoint": (C.c_int, [_P, _P]),
     "pbh_batch_launch_count": (C.c_int, [_P]),
     "pbh_batch_destroy": (None, [_P]),
+    "pbh_weight_step": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_size_t, C.c_int, C.c_float, C.c_float,
+                                  C.c_float, _P]),
     "pbh_gamma_workspace_bytes": (C.c_size_t, [_G]),
     "pbh_gamma_count_offsets": (C.c_int, [_G, C.c_double]),
     "pbh_gamma_index": (C.c_int, [_G, _P, C.c_int, _P, C.c_int, C.c_double, C.c_double, C.c_double,

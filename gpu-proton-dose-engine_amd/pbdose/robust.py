@@ -14,6 +14,10 @@ is an ordinary PBProblem:
 The prescription and voxel weights stay attached to the patient grid.
 The standard set has 21 scenarios: {nominal, +-dx, +-dy, +-dz} x {1, 1 -+ r}.
 
+``optimize`` runs inverse planning on a batch: projected-gradient descent of each
+field's summed or worst-case objective over its scenarios, one graph replay per
+iteration, with the weight update in the HIP kernel pbh_weight_step.
+
 Launch parameters (forward layer chunks, adjoint layer split, one forward
 launch per spot-chunk width or one at the widest, item order) are chosen by ``autotune``:
 every candidate is built and timed on the device, and the winner is cached per
@@ -361,3 +365,91 @@ class RobustBatch:
             b = lat.n_sy * lat.n_sx * g.nx + g.ny * lat.n_sy * g.nx
             tot += 2.0 * lat.n_layers * g.nz * min(a, b)
         return tot
+
+
+# ---------------------------------------------------------------------------
+# inverse planning on a robust batch
+# ---------------------------------------------------------------------------
+ROBUST_MODES = {"sum": 0, "worst": 1}
+
+
+def weight_step(w_old: torch.Tensor, w_new: torch.Tensor, grads: Sequence[torch.Tensor],
+                J: Optional[torch.Tensor], step: float, mode: str = "sum", w_min: float = 0.0,
+                w_max: float = float("inf")) -> None:
+    """w_new := clamp(w_old - step * g, w_min, w_max) on the device (pbh_weight_step);
+    g = sum of ``grads`` (mode 'sum') or the gradient of the scenario with the
+    largest J (mode 'worst')."""
+    dev = w_old.device
+    ptrs = torch.tensor([g.data_ptr() for g in grads], dtype=torch.int64, device=dev)
+    _lib.check(_lib.lib().pbh_weight_step(_lib.ptr(w_old), _lib.ptr(w_new), _lib.ptr(ptrs), _lib.ptr(J),
+                                          len(grads), w_old.numel(), ROBUST_MODES[mode], float(step),
+                                          float(w_min), float(w_max), _lib.stream_of(dev)),
+               "pbh_weight_step")
+
+
+def optimize(batch: "RobustBatch", iters: int = 20, mode: str = "sum", w_min: float = 0.0,
+             w_max: float = float("inf"), step0: Optional[float] = None, grow: float = 1.5,
+             shrink: float = 0.5, use_graph: bool = True) -> dict:
+    """Projected-gradient descent of every field's robust objective over its scenarios:
+    F_f(w) = sum_s J_fs(w) ('sum') or max_s J_fs(w) ('worst', min-max with the worst
+    scenario's gradient as a subgradient), with w_min <= w <= w_max.
+
+    Each iteration takes one trial step per field from the gradients of its last
+    accepted point, evaluates all fields and scenarios with one batch run (graph
+    replay) and keeps a field's step if its F decreased (step size x grow), else
+    restores the field's weights (step size x shrink). F is non-increasing.
+    Returns {'F': per-iteration list of per-field F, 'accepted': ..., 'steps': ...}."""
+    if not batch.objective:
+        raise ValueError("the batch was built without an objective")
+    if mode not in ROBUST_MODES:
+        raise ValueError(f"unknown robust mode {mode!r}")
+    run = batch.run if use_graph else batch.run_eager
+    nS, nF = len(batch.scenarios), len(batch.fields)
+
+    def robust(Jh):
+        per = Jh.reshape(nF, nS)
+        return per.sum(axis=1) if mode == "sum" else per.max(axis=1)
+
+    run()
+    Jh = batch.J.cpu().numpy()
+    F = robust(Jh)
+    w_acc = [w.clone() for w in batch.w]
+    g_acc = [g.clone() for g in batch.grad_w]
+    J_acc = batch.J.clone()
+    if step0 is None:
+        steps = []
+        for f in range(nF):
+            gs = sum(g_acc[f * nS + s].abs().max().item() for s in range(nS))
+            steps.append(0.1 * w_acc[f].abs().max().item() / gs if gs > 0 else 1.0)
+    else:
+        steps = [float(step0)] * nF
+    hist = {"F": [F.tolist()], "accepted": [], "steps": []}
+    for _ in range(iters):
+        for f in range(nF):
+            weight_step(w_acc[f], batch.w[f], g_acc[f * nS:(f + 1) * nS], J_acc[f * nS:(f + 1) * nS],
+                        steps[f], mode, w_min, w_max)
+        run()
+        Jh = batch.J.cpu().numpy()
+        Fn = robust(Jh)
+        acc = []
+        for f in range(nF):
+            if Fn[f] < F[f]:
+                F[f] = Fn[f]
+                w_acc[f].copy_(batch.w[f])
+                for s in range(nS):
+                    g_acc[f * nS + s].copy_(batch.grad_w[f * nS + s])
+                J_acc[f * nS:(f + 1) * nS].copy_(batch.J[f * nS:(f + 1) * nS])
+                steps[f] *= grow
+                acc.append(True)
+            else:
+                batch.w[f].copy_(w_acc[f])
+                steps[f] *= shrink
+                acc.append(False)
+        hist["F"].append(F.tolist())
+        hist["accepted"].append(acc)
+        hist["steps"].append(list(steps))
+    # leave the batch's weights at the best accepted point
+    for f in range(nF):
+        batch.w[f].copy_(w_acc[f])
+    torch.cuda.synchronize(batch.dev)
+    return hist

@@ -266,6 +266,21 @@ int pbh_batch_launch_count(const PbhBatch* batch);
 void pbh_batch_destroy(PbhBatch* batch);
 
 /* ---------------------------------------------------------------------------
+ * Inverse planning: projected-gradient step of one field's spot weights from
+ * the dJ/dw of its scenarios (e.g. the grad_w outputs of a robust batch).
+ * No counterpart in the reference.
+ * ------------------------------------------------------------------------- */
+#define PBH_ROBUST_SUM    0   /* g = sum over scenarios, in scenario order */
+#define PBH_ROBUST_WORST  1   /* g = gradient of the first scenario with the largest J */
+
+/* w_new := min(max(w_old - step * g, w_min), w_max) elementwise over n weights
+ * (FP32, no fused multiply-add). grads: device array of n_scen device pointers;
+ * J: device FP64 array of n_scen objectives (read for PBH_ROBUST_WORST).
+ * w_new may equal w_old. */
+int pbh_weight_step(const float* w_old, float* w_new, const float* const* grads, const double* J,
+                    int n_scen, size_t n, int mode, float step, float w_min, float w_max, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Gamma index with integer-voxel DTA offsets. gamma.py:41-160 (gamma_index);
  * also replaces gamma.py:213-261 (gamma_index_torch, which evaluates in FP32:
  * here the arithmetic is FP64 for both).

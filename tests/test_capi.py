@@ -183,3 +183,14 @@ def test_gamma_argument_checks():
     assert L.pbh_gamma_index(g, one, 2, one, 0, 2.0, 2.0, 10.0, 0.0, 0, nul, one, one, one, 1 << 20, nul) == -5
     assert L.pbh_gamma_index(g, one, 0, one, 0, 0.0, 2.0, 10.0, 0.0, 0, nul, one, one, one, 1 << 20, nul) == -2
     assert L.pbh_gamma_index(g, one, 0, one, 0, 2.0, 2.0, 10.0, 0.0, 0, nul, one, one, one, 16, nul) == -7
+
+
+def test_weight_step_argument_checks():
+    L = _lib.lib()
+    nul = C.c_void_p(0)
+    one = C.c_void_p(256)
+    assert L.pbh_weight_step(nul, one, one, one, 2, 10, 0, 0.1, 0.0, 1.0, nul) == -1
+    assert L.pbh_weight_step(one, one, one, nul, 2, 10, 1, 0.1, 0.0, 1.0, nul) == -1   # worst needs J
+    assert L.pbh_weight_step(one, one, one, one, 0, 10, 0, 0.1, 0.0, 1.0, nul) == -2
+    assert L.pbh_weight_step(one, one, one, one, 2, 10, 7, 0.1, 0.0, 1.0, nul) == -5
+    assert L.pbh_weight_step(one, one, one, one, 2, 10, 0, 0.1, 2.0, 1.0, nul) == -2   # w_min > w_max
