@@ -18,6 +18,17 @@
 // terms below exp(-pi^2 sigma^2 / (2 h^2)) (zero in FP64 for sigma >> h), so
 // the kernel sums that convolution directly, only at the fine-grid points the
 // final interpolation reads.
+//
+// Depth-dependent range straggling (straggling_depth = 1) replaces the single
+// sigma_R by a spread per CSDA source bin: the dose binned at depth z' is
+// spread with s(z') = hypot(sigma_B(z'), sigma_E), where sigma_B(z')^2 =
+// C(E0) - C(E(R0 - z')) and C is the cumulative trapezoid of the Bohr
+// integrand on the reference's 4000-point energy grid (physics.py:168-174).
+// The reference documents this as its open model defect (README.md:180-184):
+// its kernel is 6x too wide at 2 mm depth. Each bin keeps unit weight: the
+// sampled Gaussian is normalised by the continuous factor h / (sqrt(2 pi) s)
+// when s >= 2h (the discrete sum differs by < 1e-30) and by its explicit
+// discrete sum below that; s = 0 leaves the bin in place.
 #pragma clang fp contract(off)
 
 #include "pbh_common.h"
@@ -39,6 +50,7 @@ constexpr double BOHR = KBB * ZA * RHO * ME;
 
 constexpr int N_TABLE = 3797;      // 800 + 899 + 899 + 1199 (shared segment ends removed)
 constexpr int LUT_THREADS = 256;
+constexpr int STRAG_N = 4000;      // Bohr integral grid, linspace(0.5, E0, 4000) (physics.py:168)
 
 // ---- kinematics and stopping power (physics.py:72-100) ----
 __device__ inline double beta2_of(double E, double* gamma_out)
@@ -196,7 +208,18 @@ struct LayerLutArgs {
     double* hf;           // [nf]
     double* idd;          // [L][nz]
     double* sigma;        // [L][nz]
+    double* cum;          // [L][STRAG_N] cumulative Bohr integral (depth-dependent mode)
+    double* amp;          // [L][n_fine] per-source Gaussian amplitude (depth-dependent mode;
+                          // gk then holds 1 / (2 s^2), or -1 for s = 0)
 };
+
+__device__ inline double bohr_integrand(double e)
+{
+    double b = beta2_of(e, nullptr);
+    b = b < 1e-12 ? 1e-12 : b;
+    const double S = stopping_power(e);
+    return BOHR / (b * (S * S * S));
+}
 
 __global__ __launch_bounds__(LUT_THREADS) void layer_scalars_kernel(LayerLutArgs a)
 {
@@ -233,6 +256,77 @@ __global__ __launch_bounds__(LUT_THREADS) void layer_scalars_kernel(LayerLutArgs
     }
 }
 
+// ---- depth-dependent straggling, step 1: cumulative trapezoid of the Bohr
+// integrand, C[0] = 0, C[k+1] = C[k] + (e_{k+1} - e_k)(f_{k+1} + f_k) / 2.
+// Each thread sums a contiguous run of cells; the run totals are scanned in
+// order by one thread, so the result is deterministic. ----
+__global__ __launch_bounds__(LUT_THREADS) void strag_cum_kernel(LayerLutArgs a)
+{
+    constexpr int PER = (STRAG_N - 1 + LUT_THREADS - 1) / LUT_THREADS;
+    __shared__ double run_tot[LUT_THREADS];
+    const int l = blockIdx.x;
+    const double E0 = a.energies[l];
+    double* C = a.cum + (size_t)l * STRAG_N;
+    const bool live = lin_at(ELO, E0, STRAG_N, STRAG_N - 1) > lin_at(ELO, E0, STRAG_N, 0);
+    const int c0 = threadIdx.x * PER;
+    double loc[PER];
+    double run = 0.0;
+    double e_prev = lin_at(ELO, E0, STRAG_N, c0 < STRAG_N ? c0 : STRAG_N - 1);
+    double f_prev = live ? bohr_integrand(e_prev) : 0.0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int k = c0 + q;
+        if (live && k < STRAG_N - 1) {
+            const double e1 = lin_at(ELO, E0, STRAG_N, k + 1);
+            const double f1 = bohr_integrand(e1);
+            run += (e1 - e_prev) * (f1 + f_prev) / 2.0;
+            e_prev = e1;
+            f_prev = f1;
+        }
+        loc[q] = run;
+    }
+    run_tot[threadIdx.x] = run;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double acc = 0.0;
+        for (int t = 0; t < LUT_THREADS; ++t) {
+            const double v = run_tot[t];
+            run_tot[t] = acc;
+            acc += v;
+        }
+    }
+    __syncthreads();
+    const double off = run_tot[threadIdx.x];
+    if (threadIdx.x == 0) C[0] = 0.0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int k = c0 + q;
+        if (k < STRAG_N - 1) C[k + 1] = off + loc[q];
+    }
+}
+
+// Spread of the CSDA bin at depth zf (depth-dependent mode): hypot of the Bohr
+// straggling accumulated up to zf and the energy-spread term.
+__device__ inline double source_spread(const LayerLutArgs& a, const double* C, double E0, double R0,
+                                       double zf, double s_espr)
+{
+    double s2 = 0.0;
+    if (a.m.straggling == 0 && lin_at(ELO, E0, STRAG_N, STRAG_N - 1) > lin_at(ELO, E0, STRAG_N, 0)) {
+        double er = zf < R0 ? energy_from_range(a.E, a.R, R0 - zf) : ELO;
+        er = er < ELO ? ELO : (er > E0 ? E0 : er);
+        // np.interp(er, linspace(0.5, E0, 4000), C)
+        int k = (int)((er - ELO) / (E0 - ELO) * (double)(STRAG_N - 1));
+        k = k < 0 ? 0 : (k > STRAG_N - 2 ? STRAG_N - 2 : k);
+        while (k > 0 && lin_at(ELO, E0, STRAG_N, k) > er) --k;
+        while (k < STRAG_N - 2 && lin_at(ELO, E0, STRAG_N, k + 1) <= er) ++k;
+        const double ek = lin_at(ELO, E0, STRAG_N, k), ek1 = lin_at(ELO, E0, STRAG_N, k + 1);
+        const double ce = (C[k + 1] - C[k]) / (ek1 - ek) * (er - ek) + C[k];
+        s2 = C[STRAG_N - 1] - ce;
+        s2 = s2 > 0.0 ? s2 : 0.0;
+    }
+    return hypot(sqrt(s2), s_espr);
+}
+
 // ---- IDD, step 1: energy-conserving CSDA binning and the Gaussian kernel ----
 __global__ __launch_bounds__(LUT_THREADS) void idd_prep_kernel(LayerLutArgs a)
 {
@@ -251,6 +345,28 @@ __global__ __launch_bounds__(LUT_THREADS) void idd_prep_kernel(LayerLutArgs a)
         const double zf = lin_at(0.0, top, n, i);
         const double flu = exp(-zf / a.m.attenuation_length);
         dc[i] = flu * (eb0 - eb1) / h;
+        if (a.m.straggling_depth) {
+            const double sp = source_spread(a, a.cum + (size_t)l * STRAG_N, a.energies[l], R0, zf,
+                                            PR * a.m.energy_spread * R0);
+            double amp = 1.0, c = -1.0;
+            if (sp > 0.0) {
+                c = 1.0 / (2.0 * sp * sp);
+                if (sp >= 2.0 * h) {
+                    amp = h / (sqrt(2.0 * kPi) * sp);
+                } else {
+                    const int M = (int)ceil(12.0 * sp / h);
+                    double norm = 1.0;
+                    for (int m = 1; m <= M; ++m) {
+                        const double t = (double)m * h;
+                        norm += 2.0 * exp(-(t * t) * c);
+                    }
+                    amp = 1.0 / norm;
+                }
+            }
+            a.amp[(size_t)l * n + i] = amp;
+            gk[i] = c;
+            continue;
+        }
         double g;
         if (sR > 0.0) {
             const double t = (double)i * h;
@@ -273,6 +389,28 @@ __device__ inline double conv_at(const double* dc, const double* gk, int n, int 
     return wave_sum(s);
 }
 
+// Depth-dependent mode: sum_i dcsda[i] amp[i] exp(-((j - i) h)^2 c[i]) over the
+// sources within W bins of j (W covers 12 of the largest spreads). One wave.
+__device__ inline double conv_depth_at(const double* dc, const double* amp, const double* cc,
+                                       int n, int j, int W, double h)
+{
+    const int i0 = j - W > 0 ? j - W : 0;
+    const int i1 = j + W + 1 < n ? j + W + 1 : n;
+    double s = 0.0;
+    for (int i = i0 + (threadIdx.x & 63); i < i1; i += 64) {
+        const double c = cc[i];
+        double w;
+        if (c < 0.0) {
+            w = (i == j) ? 1.0 : 0.0;
+        } else {
+            const double t = (double)(j - i) * h;
+            w = amp[i] * exp(-(t * t) * c);
+        }
+        s += dc[i] * w;
+    }
+    return wave_sum(s);
+}
+
 constexpr int Z_PER_BLOCK = 16;
 
 // ---- IDD, step 2: evaluate the convolution at the bracketing fine points and
@@ -286,6 +424,10 @@ __global__ __launch_bounds__(LUT_THREADS) void idd_eval_kernel(LayerLutArgs a)
     const double h = lin_at(0.0, top, n, 1) - lin_at(0.0, top, n, 0);
     const double* dc = a.dcsda + (size_t)l * n;
     const double* gk = a.gk + (size_t)l * n;
+    const bool depth = a.m.straggling_depth != 0;
+    const double* amp = depth ? a.amp + (size_t)l * n : nullptr;
+    const int W = (int)ceil(12.0 * sR / h) + 2;
+    auto conv = [&](int j) { return depth ? conv_depth_at(dc, amp, gk, n, j, W, h) : conv_at(dc, gk, n, j); };
     const int wv = threadIdx.x >> 6;
     for (int q = wv; q < Z_PER_BLOCK; q += LUT_THREADS / 64) {
         const int k = blockIdx.x * Z_PER_BLOCK + q;
@@ -301,9 +443,9 @@ __global__ __launch_bounds__(LUT_THREADS) void idd_eval_kernel(LayerLutArgs a)
             while (j < n - 1 && lin_at(0.0, top, n, j + 1) <= z) ++j;
             const double zj = lin_at(0.0, top, n, j);
             if (j == n - 1 || zj == z) {
-                out = conv_at(dc, gk, n, j);
+                out = conv(j);
             } else {
-                const double d0 = conv_at(dc, gk, n, j), d1 = conv_at(dc, gk, n, j + 1);
+                const double d0 = conv(j), d1 = conv(j + 1);
                 const double slope = (d1 - d0) / (lin_at(0.0, top, n, j + 1) - zj);
                 out = slope * (z - zj) + d0;
             }
@@ -423,7 +565,7 @@ static int fe_fine_count(int nz, double z0, double dz, int n_sub, double max_ste
 }
 
 struct LutLayout {
-    size_t table, scal, dcsda, gk, T, hf, total;
+    size_t table, scal, dcsda, gk, T, hf, cum, amp, total;
     int nf;
 };
 
@@ -439,6 +581,9 @@ static LutLayout lut_layout(const PbhBeamModel* m, int nz, double z0, double dz,
     o.gk = take((size_t)L * m->n_fine_idd);
     o.T = take((size_t)L * o.nf);
     o.hf = take((size_t)o.nf);
+    const bool depth = m->straggling_depth != 0;
+    o.cum = take(depth ? (size_t)L * STRAG_N : 0);
+    o.amp = take(depth ? (size_t)L * m->n_fine_idd : 0);
     o.total = off;
     return o;
 }
@@ -473,6 +618,8 @@ static int check_model(const PbhBeamModel* m)
     if (m->n_fine_idd < 2 || m->n_fine_idd > (1 << 20)) return PBH_ERR_BAD_DIM;
     if (m->fe_n_sub < 1 || !(m->fe_max_step > 0)) return PBH_ERR_BAD_DIM;
     if (m->straggling < 0 || m->straggling > 2) return PBH_ERR_UNSUPPORTED;
+    if (m->straggling_depth < 0 || m->straggling_depth > 1) return PBH_ERR_UNSUPPORTED;
+    if (m->straggling_depth && m->straggling == 1) return PBH_ERR_UNSUPPORTED;
     if (!(m->attenuation_length > 0)) return PBH_ERR_BAD_DIM;
     return PBH_OK;
 }
@@ -512,11 +659,17 @@ extern "C" int pbh_build_lut(const PbhBeamModel* m, int nz, double z0, double dz
     a.T = (double*)(ws + lay.T);
     a.hf = (double*)(ws + lay.hf);
     a.idd = idd; a.sigma = sigma;
+    a.cum = (double*)(ws + lay.cum);
+    a.amp = (double*)(ws + lay.amp);
 
     hipLaunchKernelGGL(range_table_kernel, dim3(1), dim3(LUT_THREADS), 0, s, (double*)a.E, (double*)a.R);
     if ((rc = launch_rc("range_table_kernel")) != PBH_OK) return rc;
     hipLaunchKernelGGL(layer_scalars_kernel, dim3(n_layers), dim3(LUT_THREADS), 0, s, a);
     if ((rc = launch_rc("layer_scalars_kernel")) != PBH_OK) return rc;
+    if (m->straggling_depth && m->straggling == 0) {
+        hipLaunchKernelGGL(strag_cum_kernel, dim3(n_layers), dim3(LUT_THREADS), 0, s, a);
+        if ((rc = launch_rc("strag_cum_kernel")) != PBH_OK) return rc;
+    }
     hipLaunchKernelGGL(idd_prep_kernel, dim3(n_layers), dim3(LUT_THREADS), 0, s, a);
     if ((rc = launch_rc("idd_prep_kernel")) != PBH_OK) return rc;
     const dim3 eval_grid(cdiv(nz, Z_PER_BLOCK), n_layers);

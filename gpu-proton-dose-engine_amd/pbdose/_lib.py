@@ -34,7 +34,7 @@ class PbhBeamModel(C.Structure):
     _fields_ = [("sigma_x0", C.c_double), ("sigma_theta0", C.c_double), ("rho0", C.c_double),
                 ("energy_spread", C.c_double), ("attenuation_length", C.c_double),
                 ("fe_max_step", C.c_double), ("n_fine_idd", C.c_int32),
-                ("fe_n_sub", C.c_int32), ("straggling", C.c_int32), ("_pad", C.c_int32),
+                ("fe_n_sub", C.c_int32), ("straggling", C.c_int32), ("straggling_depth", C.c_int32),
                 ("icru_coef", C.c_double), ("icru_exp", C.c_double)]
 
 

@@ -42,6 +42,11 @@ class IDDModel:
     attenuation_length: float = 150.0
     n_fine: int = 6000
     e_cut: float = 0.05          # declared by the reference, unused there and here
+    # False: one range spread sigma_R for the whole CSDA curve (physics.py:370-373).
+    # True: the CSDA dose binned at depth z' is spread with hypot(sigma_B(z'), sigma_E),
+    # where sigma_B(z') is the Bohr straggling accumulated up to z' -- the model fix the
+    # reference lists first (README.md:180-184). Needs straggling_model "bohr" or "none".
+    depth_straggling: bool = False
 
 
 DEFAULT_IDD = IDDModel()
@@ -52,10 +57,13 @@ def beam_model(idd_model: IDDModel = DEFAULT_IDD, sigma_x0: float = 0.30,
                n_sub: int = 4, max_step: float = 0.02) -> _lib.PbhBeamModel:
     if idd_model.straggling_model not in _STRAGGLING:
         raise ValueError(f"unknown straggling model: {idd_model.straggling_model}")
+    if idd_model.depth_straggling and idd_model.straggling_model == "icru49":
+        raise ValueError("depth-dependent straggling is defined for the Bohr integral only")
     return _lib.PbhBeamModel(float(sigma_x0), float(sigma_theta0), float(rho0),
                              float(idd_model.energy_spread), float(idd_model.attenuation_length),
                              float(max_step), int(idd_model.n_fine), int(n_sub),
-                             _STRAGGLING[idd_model.straggling_model], 0,
+                             _STRAGGLING[idd_model.straggling_model],
+                             int(bool(idd_model.depth_straggling)),
                              float(idd_model.straggling_coef), float(idd_model.straggling_exp))
 
 

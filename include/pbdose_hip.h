@@ -90,7 +90,14 @@ typedef struct PbhBeamModel {
     int32_t n_fine_idd;         /* fine binning grid points          (6000) */
     int32_t fe_n_sub;           /* Fermi-Eyges subdivision           (4)    */
     int32_t straggling;         /* 0 = Bohr integral, 1 = ICRU49 fit, 2 = none */
-    int32_t _pad;
+    int32_t straggling_depth;   /* 0: one range spread sigma_R for the whole CSDA curve
+                                      (physics.py:370-373, the reference's model);
+                                   1: depth-dependent spread, the fix the reference lists
+                                      first (README.md:180-184): the CSDA dose binned at
+                                      depth z' is spread with hypot(sigma_B(z'), sigma_E),
+                                      sigma_B(z')^2 = int_{E(R0-z')}^{E0} BOHR/(beta^2 S^3) dE
+                                      (the Bohr straggling accumulated up to z') and
+                                      sigma_E = 1.77 energy_spread R0. Bohr or none only. */
     double icru_coef, icru_exp; /* ICRU49 fit sigma = coef * R0^exp (0.012, 0.935) */
 } PbhBeamModel;
 
@@ -123,6 +130,9 @@ size_t pbh_lut_workspace_bytes(const PbhBeamModel* m, int nz, int n_layers);
 
 /* Per-layer IDD (fluence-normalised, MeV/cm) and lateral sigma (cm) on the
  * depths z_k = z0 + k dz, k < nz, for layer energies energies[L] (device).
+ * With m->straggling_depth = 1 the IDD uses the depth-dependent range spread
+ * described at PbhBeamModel; sigma_R in layer_info is the spread at the end
+ * of range, which both models share.
  * Writes idd[L*nz], sigma[L*nz] (FP64) and, if layer_info != NULL,
  * layer_info[L][4] = {CSDA range R0, range spread sigma_R, depth of the IDD
  * maximum on the grid (first maximum), IDD maximum}.

@@ -85,6 +85,15 @@ def test_lut_workspace_query():
     bad.n_fine_idd = 1
     assert _lib.lib().pbh_lut_workspace_bytes(bad, 128, 40) == 0
     assert _lib.lib().pbh_build_lut(m, 128, 0.0, 0.2, 40, None, None, None, None, None, 0, None) == -1
+    # depth-dependent straggling: extra cumulative-integral and amplitude tables
+    d = physics.beam_model(physics.IDDModel(depth_straggling=True))
+    assert d.straggling_depth == 1
+    assert _lib.lib().pbh_lut_workspace_bytes(d, 128, 40) >= n + 40 * (4000 + 6000) * 8
+    icru = physics.beam_model(physics.IDDModel(straggling_model="icru49"))
+    icru.straggling_depth = 1                  # defined for the Bohr integral only
+    assert _lib.lib().pbh_lut_workspace_bytes(icru, 128, 40) == 0
+    d.straggling_depth = 2
+    assert _lib.lib().pbh_lut_workspace_bytes(d, 128, 40) == 0
 
 
 def test_product_refuses_cpu_device():

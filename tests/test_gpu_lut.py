@@ -55,3 +55,41 @@ def test_lut_with_divergence_and_icru_model():
         rs = O.fe_moments(z, e, 0.25, 0.003, 0.2)["sigma_x"]
         assert np.abs(idd[l].cpu().numpy() - ri).max() <= 1e-12 * ri.max()
         assert np.abs(sig[l].cpu().numpy() - rs).max() <= 1e-12 * rs.max()
+
+
+@pytest.mark.parametrize("case", ["small", "c1", "c2"])
+def test_depth_straggling_luts(ref_physics, case):
+    """Depth-dependent range straggling on the device vs the oracle (1e-12);
+    the lateral sigma and layer sigma_R are those of the reference model."""
+    from pbdose import physics
+    z, E = ref_physics[f"{case}_z"], ref_physics[f"{case}_E"]
+    print(f"depth-straggling LUT case {case}: {len(E)} layers x {len(z)} depths", flush=True)
+    model = physics.IDDModel(depth_straggling=True)
+    idd, sig, info = physics.build_luts(len(z), float(z[0]), float(z[1] - z[0]), E,
+                                        idd_model=model)
+    idd, sig, info = idd.cpu().numpy(), sig.cpu().numpy(), info.cpu().numpy()
+    ref = np.stack([O.idd_depth_straggling(z, e) for e in E])
+    assert np.abs(idd - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert np.abs(sig - ref_physics[f"{case}_sigma"]).max() <= 1e-12 * np.abs(sig).max()
+    np.testing.assert_allclose(info[:, 1], ref_physics[f"{case}_sigR"], rtol=1e-13)
+
+
+def test_depth_straggling_small_spreads_and_no_straggling():
+    """No energy spread: the entrance bins have spreads below two fine-grid
+    steps (discretely normalised kernels; zero spread at the surface). With
+    straggling "none" the depth model equals the reference model."""
+    from pbdose import physics
+    z = np.arange(120) * 0.1 - 0.05
+    E = np.array([E for E in (O.energy_from_range(10.0), 70.0, 180.0)])
+    m0 = physics.IDDModel(depth_straggling=True, energy_spread=0.0)
+    idd, _, _ = physics.build_luts(len(z), float(z[0]), 0.1, E, idd_model=m0)
+    ref = np.stack([O.idd_depth_straggling(z, e, energy_spread=0.0) for e in E])
+    assert np.abs(idd.cpu().numpy() - ref).max() <= 1e-12 * np.abs(ref).max()
+    m1 = physics.IDDModel(depth_straggling=True, straggling_model="none")
+    m2 = physics.IDDModel(straggling_model="none")
+    a, _, _ = physics.build_luts(len(z), float(z[0]), 0.1, E, idd_model=m1)
+    b, _, _ = physics.build_luts(len(z), float(z[0]), 0.1, E, idd_model=m2)
+    a, b = a.cpu().numpy(), b.cpu().numpy()
+    assert np.abs(a - b).max() <= 1e-13 * np.abs(b).max()
+    with pytest.raises(ValueError):
+        physics.beam_model(physics.IDDModel(depth_straggling=True, straggling_model="icru49"))
