@@ -123,17 +123,54 @@ __device__ inline void combine_body(const FwdArgs& a, const FwdPlan& pl, int til
 
 constexpr int FT_LB = 64;    // layers gathered per pass
 
+// Window of the contraction: a Gaussian factor exp(-d^2 / (2 s^2)) below
+// 2^-32 of its peak (|d| > sqrt(64 ln 2) s = 6.66 s) is dropped. Dropped taps
+// change a sum by less than 2^-32 of its largest term, far below FP32 rounding.
+constexpr double FT_WIN_SIGMAS = 6.6604;     // sqrt(2 * 32 * ln 2), rounded up
+
 // Active layers of one depth slice, in layer order, with their FP64-derived
-// scalars: c = IDD / (2 pi s^2), k2 = log2(e) / (2 s^2) (FP32 and FP64) and the
-// distance to the layer's Bragg peak (mixed "_peak" forms).
+// scalars: c = IDD / (2 pi s^2), k2 = log2(e) / (2 s^2) (FP32 and FP64), the
+// window half-width cut = FT_WIN_SIGMAS s (cm) and the distance to the layer's
+// Bragg peak (mixed "_peak" forms).
 struct LayerList {
     int lay[FT_LB];
     float c[FT_LB];
     float k2[FT_LB];
     double k2d[FT_LB];
+    float cut[FT_LB];
     float zpk[FT_LB];
     int wcount[FT_THREADS / 64];
 };
+
+// Spots of a chunk that lie within `cut` of the coordinate range [lo, hi] (all
+// in spot-index units relative to the chunk's first spot), as nstep steps of
+// `step` spots from k0. k0 is a multiple of 8 and the window ends within kpad
+// (a multiple of 8, >= n); one spot of margin on each side absorbs rounding.
+// nstep = 0 when no spot is in reach.
+struct SpotWindow { int k0, nstep; };
+
+// First and last spot (of n) within `cut` of [lo, hi], with one spot of margin;
+// first > last when none is.
+__device__ __forceinline__ void spot_range(float lo, float hi, float cut, int n, int& first, int& last)
+{
+    const float f = fminf(fmaxf(lo - cut - 1.0f, -1.0f), (float)(n + 1));
+    const float l = fmaxf(fminf(hi + cut + 1.0f, (float)n), -2.0f);
+    first = max((int)ceilf(f), 0);
+    last = min((int)floorf(l), n - 1);
+}
+
+__device__ __forceinline__ SpotWindow spot_window(float lo, float hi, float cut, int n, int step, int kpad)
+{
+    int first, last;
+    spot_range(lo, hi, cut, n, first, last);
+    SpotWindow w{0, 0};
+    if (first > last) return w;
+    w.k0 = first & ~7;
+    w.nstep = (last + 1 - w.k0 + step - 1) / step;
+    if (w.k0 + w.nstep * step > kpad) w.k0 = kpad - w.nstep * step;
+    return w;
+}
+
 
 // Gathers layers l = lb + t (t < FT_LB, l < L) whose IDD at depth z is non-zero
 // into ll (stable order, wave ballots + prefix); returns their count.
@@ -170,6 +207,7 @@ __device__ inline int collect_layers(const FwdArgs& a, int lb, int z, float zc,
         ll.c[p] = sc.c;
         ll.k2[p] = sc.k2;
         ll.k2d[p] = kLog2e / (2.0 * sd * sd);
+        ll.cut[p] = (float)(FT_WIN_SIGMAS * sd);
         ll.zpk[p] = r_peak ? fabsf(zc - r_peak[l]) : 0.0f;
     }
     __syncthreads();

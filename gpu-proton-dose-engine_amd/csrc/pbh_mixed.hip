@@ -122,6 +122,9 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
     __shared__ float uscale[2][2];
     __shared__ float bmax[2][4];
     __shared__ LayerList ll;
+    // contraction windows of the current spot chunk, per active layer: [0, 1] stage A of
+    // row block 0 / 1, [4 + w] stage B of wave w; k0 | nstep << 6
+    __shared__ unsigned char win[FT_LB][8];
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -151,6 +154,7 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
     const int ntX = pl.tx.P * (FT_TX - 1) + pl.tx.Q * (KX - 1) + 1;
 
     const int ncx = chunk_count(a.nsx), ncy = chunk_count(a.nsy);
+    const double inv_dsx = 1.0 / a.dsx, inv_dsy = 1.0 / a.dsy;
 
     v16f acc = {};
     u4v wreg[NV];
@@ -164,11 +168,32 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
         if (m1 <= m0) continue;
         for (int cy = 0; cy < ncy; ++cy) {
             const int j0 = chunk_begin(a.nsy, ncy, cy), nj = chunk_begin(a.nsy, ncy, cy + 1) - j0;
-            const int nkA = TAB ? 2 : ((nj + 31) >> 5);
             const double Cy = (a.oy + (double)y0 * a.dy) - (a.sy0 + (double)j0 * a.dsy);
             for (int cx = 0; cx < ncx; ++cx) {
                 const int i0 = chunk_begin(a.nsx, ncx, cx), ni = chunk_begin(a.nsx, ncx, cx + 1) - i0;
                 const double Cx = (a.ox + (double)x0 * a.dx) - (a.sx0 + (double)i0 * a.dsx);
+                // spots within reach of each wave's rows (stage A, K = y spots) and columns
+                // (stage B, K = x spots), in spot units of the chunk; read after the next barrier
+                for (int q = threadIdx.x; q < (m1 - m0) * 8; q += FT_THREADS) {
+                    const int m = m0 + (q >> 3), w = q & 7;
+                    if (w == 2 || w == 3) continue;
+                    SpotWindow sw;
+                    if (w < 2) {
+                        const float lo = (float)((Cy + (double)(16 * w) * a.dy) * inv_dsy);
+                        sw = spot_window(lo, lo + (float)(15.0 * a.dy * inv_dsy),
+                                              ll.cut[m] * (float)inv_dsy, nj, 32, KYP);
+                    } else {
+                        const float lo = (float)((Cx + (double)(32 * (w - 4)) * a.dx) * inv_dsx);
+                        sw = spot_window(lo, lo + (float)(31.0 * a.dx * inv_dsx),
+                                              ll.cut[m] * (float)inv_dsx, ni, 16, 16 * KB);
+                    }
+                    win[m][w] = (unsigned char)(sw.k0 | (sw.nstep << 6));
+                }
+                auto window = [&](int m, int w, int& k0) {
+                    const int code = __builtin_amdgcn_readfirstlane((int)win[m][w]);
+                    k0 = code & 63;
+                    return code >> 6;
+                };
                 const unsigned short* wchunk = pl.wsplit + ((size_t)cy * ncx + cx) * FT_WSPLIT;
                 const size_t wlayer = (size_t)ncy * ncx * FT_WSPLIT;
 
@@ -219,20 +244,24 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                     const int b = m & 1;
                     const float c = ll.c[m], k2 = ll.k2[m];
                     const bool x3 = is_x3(m);
+                    int k0;                  // K steps k0 + 32 ks, ks < nkA: y spots within reach
+                    const int nkA = window(m, rbA, k0);
+                    TabStream<QC> sw = sA;
+                    sw.shift(pl.ty, k0);
                     v4f u0 = {}, u1 = {};
 #pragma unroll
                     for (int ks = 0; ks < 2; ++ks) {
                         if (ks < nkA) {
-                            V8 gh, gl;      // Gy^T operand: j = 32 ks + 8 gA + e, y = yA
+                            V8 gh, gl;      // Gy^T operand: j = k0 + 32 ks + 8 gA + e, y = yA
                             if (TAB) {
                                 unsigned t8[8];
 #pragma unroll
-                                for (int e = 0; e < 8; ++e) t8[e] = Ty[b][sA.slot(pl.ty, 32 * ks + e)];
+                                for (int e = 0; e < 8; ++e) t8[e] = Ty[b][sw.slot(pl.ty, 32 * ks + e)];
                                 unpack8(t8, gh, gl);
                             } else {
 #pragma unroll
                                 for (int e = 0; e < 8; ++e) {
-                                    const int j = 32 * ks + 8 * gA + e;
+                                    const int j = k0 + 32 * ks + 8 * gA + e;
                                     const double d = yAc - (a.sy0 + (double)(j0 + j) * a.dsy);
                                     const float g = (j < nj) ? gauss2((float)(d * d), k2) : 0.0f;
                                     T h, lo;
@@ -241,7 +270,7 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                                     gl[e] = lo;
                                 }
                             }
-                            const int kofs = 32 * ks + 8 * gA;
+                            const int kofs = k0 + 32 * ks + 8 * gA;
                             const int off0 = (16 * cbA0 + (lane & 15)) * WT_STRIDE + kofs;
                             const int off1 = (32 + (lane & 15)) * WT_STRIDE + kofs;
                             const V8 w0h = *(const V8*)&Wh[b][off0];
@@ -305,18 +334,23 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                     const int b = m & 1;
                     const bool x3 = is_x3(m);
                     const float k2 = ll.k2[m];
+                    int k0;                  // K steps k0 + 16 ks, ks < nkB: x spots within reach
+                    const int nkB = window(m, 4 + wave, k0);
+                    TabStream<QC> sw = sB;
+                    sw.shift(pl.tx, k0);
                     V8 bh[KB], bl[KB];
 #pragma unroll
                     for (int ks = 0; ks < KB; ++ks) {
+                        if (ks >= nkB) break;
                         if (TAB) {
                             unsigned t8[8];
 #pragma unroll
-                            for (int e = 0; e < 8; ++e) t8[e] = Tx[b][sB.slot(pl.tx, 16 * ks + e)];
+                            for (int e = 0; e < 8; ++e) t8[e] = Tx[b][sw.slot(pl.tx, 16 * ks + e)];
                             unpack8(t8, bh[ks], bl[ks]);
                         } else {
 #pragma unroll
                             for (int e = 0; e < 8; ++e) {
-                                const int i = 16 * ks + 8 * gB + e;
+                                const int i = k0 + 16 * ks + 8 * gB + e;
                                 const double d = xBc - (a.sx0 + (double)(i0 + i) * a.dsx);
                                 const float g = (i < ni) ? gauss2((float)(d * d), k2) : 0.0f;
                                 T h, lo;
@@ -329,7 +363,8 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                     v16f p = {};
 #pragma unroll
                     for (int ks = 0; ks < KB; ++ks) {
-                        const int off = (lane & 31) * U_STRIDE + 16 * ks + 8 * gB;
+                        if (ks >= nkB) break;
+                        const int off = (lane & 31) * U_STRIDE + k0 + 16 * ks + 8 * gB;
                         const V8 ah = *(const V8*)&Uh[b][off];
                         p = mfma32(ah, bh[ks], p);
                         if (SPLIT && x3) {

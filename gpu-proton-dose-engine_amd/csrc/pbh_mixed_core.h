@@ -39,6 +39,17 @@ struct TabStream {
             bO = 0;
         }
     }
+    // Moves the stream k0 spots on (k0 a multiple of 8): slot(k) becomes slot(k0 + k).
+    __device__ __forceinline__ void shift(const AxisTable& g, int k0)
+    {
+        if (QC > 0) {
+            const int d = (QC * k0) >> 1;
+            bE += d;
+            bO += d;
+        } else {
+            bE += g.Q * k0;
+        }
+    }
     // QC > 0: slot of entry t0 + c for any integer offset c (compile-time after unrolling)
     __device__ __forceinline__ int at(int c) const { return ((c & 1) ? bO : bE) + (c >> 1); }
     __device__ __forceinline__ int slot(const AxisTable& g, int k) const

@@ -164,3 +164,26 @@ def test_quarter_pitch_grid():
     print("case 2.5 mm voxels / 5 mm spots", flush=True)
     p = make_problem(96, 96, 10, 0.25, 31, 31, 0.5, np.linspace(90, 160, 8), seed=17)
     check(p)
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "bf16", "fp16x3", "bf16x3_peak"])
+@pytest.mark.parametrize("case", ["offset_field", "direct_factors", "spot_chunks"])
+def test_windowed_contraction(precision, case):
+    """The mixed kernels skip K steps whose spots lie beyond 6.66 sigma (Gaussian
+    factor below 2^-32 of its peak) of every row or column of a wave. Small,
+    off-centre fields and narrow beams on wide grids leave most waves with partial
+    or empty windows; every form must stay within its bound, and voxels far from
+    every spot must stay negligible."""
+    print(f"case windowed {case} {precision}", flush=True)
+    if case == "offset_field":
+        p = make_problem(128, 96, 10, 0.2, 9, 7, 0.5, [70.0, 95.0], x0=-6.0, y0=4.0,
+                         sigma_x0=0.12, seed=31)
+    elif case == "direct_factors":
+        p = make_problem(128, 64, 6, 0.2, 11, 9, 0.37, [80.0, 120.0], x0=5.0,
+                         sigma_x0=0.15, seed=32)
+    else:
+        p = make_problem(160, 40, 4, 0.2, 97, 13, 0.3, [90.0, 140.0], sigma_x0=0.1, seed=33)
+    D, ref = check(p, tol=MIXED_TOL[precision], precision=precision)
+    far = ref < 1e-12 * ref.max()
+    assert far.any()
+    assert np.abs(D[far]).max() <= 1e-7 * ref.max()
