@@ -58,6 +58,12 @@ __device__ __forceinline__ void adj_mixed_body(const AdjArgs& A, int blk)
     __shared__ int lay[AM_MAXK];        // this workgroup's active layers, in layer order,
     __shared__ double lk2[AM_MAXK];     //   their log2(e) / (2 s^2)
     __shared__ double lcn[AM_MAXK];     //   and c = IDD / (2 pi s^2), in layer order
+    __shared__ float lcut[AM_MAXK];     //   and window half-width FT_WIN_SIGMAS s (cm)
+    // K-step windows per layer of the current tile group (double-buffered by group):
+    // [0..2] stage 1, x voxels within reach of spot block nb (steps of 32 voxels);
+    // [3 + 3 u + mb] stage 2, rows of tile u within reach of spot block mb (steps of 32
+    // rows); each lo | hi << 4, steps lo .. hi - 1
+    __shared__ unsigned char awin[2][AM_MAXK][9];
     __shared__ int wcnt[FT_THREADS / 64];
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -116,6 +122,7 @@ __device__ __forceinline__ void adj_mixed_body(const AdjArgs& A, int blk)
             const double sd = sg > kSigmaFloor ? sg : kSigmaFloor;
             lk2[idx / A.split] = kLog2e / (2.0 * sd * sd);
             lcn[idx / A.split] = a.idd[(size_t)l * a.nz + z] / (2.0 * kPi * sd * sd);
+            lcut[idx / A.split] = (float)(FT_WIN_SIGMAS * sd);
             lay[idx / A.split] = l;
         }
         base += tot;
@@ -137,6 +144,30 @@ __device__ __forceinline__ void adj_mixed_body(const AdjArgs& A, int blk)
                 for (int tg = 0; tg < tiles_y; tg += AM_NT) {
                     const int nt = min(AM_NT, tiles_y - tg);
                     const bool first_group = txi == 0 && tg == 0;
+                    // windows of this group's layers (read after the first layer's barrier;
+                    // the other buffer may still be read by the previous group's stage 2)
+                    const int par = (tg / AM_NT) & 1;
+                    for (int e = threadIdx.x; e < nmine * 9; e += FT_THREADS) {
+                        const int q = e / 9, w = e % 9;
+                        int first = 1, last = 0;
+                        if (w < 3) {
+                            if (16 * w < ni) {
+                                const float s0 = (float)((16.0 * w * a.dsx - Cx) / a.dx);
+                                const float s1 = (float)(((double)min(16 * w + 15, ni - 1) * a.dsx - Cx) / a.dx);
+                                spot_range(s0, s1, lcut[q] * (float)(1.0 / a.dx), AM_TX, first, last);
+                            }
+                        } else {
+                            const int u = (w - 3) / 3, mb = (w - 3) % 3;
+                            if (u < nt && 16 * mb < nj) {
+                                const double Cy = (a.oy + (double)((tg + u) * AM_TY) * a.dy) -
+                                                  (a.sy0 + (double)j0 * a.dsy);
+                                const float s0 = (float)((16.0 * mb * a.dsy - Cy) / a.dy);
+                                const float s1 = (float)(((double)min(16 * mb + 15, nj - 1) * a.dsy - Cy) / a.dy);
+                                spot_range(s0, s1, lcut[q] * (float)(1.0 / a.dy), AM_TY, first, last);
+                            }
+                        }
+                        awin[par][q][w] = first > last ? 0 : (unsigned char)((first >> 5) | ((last / 32 + 1) << 4));
+                    }
                     V8 rh[AM_NT][4], rl[AM_NT][4];
 #pragma unroll
                     for (int u = 0; u < AM_NT; ++u) {
@@ -192,9 +223,12 @@ __device__ __forceinline__ void adj_mixed_body(const AdjArgs& A, int blk)
                             if (u >= nt) break;
 #pragma unroll 1
                             for (int nb = 0; nb < 3; ++nb) {
+                                const int wc = __builtin_amdgcn_readfirstlane((int)awin[par][q][nb]);
+                                const int k_lo = wc & 15, k_hi = wc >> 4;
                                 v4f acc1 = {};
 #pragma unroll
                                 for (int ks = 0; ks < 4; ++ks) {
+                                    if (ks < k_lo || ks >= k_hi) continue;
                                     unsigned t8[8];
 #pragma unroll
                                     for (int e = 0; e < 8; ++e) t8[e] = Tx[slot1(nb, ks, e)];
@@ -233,8 +267,11 @@ __device__ __forceinline__ void adj_mixed_body(const AdjArgs& A, int blk)
                             const int mb = b2 / 3, nb = b2 % 3;
                             v4f p = {};
                             for (int u = 0; u < nt; ++u) {
+                                const int wc = __builtin_amdgcn_readfirstlane((int)awin[par][q][3 + 3 * u + mb]);
+                                const int k_lo = wc & 15, k_hi = wc >> 4;
 #pragma unroll
                                 for (int ks = 0; ks < 2; ++ks) {
+                                    if (ks < k_lo || ks >= k_hi) continue;
                                     unsigned t8[8];
 #pragma unroll
                                     for (int e = 0; e < 8; ++e) t8[e] = Ty[u][slot2(mb, ks, e)];

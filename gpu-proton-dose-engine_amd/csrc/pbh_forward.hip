@@ -26,6 +26,10 @@
 // Each layer's contribution is summed in fresh accumulators and added to the
 // running dose once (rounding chain per voxel: K steps + layers). Tables, W and
 // U are double-buffered in LDS: one barrier per layer.
+#include <algorithm>
+#include <functional>
+#include <vector>
+
 #include "pbh_fwd_core.h"
 
 namespace pbh {
@@ -344,6 +348,49 @@ int pbh_auto_chunk(const FwdArgs& a)
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Layer chunk from the active-layer count of every depth slice: simulates the
+// dispatch of the workgroups in launch order (chunk-major, slices in order)
+// onto `slots` resident workgroup slots, each taking the next workgroup when it
+// frees, with a workgroup costing its layers plus FT_WG_COST layers of fixed
+// work (empty later chunks exit at once), and returns the chunk with the
+// shortest simulated makespan (the larger chunk within 2 %: fewer partials).
+// Measured on the 128^3 field this picks 18 (bf16x3, 3 slots per CU) and 26
+// (FP32, 2 slots per CU), 4 % and 14 % faster than two chunks per tile.
+constexpr double FT_WG_COST = 2.0;
+
+static int model_chunk(const FwdArgs& a, const int32_t* active, long long slots)
+{
+    const long long per_z = (long long)a.tiles_x * a.tiles_y;
+    if (slots < 1) slots = 1;
+    if (slots > (1 << 20)) slots = 1 << 20;
+    int best = a.L;
+    double best_t = 0.0;
+    std::vector<double> heap;
+    for (int c = a.L; c >= 1; --c) {
+        const int nch = (a.L + c - 1) / c;
+        if ((long long)nch * per_z * a.nz > (1LL << 24)) break;     // keep the model cheap
+        heap.assign((size_t)slots, 0.0);                              // min-heap of slot finish times
+        for (int ch = 0; ch < nch; ++ch) {
+            for (int z = 0; z < a.nz; ++z) {
+                const int act = active[z] < 0 ? 0 : (active[z] > a.L ? a.L : active[z]);
+                const int n = std::min(c, std::max(0, act - ch * c));
+                const double w = (ch > 0 && n == 0) ? 0.05 : n + FT_WG_COST;
+                for (long long t = 0; t < per_z; ++t) {
+                    std::pop_heap(heap.begin(), heap.end(), std::greater<double>());
+                    heap.back() += w;
+                    std::push_heap(heap.begin(), heap.end(), std::greater<double>());
+                }
+            }
+        }
+        const double span = *std::max_element(heap.begin(), heap.end());
+        if (c == a.L || span < best_t * 0.98) {
+            best = c;
+            best_t = span;
+        }
+    }
+    return best;
+}
+
 static int nchunks(const FwdArgs& a, int chunk) { return (a.L + chunk - 1) / chunk; }
 
 // Workspace layout: [layer peaks (mixed "_peak" forms)][split 16-bit weights
@@ -419,6 +466,28 @@ extern "C" size_t pbh_forward_workspace_bytes_tuned(const PbhGrid* g, const PbhL
     (void)precision;
     const FwdArgs a = make_fwd_args(g, lat, nullptr, nullptr, nullptr, nullptr);
     return pbh_forward_ws_bytes(a, layer_chunk > 0 && layer_chunk < a.L ? layer_chunk : (layer_chunk ? a.L : 0));
+}
+
+extern "C" int pbh_forward_choose_chunk(const PbhGrid* g, const PbhLattice* lat, int precision,
+                                        const int32_t* active_per_z, int compute_units)
+{
+    int rc = validate_problem(g, lat);
+    if (rc != PBH_OK) return rc;
+    if (!active_per_z) return PBH_ERR_NULL_ARG;
+    if (precision < PBH_PREC_FP32 || precision > PBH_PREC_FP16X3_PEAK) return PBH_ERR_UNSUPPORTED;
+    const FwdArgs a = make_fwd_args(g, lat, nullptr, nullptr, nullptr, nullptr);
+    // resident workgroups per CU: the mixed kernels on the P = 2, Q = 5 table geometry
+    // hold 3, the other forward kernels 2
+    int per_cu = 2;
+    if (precision != PBH_PREC_FP32) {
+        FwdPlan pl;
+        if (pbh_plan_forward(a, chunk_pad(a.nsx, a.nsy), chunk_pad(a.nsx, a.nsy), nullptr, 0, a.L, &pl) ==
+                PBH_OK &&
+            pl.table && pl.tx.P == 2 && pl.tx.Q == 5 && pl.ty.P == 2 && pl.ty.Q == 5)
+            per_cu = 3;
+    }
+    const int cus = compute_units > 0 ? compute_units : device_cus();
+    return model_chunk(a, active_per_z, (long long)per_cu * cus);
 }
 
 extern "C" size_t pbh_forward_workspace_bytes(const PbhGrid* g, const PbhLattice* lat, int precision)

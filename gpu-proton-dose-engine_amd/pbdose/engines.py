@@ -48,6 +48,18 @@ class DeviceProblem:
             raise ValueError(f"idd/sigma must be (L, nz) = {(L, g.nz)}")
         _lib.check(_lib.lib().pbh_validate(self.grid_s, self.lat_s), "pbh_validate")
         self._ws = {}
+        # active layers per depth slice (one small copy to the host), for the layer chunk
+        self._active = (self.idd != 0).sum(dim=0).to(torch.int32).cpu().contiguous()
+        self._chunk = {}
+
+    def auto_chunk(self, precision: int) -> int:
+        """Layer chunk of the automatic forward for this problem's active layers."""
+        if precision not in self._chunk:
+            c = _lib.lib().pbh_forward_choose_chunk(self.grid_s, self.lat_s, precision,
+                                                    self._active.data_ptr(), 0)
+            _lib.check(min(c, 0), "pbh_forward_choose_chunk")
+            self._chunk[precision] = int(c)
+        return self._chunk[precision]
 
     def workspace(self, kind: str, precision: int, layer_chunk: int = 0) -> torch.Tensor:
         key = (kind, precision, layer_chunk)
@@ -64,7 +76,8 @@ class DeviceProblem:
                 w: Optional[torch.Tensor] = None, layer_chunk: int = 0) -> torch.Tensor:
         """Dose (nz, ny, nx) float32, asynchronous on the current stream.
         ``layer_chunk``: at most this many active layers of a depth slice per
-        workgroup (0 = automatic, >= L = one workgroup per dose tile)."""
+        workgroup (0 = chosen from this problem's active layers per slice by
+        pbh_forward_choose_chunk, >= L = one workgroup per dose tile)."""
         p = _lib.PREC[precision]
         if out is None:
             out = torch.empty(self.shape, dtype=torch.float32, device=self.dev)
@@ -73,6 +86,8 @@ class DeviceProblem:
         wt = self.w if w is None else w
         if wt.dtype != torch.float32 or tuple(wt.shape) != tuple(self.w.shape) or not wt.is_contiguous():
             raise ValueError("weights must be a contiguous float32 (L, nsy, nsx) tensor")
+        if int(layer_chunk) == 0:
+            layer_chunk = self.auto_chunk(p)
         ws = self.workspace("fwd", p, int(layer_chunk))
         _lib.check(_lib.lib().pbh_forward_tuned(self.grid_s, self.lat_s, _lib.ptr(wt),
                                                 _lib.ptr(self.idd), _lib.ptr(self.sigma),

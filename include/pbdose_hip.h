@@ -171,6 +171,15 @@ int pbh_forward_tuned(const PbhGrid* g, const PbhLattice* lat,
                       float* dose, int precision, int layer_chunk,
                       void* workspace, size_t workspace_bytes, void* stream);
 
+/* Layer chunk for pbh_forward_tuned from the number of active layers (IDD != 0)
+ * of every depth slice, active_per_z[nz] (host memory): simulates the dispatch
+ * of the forward's workgroups onto the resident slots (compute_units CUs, <= 0:
+ * the current device's) and returns the chunk with the shortest makespan, or a
+ * negative PBH_ERR_* code. Host only; no counterpart in the reference (its
+ * layer_chunk, engines.py:208, bounds memory, not load balance). */
+int pbh_forward_choose_chunk(const PbhGrid* g, const PbhLattice* lat, int precision,
+                             const int32_t* active_per_z, int compute_units);
+
 /* ---------------------------------------------------------------------------
  * Adjoint of the forward map (inverse planning).
  * ------------------------------------------------------------------------- */

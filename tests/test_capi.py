@@ -203,3 +203,26 @@ def test_weight_step_argument_checks():
     assert L.pbh_weight_step(one, one, one, one, 0, 10, 0, 0.1, 0.0, 1.0, nul) == -2
     assert L.pbh_weight_step(one, one, one, one, 2, 10, 7, 0.1, 0.0, 1.0, nul) == -5
     assert L.pbh_weight_step(one, one, one, one, 2, 10, 0, 0.1, 2.0, 1.0, nul) == -2   # w_min > w_max
+
+
+def test_forward_chunk_chooser():
+    """Host-only layer-chunk model: argument checks, no chunking when every slice
+    has all layers active and the tiles fill the resident slots exactly, and the
+    chunks measured fastest on the 128^3 / 41x41 x 40-layer field."""
+    import numpy as np
+    from oracle import pb_oracle as O
+    from pbdose import model
+    L = _lib.lib()
+    g = model.DoseGrid(128, 128, 128, 0.2, 0.2, 0.2, -12.7, -12.7, 0.0)
+    lat = model.SpotLattice(41, 41, 40, 0.5, 0.5)
+    gs, ls = _lib.grid_struct(g), _lib.lattice_struct(lat)
+    full = np.full(128, 40, np.int32)
+    assert L.pbh_forward_choose_chunk(gs, ls, 0, None, 256) == -1
+    assert L.pbh_forward_choose_chunk(gs, ls, 9, full.ctypes.data, 256) == -5
+    # 512 tiles on 2 x 256 slots: one round without chunking
+    assert L.pbh_forward_choose_chunk(gs, ls, 0, full.ctypes.data, 256) == 40
+    idd, _ = O.layer_luts(g.z, np.linspace(70, 180, 40))
+    act = np.ascontiguousarray((idd != 0).sum(0).astype(np.int32))
+    assert L.pbh_forward_choose_chunk(gs, ls, 0, act.ctypes.data, 256) == 26     # FP32
+    for prec in (1, 3):                                                          # BF16, BF16x3
+        assert 16 <= L.pbh_forward_choose_chunk(gs, ls, prec, act.ctypes.data, 256) <= 20
