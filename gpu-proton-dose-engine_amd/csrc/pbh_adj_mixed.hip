@@ -201,19 +201,25 @@ __device__ __forceinline__ void adj_mixed_body(const AdjArgs& A, int blk)
                                 rl[u][ks][e] = l;
                             }
                     }
+                    const TableRow rowx = table_row(gx, Cx, AM_TX);
+                    TableRow rowy[AM_NT];
+#pragma unroll
+                    for (int u = 0; u < AM_NT; ++u)
+                        rowy[u] = table_row(gy, (a.oy + (double)((tg + u) * AM_TY) * a.dy) -
+                                                    (a.sy0 + (double)j0 * a.dsy), AM_TY);
                     for (int q = 0; q < nmine; ++q) {
-                        const double k2d = lk2[q];
+                        const float k2 = (float)lk2[q];
                         __syncthreads();        // previous layer's readers of the tables and Q are done
                         for (int t = threadIdx.x; t < ntX; t += FT_THREADS) {
-                            const double d = table_distance(gx, Cx, AM_TX, t);
-                            Tx[table_slot(gx, t)] = pack_split<T>(__builtin_amdgcn_exp2f((float)(-(d * d) * k2d)));
+                            const float d = table_distance_f(rowx, t);
+                            Tx[table_slot(gx, t)] = pack_split<T>(__builtin_amdgcn_exp2f(-(d * d) * k2));
                         }
-                        for (int u = 0; u < nt; ++u) {
-                            const double Cy = (a.oy + (double)((tg + u) * AM_TY) * a.dy) - (a.sy0 + (double)j0 * a.dsy);
+#pragma unroll
+                        for (int u = 0; u < AM_NT; ++u) {
+                            if (u >= nt) break;
                             for (int t = threadIdx.x; t < ntY; t += FT_THREADS) {
-                                const double d = table_distance(gy, Cy, AM_TY, t);
-                                Ty[u][table_slot(gy, t)] =
-                                    pack_split<T>(__builtin_amdgcn_exp2f((float)(-(d * d) * k2d)));
+                                const float d = table_distance_f(rowy[u], t);
+                                Ty[u][table_slot(gy, t)] = pack_split<T>(__builtin_amdgcn_exp2f(-(d * d) * k2));
                             }
                         }
                         __syncthreads();

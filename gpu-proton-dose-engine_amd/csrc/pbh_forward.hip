@@ -18,10 +18,10 @@
 // Gaussian factors. On regular grids whose voxel and spot pitches have a small
 // integer ratio (dx = P u, ds = Q u: 2 mm voxels and 5 mm spots give P = 2,
 // Q = 5), voxel-spot distances within a tile take only P (T-1) + Q (K-1) + 1
-// values, so each (layer, depth) builds one small table per axis in LDS (FP64
-// argument, one exp2 each) and the MFMA operands are gathered from it: ~490
-// exponentials per 128 x 48 tile instead of ~6000. Other grids evaluate every
-// factor directly (FP64 distance, FP32 exponent).
+// values, so each (layer, depth) builds one small table per axis in LDS (FP32
+// argument with an exact integer part, one exp2 each) and the MFMA operands are
+// gathered from it: ~490 exponentials per 128 x 48 tile instead of ~6000. Other
+// grids evaluate every factor directly (FP64 distance, FP32 exponent).
 //
 // Each layer's contribution is summed in fresh accumulators and added to the
 // running dose once (rounding chain per voxel: K steps + layers). Tables, W and
@@ -102,6 +102,8 @@ __device__ __forceinline__ void fwd_fp32_body(const FwdArgs& a, const FwdPlan& p
             for (int cx = 0; cx < ncx; ++cx) {
                 const int i0 = chunk_begin(a.nsx, ncx, cx), ni = chunk_begin(a.nsx, ncx, cx + 1) - i0;
                 const double Cx = (a.ox + (double)x0 * a.dx) - (a.sx0 + (double)i0 * a.dsx);
+                const TableRow rowx = TAB ? table_row(pl.tx, Cx, FT_TX) : TableRow{0, 0.0f, 0.0f};
+                const TableRow rowy = TAB ? table_row(pl.ty, Cy, FT_TY) : TableRow{0, 0.0f, 0.0f};
 
                 auto load_w = [&](int m) {
                     const float* wl = a.w + (size_t)ll.lay[m] * a.nsy * a.nsx + (size_t)j0 * a.nsx + i0;
@@ -121,18 +123,18 @@ __device__ __forceinline__ void fwd_fp32_body(const FwdArgs& a, const FwdPlan& p
                 };
                 auto build_tx = [&](int m) {
                     if (!TAB) return;
-                    const double k2d = ll.k2d[m];
+                    const float k2 = ll.k2[m];
                     for (int t = threadIdx.x; t < ntX; t += FT_THREADS) {
-                        const double d = table_distance(pl.tx, Cx, FT_TX, t);
-                        Tx[m & 1][table_slot(pl.tx, t)] = __builtin_amdgcn_exp2f((float)(-(d * d) * k2d));
+                        const float d = table_distance_f(rowx, t);
+                        Tx[m & 1][table_slot(pl.tx, t)] = __builtin_amdgcn_exp2f(-(d * d) * k2);
                     }
                 };
                 auto build_ty = [&](int m) {
                     if (!TAB) return;
-                    const double k2d = ll.k2d[m];
+                    const float k2 = ll.k2[m];
                     for (int t = threadIdx.x; t < ntY; t += FT_THREADS) {
-                        const double d = table_distance(pl.ty, Cy, FT_TY, t);
-                        Ty[m & 1][table_slot(pl.ty, t)] = __builtin_amdgcn_exp2f((float)(-(d * d) * k2d));
+                        const float d = table_distance_f(rowy, t);
+                        Ty[m & 1][table_slot(pl.ty, t)] = __builtin_amdgcn_exp2f(-(d * d) * k2);
                     }
                 };
                 // stage A of layer m: U[m&1] = c Gy W
@@ -354,8 +356,9 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 // frees, with a workgroup costing its layers plus FT_WG_COST layers of fixed
 // work (empty later chunks exit at once), and returns the chunk with the
 // shortest simulated makespan (the larger chunk within 2 %: fewer partials).
-// Measured on the 128^3 field this picks 18 (bf16x3, 3 slots per CU) and 26
-// (FP32, 2 slots per CU), 4 % and 14 % faster than two chunks per tile.
+// On the 128^3 field this picks 17 (bf16x3, 3 slots per CU) and 26 (FP32, 2
+// slots per CU); a measured sweep had 18-24 and 26 fastest, 4 % and 14 % ahead
+// of two chunks per tile.
 constexpr double FT_WG_COST = 2.0;
 
 static int model_chunk(const FwdArgs& a, const int32_t* active, long long slots)
@@ -476,8 +479,10 @@ extern "C" int pbh_forward_choose_chunk(const PbhGrid* g, const PbhLattice* lat,
     if (!active_per_z) return PBH_ERR_NULL_ARG;
     if (precision < PBH_PREC_FP32 || precision > PBH_PREC_FP16X3_PEAK) return PBH_ERR_UNSUPPORTED;
     const FwdArgs a = make_fwd_args(g, lat, nullptr, nullptr, nullptr, nullptr);
-    // resident workgroups per CU: the mixed kernels on the P = 2, Q = 5 table geometry
-    // hold 3, the other forward kernels 2
+    // workgroup slots per CU: 3 for the mixed kernels on the P = 2, Q = 5 table
+    // geometry, 2 for the others. (The single-product kernel holds 4, but those four
+    // share the CU's issue bandwidth; a sweep measured its best chunk where the
+    // 3-slot model puts it: 16-17, against 13 with 4 slots.)
     int per_cu = 2;
     if (precision != PBH_PREC_FP32) {
         FwdPlan pl;

@@ -7,7 +7,7 @@
 //   stage B  D[32 x 128] += (c U) Gx^T                 v_mfma_f32_32x32x16_{bf16,f16}
 // Stage A computes U transposed so that each lane's four results are four
 // consecutive spots of one row of U, stored with one 8-byte LDS write per
-// (hi, lo) half. Factors are formed in FP32 (table: FP64 argument), then rounded
+// (hi, lo) half. Factors are formed in FP32 (table: exact-integer-part argument), then rounded
 // to the 16-bit type. In the split ("x3") forms every operand a is stored as
 // hi = T(a), lo = T(a - hi) and a product is hi*hi + hi*lo + lo*hi (lo*lo
 // dropped), which restores ~2^-16 (BF16) or ~2^-21 (FP16) relative accuracy per
@@ -172,6 +172,8 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
             for (int cx = 0; cx < ncx; ++cx) {
                 const int i0 = chunk_begin(a.nsx, ncx, cx), ni = chunk_begin(a.nsx, ncx, cx + 1) - i0;
                 const double Cx = (a.ox + (double)x0 * a.dx) - (a.sx0 + (double)i0 * a.dsx);
+                const TableRow rowx = TAB ? table_row(pl.tx, Cx, FT_TX) : TableRow{0, 0.0f, 0.0f};
+                const TableRow rowy = TAB ? table_row(pl.ty, Cy, FT_TY) : TableRow{0, 0.0f, 0.0f};
                 // spots within reach of each wave's rows (stage A, K = y spots) and columns
                 // (stage B, K = x spots), in spot units of the chunk; read after the next barrier
                 for (int q = threadIdx.x; q < (m1 - m0) * 8; q += FT_THREADS) {
@@ -223,20 +225,18 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                 };
                 auto build_tx = [&](int m) {
                     if (!TAB) return;
-                    const double k2d = ll.k2d[m];
+                    const float k2 = ll.k2[m];
                     for (int t = threadIdx.x; t < ntX; t += FT_THREADS) {
-                        const double d = table_distance(pl.tx, Cx, FT_TX, t);
-                        Tx[m & 1][table_slot(pl.tx, t)] =
-                            pack_split<T>(__builtin_amdgcn_exp2f((float)(-(d * d) * k2d)));
+                        const float d = table_distance_f(rowx, t);
+                        Tx[m & 1][table_slot(pl.tx, t)] = pack_split<T>(__builtin_amdgcn_exp2f(-(d * d) * k2));
                     }
                 };
                 auto build_ty = [&](int m) {
                     if (!TAB) return;
-                    const double k2d = ll.k2d[m];
+                    const float k2 = ll.k2[m];
                     for (int t = threadIdx.x; t < ntY; t += FT_THREADS) {
-                        const double d = table_distance(pl.ty, Cy, FT_TY, t);
-                        Ty[m & 1][table_slot(pl.ty, t)] =
-                            pack_split<T>(__builtin_amdgcn_exp2f((float)(-(d * d) * k2d)));
+                        const float d = table_distance_f(rowy, t);
+                        Ty[m & 1][table_slot(pl.ty, t)] = pack_split<T>(__builtin_amdgcn_exp2f(-(d * d) * k2));
                     }
                 };
                 // stage A of layer m: U[m&1] = c Gy W (computed as W^T Gy^T), split into (hi, lo)
@@ -414,12 +414,15 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
     store_block(a, pl, tile, ch, z, y0, x0, 0, cnB, acc, (size_t)tiles);
 }
 
-// Workgroups per CU: three on the P = 2, Q = 5 geometry (exact-size tables keep
-// LDS near 51 KB), two otherwise.
-template <int QC> struct MixedOcc { static constexpr int value = QC ? 3 : 2; };
+// Workgroups per CU: on the P = 2, Q = 5 geometry (exact-size tables) three for
+// the split forms (LDS near 51 KB) and four for the single-product forms (30 KB,
+// VGPRs held to 128); two otherwise.
+template <int MODE, int QC> struct MixedOcc {
+    static constexpr int value = !QC ? 2 : (MODE == MODE_PLAIN ? 4 : 3);
+};
 
 template <typename T, int MODE, bool TAB, int KB, int QC>
-__global__ __launch_bounds__(FT_THREADS, MixedOcc<QC>::value)
+__global__ __launch_bounds__(FT_THREADS, (MixedOcc<MODE, QC>::value))
 void fwd_mixed_kernel(FwdArgs a, FwdPlan pl, double oz, double dz, const float* r_peak)
 {
     fwd_mixed_body<T, MODE, TAB, KB, QC>(a, pl, oz, dz, r_peak, blockIdx.x);
@@ -427,7 +430,7 @@ void fwd_mixed_kernel(FwdArgs a, FwdPlan pl, double oz, double dz, const float* 
 
 // One launch over a batch of problems sharing the template parameters.
 template <typename T, int MODE, bool TAB, int KB, int QC>
-__global__ __launch_bounds__(FT_THREADS, MixedOcc<QC>::value)
+__global__ __launch_bounds__(FT_THREADS, (MixedOcc<MODE, QC>::value))
 void fwd_mixed_batch_kernel(const BatchFwdItem* __restrict__ items, const int* __restrict__ wg0, int n)
 {
     const int k = batch_item(wg0, n, blockIdx.x);
