@@ -265,33 +265,44 @@ void adj_fp32_batch_kernel(const AdjArgs* __restrict__ items, const int* __restr
 }
 
 // grad_w[e] = sum_z part[z][e] in FP64, increasing z (8 loads in flight per thread)
-__device__ __forceinline__ void adj_reduce_body(const float* part, int nz, size_t n, float* grad_w, size_t e)
+// grad_w[e] = sum_z part[z][e] in FP64. Workgroup blk takes elements 64 blk ..
+// 64 blk + 63; wave w sums the slices z = w, w + 4, ... of one element per lane
+// (coalesced rows), and the four wave sums are added in wave order, so the
+// result does not depend on scheduling.
+
+__device__ __forceinline__ void adj_reduce_body(const float* part, int nz, size_t n, float* grad_w, int blk)
 {
-    if (e >= n) return;
+    __shared__ double ws[4][RED_ELEMS];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const size_t e = (size_t)blk * RED_ELEMS + lane;
     double acc = 0.0;
-    int z = 0;
-    for (; z + 8 <= nz; z += 8) {
-        float v[8];
+    if (e < n) {
+        int z = wave;
+        for (; z + 12 < nz; z += 16) {
+            float v[4];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) v[k] = part[(size_t)(z + k) * n + e];
+            for (int k = 0; k < 4; ++k) v[k] = part[(size_t)(z + 4 * k) * n + e];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) acc += (double)v[k];
+            for (int k = 0; k < 4; ++k) acc += (double)v[k];
+        }
+        for (; z < nz; z += 4) acc += (double)part[(size_t)z * n + e];
     }
-    for (; z < nz; ++z) acc += (double)part[(size_t)z * n + e];
-    grad_w[e] = (float)acc;
+    ws[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0 && e < n) grad_w[e] = (float)(((ws[0][lane] + ws[1][lane]) + ws[2][lane]) + ws[3][lane]);
 }
 
-__global__ void adj_reduce_kernel(const float* part, int nz, size_t n, float* grad_w)
+__global__ __launch_bounds__(256) void adj_reduce_kernel(const float* part, int nz, size_t n, float* grad_w)
 {
-    adj_reduce_body(part, nz, n, grad_w, (size_t)blockIdx.x * blockDim.x + threadIdx.x);
+    adj_reduce_body(part, nz, n, grad_w, blockIdx.x);
 }
 
-__global__ void adj_reduce_batch_kernel(const BatchRedItem* __restrict__ items, const int* __restrict__ wg0, int n)
+__global__ __launch_bounds__(256) void adj_reduce_batch_kernel(const BatchRedItem* __restrict__ items,
+                                                               const int* __restrict__ wg0, int n)
 {
     const int k = batch_item(wg0, n, blockIdx.x);
     const BatchRedItem it = items[k];
-    adj_reduce_body(it.part, it.nz, it.n, it.grad_w,
-                    (size_t)((int)blockIdx.x - wg0[k]) * blockDim.x + threadIdx.x);
+    adj_reduce_body(it.part, it.nz, it.n, it.grad_w, (int)blockIdx.x - wg0[k]);
 }
 
 // ---- weighted quadratic objective: R = 2 wv (D - Dp), J = sum wv (D - Dp)^2 ----
@@ -303,12 +314,31 @@ __device__ __forceinline__ void objective_body(const float* D, const float* Dp, 
 {
     __shared__ double red[OBJ_THREADS / 64];
     double acc = 0.0;
-    for (size_t e = (size_t)blk * OBJ_THREADS + threadIdx.x; e < n; e += (size_t)nblk * OBJ_THREADS) {
-        const double r = (double)D[e] - (double)Dp[e];
-        const double w = wv ? (double)wv[e] : 1.0;
+    auto one = [&](float d, float p, float wf, float* r_out) {
+        const double r = (double)d - (double)p;
+        const double w = (double)wf;
         acc += w * r * r;
-        if (R) R[e] = (float)(2.0 * w * r);
+        if (r_out) *r_out = (float)(2.0 * w * r);
+    };
+    const bool vec = (((uintptr_t)D | (uintptr_t)Dp | (uintptr_t)wv | (uintptr_t)R) & 15) == 0;
+    size_t done = 0;
+    if (vec) {
+        // four voxels per thread and step (16-byte loads and stores)
+        const size_t n4 = n / 4;
+        for (size_t e = (size_t)blk * OBJ_THREADS + threadIdx.x; e < n4; e += (size_t)nblk * OBJ_THREADS) {
+            const float4 d = ((const float4*)D)[e], p = ((const float4*)Dp)[e];
+            const float4 w = wv ? ((const float4*)wv)[e] : make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+            float4 r;
+            one(d.x, p.x, w.x, &r.x);
+            one(d.y, p.y, w.y, &r.y);
+            one(d.z, p.z, w.z, &r.z);
+            one(d.w, p.w, w.w, &r.w);
+            if (R) ((float4*)R)[e] = r;
+        }
+        done = 4 * n4;
     }
+    for (size_t e = done + (size_t)blk * OBJ_THREADS + threadIdx.x; e < n; e += (size_t)nblk * OBJ_THREADS)
+        one(D[e], Dp[e], wv ? wv[e] : 1.0f, R ? R + e : nullptr);
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
     __syncthreads();
@@ -456,8 +486,8 @@ extern "C" int pbh_adjoint_tuned(const PbhGrid* g, const PbhLattice* lat, const 
     if (mixed) {
         if (!A.tab) return PBH_ERR_UNSUPPORTED;   // the BF16 adjoint gathers every factor from tables
         if ((rc = pbh_launch_adj_mixed(A, precision, g->nz, s)) != PBH_OK) return rc;
-        hipLaunchKernelGGL(adj_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                           (const float*)A.part, g->nz, n, grad_w);
+        hipLaunchKernelGGL(adj_reduce_kernel, dim3((unsigned)((n + RED_ELEMS - 1) / RED_ELEMS)), dim3(256), 0,
+                           s, (const float*)A.part, g->nz, n, grad_w);
         return launch_rc("adj_reduce_kernel");
     }
     const dim3 grid((unsigned)(g->nz * A.split)), block(AJ_THREADS);
@@ -470,7 +500,7 @@ extern "C" int pbh_adjoint_tuned(const PbhGrid* g, const PbhLattice* lat, const 
         else hipLaunchKernelGGL((adj_fp32_kernel<false, false>), grid, block, 0, s, A);
     }
     if ((rc = launch_rc("adj_fp32_kernel")) != PBH_OK) return rc;
-    hipLaunchKernelGGL(adj_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL(adj_reduce_kernel, dim3((unsigned)((n + RED_ELEMS - 1) / RED_ELEMS)), dim3(256), 0, s,
                        (const float*)A.part, g->nz, n, grad_w);
     return launch_rc("adj_reduce_kernel");
 }

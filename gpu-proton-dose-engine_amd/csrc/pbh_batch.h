@@ -39,6 +39,7 @@ struct BatchRedItem {
 };
 
 constexpr int OBJ_BATCH_BLOCKS = 256;   // objective workgroups per item
+constexpr int RED_ELEMS = 64;           // dJ/dw elements per depth-reduction workgroup
 
 }  // namespace pbh
 

@@ -24,7 +24,6 @@ extern "C" int pbh_validate(const PbhGrid* g, const PbhLattice* lat);
 
 namespace {
 
-constexpr int RED_THREADS = 256;
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -347,7 +346,7 @@ extern "C" int pbh_batch_create(const PbhBatchItem* items, int n, int precision,
         long long acc = 0;
         for (int q = 0; q < b->n_obj; ++q) {
             red_wg0[q] = (int)acc;
-            acc += (long long)cdiv((int)std::min<size_t>(hr[q].n, 0x7fffffff), RED_THREADS);
+            acc += (long long)cdiv((int)std::min<size_t>(hr[q].n, 0x7fffffff), RED_ELEMS);
         }
         red_wg0[b->n_obj] = (int)acc;
         if (acc > 0x7fffffffLL) {

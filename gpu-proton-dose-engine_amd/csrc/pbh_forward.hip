@@ -354,8 +354,9 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 // dispatch of the workgroups in launch order (chunk-major, slices in order)
 // onto `slots` resident workgroup slots, each taking the next workgroup when it
 // frees, with a workgroup costing its layers plus FT_WG_COST layers of fixed
-// work (empty later chunks exit at once), and returns the chunk with the
-// shortest simulated makespan (the larger chunk within 2 %: fewer partials).
+// work (empty later chunks exit at once), and returns the chunk (at least
+// FT_MIN_CHUNK) with the shortest simulated makespan (the larger chunk within
+// 2 %: fewer partials).
 // On the 128^3 field this picks 17 (bf16x3, 3 slots per CU) and 26 (FP32, 2
 // slots per CU); a measured sweep had 18-24 and 26 fastest, 4 % and 14 % ahead
 // of two chunks per tile.
@@ -369,7 +370,7 @@ static int model_chunk(const FwdArgs& a, const int32_t* active, long long slots)
     int best = a.L;
     double best_t = 0.0;
     std::vector<double> heap;
-    for (int c = a.L; c >= 1; --c) {
+    for (int c = a.L; c >= std::min(a.L, FT_MIN_CHUNK); --c) {
         const int nch = (a.L + c - 1) / c;
         if ((long long)nch * per_z * a.nz > (1LL << 24)) break;     // keep the model cheap
         heap.assign((size_t)slots, 0.0);                              // min-heap of slot finish times

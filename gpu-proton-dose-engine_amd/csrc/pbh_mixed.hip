@@ -209,14 +209,14 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
 #pragma unroll
                     for (int q = 0; q < NV; ++q) {
                         const int v = threadIdx.x + q * FT_THREADS;
-                        if (v < NWV) wreg[q] = src[v];
+                        if (NWV % FT_THREADS == 0 || v < NWV) wreg[q] = src[v];
                     }
                 };
                 auto store_w = [&](int b) {
 #pragma unroll
                     for (int q = 0; q < NV; ++q) {
                         const int v = threadIdx.x + q * FT_THREADS;
-                        if (v < NWV) {
+                        if (NWV % FT_THREADS == 0 || v < NWV) {
                             const int e8 = v % WV;
                             T* dst = (v < WV ? Wh[b] : Wl[b]) + (e8 >> 3) * WT_STRIDE + 8 * (e8 & 7);
                             *(u4v*)dst = wreg[q];

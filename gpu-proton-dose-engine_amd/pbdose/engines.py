@@ -24,6 +24,9 @@ INV_SQRT_2PI = 1.0 / np.sqrt(2.0 * np.pi)
 PRECISIONS = tuple(_lib.PREC)
 
 
+_CHUNK_CACHE: dict = {}
+
+
 class DeviceProblem:
     """A PBProblem resident on one GPU: weights (L, nsy, nsx) FP32, idd and
     sigma (L, nz) FP64, plus the C-ABI descriptors and cached workspaces."""
@@ -48,16 +51,32 @@ class DeviceProblem:
             raise ValueError(f"idd/sigma must be (L, nz) = {(L, g.nz)}")
         _lib.check(_lib.lib().pbh_validate(self.grid_s, self.lat_s), "pbh_validate")
         self._ws = {}
-        # active layers per depth slice (one small copy to the host), for the layer chunk
-        self._active = (self.idd != 0).sum(dim=0).to(torch.int32).cpu().contiguous()
+        # active layers per depth slice, for the layer chunk: from the host tables when
+        # the problem carries them, else one small copy from the device
+        if idd is None and isinstance(problem.idd, np.ndarray):
+            act = (problem.idd != 0).sum(axis=0)
+        else:
+            act = (self.idd != 0).sum(dim=0).cpu().numpy()
+        self._active = np.ascontiguousarray(act, dtype=np.int32)
         self._chunk = {}
 
     def auto_chunk(self, precision: int) -> int:
-        """Layer chunk of the automatic forward for this problem's active layers."""
+        """Layer chunk of the automatic forward for this problem's active layers
+        (pbh_forward_choose_chunk; results are cached per problem shape, so that
+        per-call wrappers such as torch_separable pay for the model once)."""
         if precision not in self._chunk:
-            c = _lib.lib().pbh_forward_choose_chunk(self.grid_s, self.lat_s, precision,
-                                                    self._active.data_ptr(), 0)
-            _lib.check(min(c, 0), "pbh_forward_choose_chunk")
+            g, lat = self.problem.grid, self.problem.lattice
+            key = (g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, g.ox, g.oy, g.oz, lat.n_sx, lat.n_sy,
+                   lat.n_layers, lat.dx_spot, lat.dy_spot, lat.x0, lat.y0, self.dev.index,
+                   precision, self._active.tobytes())
+            c = _CHUNK_CACHE.get(key)
+            if c is None:
+                c = _lib.lib().pbh_forward_choose_chunk(self.grid_s, self.lat_s, precision,
+                                                        self._active.ctypes.data, 0)
+                _lib.check(min(c, 0), "pbh_forward_choose_chunk")
+                if len(_CHUNK_CACHE) > 4096:
+                    _CHUNK_CACHE.clear()
+                _CHUNK_CACHE[key] = c
             self._chunk[precision] = int(c)
         return self._chunk[precision]
 
