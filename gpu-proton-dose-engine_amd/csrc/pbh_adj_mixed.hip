@@ -6,9 +6,9 @@
 //   P  += Gy_lz^T Q          (48 x 48: spots j, i; K = 64 voxels y)
 //   dJ/dW_l = sum_z c_lz P_lz,  c_lz = IDD_l(z) / (2 pi s^2)
 // Workgroup (z, c) of the `split` workgroups of depth slice z takes the active
-// layers (IDD != 0) of the slice whose active index is c modulo split, and
-// zeroes the partials of the inactive layers l with l % split == c: the
-// workgroups of a slice share its active layers evenly (at most
+// layers (IDD != 0) of the slice whose active index is c modulo split (the
+// partials of inactive layers are zero and never written: the depth reduction
+// skips them): the workgroups of a slice share its active layers evenly (at most
 // ceil(active / split) each), and those left without one skip the slice's R
 // tiles entirely. The slice is processed in groups of two 64 x 128 tiles
 // (y): the wave's 16 rows of R of both are loaded once into registers and split
@@ -96,12 +96,8 @@ __device__ __forceinline__ void adj_mixed_body(const AdjArgs& A, int blk)
         return table_slot(gy, s2.bE + 16 * gy.Q * mb - 32 * gy.P * ks - gy.P * e);
     };
 
-    // zero the partials of the inactive layers l % split == s (uniform test)
-    for (int l = s; l < a.L; l += A.split) {
-        if (a.idd[(size_t)l * a.nz + z] != 0.0) continue;
-        float* pz = A.part + ((size_t)z * a.L + l) * a.nsy * a.nsx;
-        for (int e = threadIdx.x; e < a.nsy * a.nsx; e += FT_THREADS) pz[e] = 0.0f;
-    }
+    // Inactive layers (IDD = 0 at z) have a zero partial, which the depth reduction skips:
+    // nothing is written for them.
     // active layers of the slice (stable order, ballots over passes of 256); keep
     // those with active index = s (mod split), at most ceil(L / split) <= AM_MAXK
     int nmine = 0, base = 0;

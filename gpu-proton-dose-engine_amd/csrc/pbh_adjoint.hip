@@ -81,16 +81,9 @@ __device__ __forceinline__ void adj_fp32_body(const AdjArgs& A, int blk)
                         for (int q = 0; q < nl; ++q) {
                             const int l = lb + q * A.split;
                             const double iddv = a.idd[(size_t)l * a.nz + z];
-                            if (iddv == 0.0 && !lut_grads) {
-                                // no dose from this layer at this depth: its dJ/dw partial is zero
-                                // (uniform in the workgroup: one address for all threads)
-                                if (first_tile) {
-                                    float* pz = A.part + ((size_t)z * a.L + l) * a.nsy * a.nsx;
-                                    for (int e = threadIdx.x; e < ni * nj; e += AJ_THREADS)
-                                        pz[(size_t)(j0 + e / ni) * a.nsx + i0 + e % ni] = 0.0f;
-                                }
-                                continue;
-                            }
+                            // no dose from this layer at this depth: its dJ/dw partial is zero
+                            // and the depth reduction skips it (uniform in the workgroup)
+                            if (iddv == 0.0 && !lut_grads) continue;
                             const double sg = a.sigma[(size_t)l * a.nz + z];
                             const double sd = sg > kSigmaFloor ? sg : kSigmaFloor;
                             const double k2d = kLog2e / (2.0 * sd * sd);
@@ -265,44 +258,56 @@ void adj_fp32_batch_kernel(const AdjArgs* __restrict__ items, const int* __restr
 }
 
 // grad_w[e] = sum_z part[z][e] in FP64, increasing z (8 loads in flight per thread)
-// grad_w[e] = sum_z part[z][e] in FP64. Workgroup blk takes elements 64 blk ..
-// 64 blk + 63; wave w sums the slices z = w, w + 4, ... of one element per lane
-// (coalesced rows), and the four wave sums are added in wave order, so the
-// result does not depend on scheduling.
-
-__device__ __forceinline__ void adj_reduce_body(const float* part, int nz, size_t n, float* grad_w, int blk)
+// grad_w[e] = sum_z part[z][e] in FP64 over the slices where element e's layer
+// has IDD != 0: elsewhere its c P partial is exactly zero, so the adjoint kernels
+// neither write nor zero it. Workgroup blk takes elements 64 blk .. 64 blk + 63;
+// wave w sums the slices z = w, w + 4, ... of one element per lane (coalesced
+// rows), and the four wave sums are added in wave order, so the result does not
+// depend on scheduling.
+__device__ __forceinline__ void adj_reduce_body(const BatchRedItem& it, int blk)
 {
     __shared__ double ws[4][RED_ELEMS];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const size_t e = (size_t)blk * RED_ELEMS + lane;
     double acc = 0.0;
-    if (e < n) {
+    if (e < it.n) {
+        // Every slice's entry is loaded (no divergent loads); entries of inactive slices,
+        // which nothing wrote, are replaced by zero with a select, so that whatever the
+        // workspace held there cannot reach the sum.
+        const double* idd = it.idd + (e / (size_t)it.per_layer) * it.nz;
         int z = wave;
-        for (; z + 12 < nz; z += 16) {
+        for (; z + 12 < it.nz; z += 16) {
             float v[4];
+            bool on[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) v[k] = part[(size_t)(z + 4 * k) * n + e];
+            for (int k = 0; k < 4; ++k) {
+                v[k] = it.part[(size_t)(z + 4 * k) * it.n + e];
+                on[k] = idd[z + 4 * k] != 0.0;
+            }
 #pragma unroll
-            for (int k = 0; k < 4; ++k) acc += (double)v[k];
+            for (int k = 0; k < 4; ++k) acc += on[k] ? (double)v[k] : 0.0;
         }
-        for (; z < nz; z += 4) acc += (double)part[(size_t)z * n + e];
+        for (; z < it.nz; z += 4) {
+            const float v = it.part[(size_t)z * it.n + e];
+            acc += idd[z] != 0.0 ? (double)v : 0.0;
+        }
     }
     ws[wave][lane] = acc;
     __syncthreads();
-    if (wave == 0 && e < n) grad_w[e] = (float)(((ws[0][lane] + ws[1][lane]) + ws[2][lane]) + ws[3][lane]);
+    if (wave == 0 && e < it.n)
+        it.grad_w[e] = (float)(((ws[0][lane] + ws[1][lane]) + ws[2][lane]) + ws[3][lane]);
 }
 
-__global__ __launch_bounds__(256) void adj_reduce_kernel(const float* part, int nz, size_t n, float* grad_w)
+__global__ __launch_bounds__(256) void adj_reduce_kernel(BatchRedItem it)
 {
-    adj_reduce_body(part, nz, n, grad_w, blockIdx.x);
+    adj_reduce_body(it, blockIdx.x);
 }
 
 __global__ __launch_bounds__(256) void adj_reduce_batch_kernel(const BatchRedItem* __restrict__ items,
                                                                const int* __restrict__ wg0, int n)
 {
     const int k = batch_item(wg0, n, blockIdx.x);
-    const BatchRedItem it = items[k];
-    adj_reduce_body(it.part, it.nz, it.n, it.grad_w, (int)blockIdx.x - wg0[k]);
+    adj_reduce_body(items[k], (int)blockIdx.x - wg0[k]);
 }
 
 // ---- weighted quadratic objective: R = 2 wv (D - Dp), J = sum wv (D - Dp)^2 ----
@@ -487,7 +492,7 @@ extern "C" int pbh_adjoint_tuned(const PbhGrid* g, const PbhLattice* lat, const 
         if (!A.tab) return PBH_ERR_UNSUPPORTED;   // the BF16 adjoint gathers every factor from tables
         if ((rc = pbh_launch_adj_mixed(A, precision, g->nz, s)) != PBH_OK) return rc;
         hipLaunchKernelGGL(adj_reduce_kernel, dim3((unsigned)((n + RED_ELEMS - 1) / RED_ELEMS)), dim3(256), 0,
-                           s, (const float*)A.part, g->nz, n, grad_w);
+                           s, BatchRedItem{A.part, idd, grad_w, n, g->nz, lat->n_sy * lat->n_sx});
         return launch_rc("adj_reduce_kernel");
     }
     const dim3 grid((unsigned)(g->nz * A.split)), block(AJ_THREADS);
@@ -501,7 +506,7 @@ extern "C" int pbh_adjoint_tuned(const PbhGrid* g, const PbhLattice* lat, const 
     }
     if ((rc = launch_rc("adj_fp32_kernel")) != PBH_OK) return rc;
     hipLaunchKernelGGL(adj_reduce_kernel, dim3((unsigned)((n + RED_ELEMS - 1) / RED_ELEMS)), dim3(256), 0, s,
-                       (const float*)A.part, g->nz, n, grad_w);
+                       BatchRedItem{A.part, idd, grad_w, n, g->nz, lat->n_sy * lat->n_sx});
     return launch_rc("adj_reduce_kernel");
 }
 

@@ -32,10 +32,12 @@ struct BatchObjItem {
 
 // Depth reduction of one problem's per-slice weight-gradient partials.
 struct BatchRedItem {
-    const float* part;     // [nz][n]
+    const float* part;     // [nz][n]: slice z, layer l's block written only where IDD_l(z) != 0
+    const double* idd;     // [L][nz]
     float* grad_w;         // [n]
     size_t n;
     int nz;
+    int per_layer;         // nsy * nsx
 };
 
 constexpr int OBJ_BATCH_BLOCKS = 256;   // objective workgroups per item

@@ -313,7 +313,8 @@ extern "C" int pbh_batch_create(const PbhBatchItem* items, int n, int precision,
         atabs.push_back((char)(mix ? (3 | (pbh_adj_mixed_q5(A) ? 4 : 0)) : (A.tab ? 1 : 0)));
         aw.push_back(it.grid.nz * A.split);
         const size_t ne = (size_t)it.lattice.n_layers * it.lattice.n_sy * it.lattice.n_sx;
-        hr.push_back(BatchRedItem{part, it.grad_w, ne, it.grid.nz});
+        hr.push_back(BatchRedItem{part, it.idd, it.grad_w, ne, it.grid.nz,
+                                  it.lattice.n_sy * it.lattice.n_sx});
     }
     b->n_obj = (int)ho.size();
     std::vector<int> adj_wg0, red_wg0;
