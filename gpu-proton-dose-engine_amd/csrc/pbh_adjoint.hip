@@ -134,15 +134,33 @@ __device__ __forceinline__ void adj_fp32_body(const AdjArgs& A, int blk)
                                 return gv;
                             };
 
+                            // Windows: K steps (4 voxels each) within FT_WIN_SIGMAS s of a spot
+                            // block; a factor beyond is below 2^-32 of its peak. Returns the steps
+                            // [lo, hi) of voxels 0 .. T-1 in reach of block spots b0 .. b0 + 15 of
+                            // n, for voxel v at C0 + v dv and spot k at k ds (uniform).
+                            const float cut = (float)(FT_WIN_SIGMAS * sd);
+                            auto steps = [&](int b0, int n, double C0, double dv, double ds, int T, int& lo) {
+                                lo = 0;
+                                if (b0 >= n) return 0;
+                                const float s0 = (float)(((double)b0 * ds - C0) / dv);
+                                const float s1 = (float)(((double)min(b0 + 15, n - 1) * ds - C0) / dv);
+                                int first, last;
+                                spot_range(s0, s1, cut * (float)(1.0 / dv), T, first, last);
+                                if (first > last) return 0;
+                                lo = __builtin_amdgcn_readfirstlane(first >> 2);
+                                return __builtin_amdgcn_readfirstlane((last >> 2) + 1);
+                            };
                             // stage 1: Q[y][i] = sum_x R[y][x] Gx[x][i]; wave w owns rows 16w..16w+15
                             {
                                 v4f q1[3] = {}, q2[3] = {};
-#pragma unroll 4
-                                for (int kk = 0; kk < AJ_TX / 4; ++kk) {
-                                    const int xx = 4 * kk + g;
-                                    const float av = Rs[(16 * wave + c16) * AJ_RS + xx];
 #pragma unroll
-                                    for (int nb = 0; nb < 3; ++nb) {
+                                for (int nb = 0; nb < 3; ++nb) {
+                                    int klo;
+                                    const int khi = steps(16 * nb, ni, Cx, a.dx, a.dsx, AJ_TX, klo);
+#pragma unroll 4
+                                    for (int kk = klo; kk < khi; ++kk) {
+                                        const int xx = 4 * kk + g;
+                                        const float av = Rs[(16 * wave + c16) * AJ_RS + xx];
                                         float dg = 0.0f;
                                         const float bv = gx(xx, 16 * nb + c16, &dg);
                                         q1[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, q1[nb], 0, 0, 0);
@@ -171,8 +189,10 @@ __device__ __forceinline__ void adj_fp32_body(const AdjArgs& A, int blk)
                                 if (blk < 9) {
                                     const int mb = blk / 3, nb = blk % 3;
                                     v4f p = {}, p2 = {};
+                                    int klo;
+                                    const int khi = steps(16 * mb, nj, Cy, a.dy, a.dsy, AJ_TY, klo);
 #pragma unroll 4
-                                    for (int kk = 0; kk < AJ_TY / 4; ++kk) {
+                                    for (int kk = klo; kk < khi; ++kk) {
                                         const int yy = 4 * kk + g;
                                         float dgy = 0.0f;
                                         const float av = gy(yy, 16 * mb + c16, &dgy);
