@@ -93,17 +93,18 @@ __device__ __forceinline__ void adj_fp32_body(const AdjArgs& A, int blk)
                             if (TAB) {
                                 const int ntX = A.tx.P * (AJ_TX - 1) + A.tx.Q * (AJ_SC - 1) + 1;
                                 const int ntY = A.ty.P * (AJ_TY - 1) + A.ty.Q * (AJ_SC - 1) + 1;
+                                const TableRow rx = table_row(A.tx, Cx, AJ_TX), ry = table_row(A.ty, Cy, AJ_TY);
                                 for (int t = threadIdx.x; t < ntX; t += AJ_THREADS) {
-                                    const double d = table_distance(A.tx, Cx, AJ_TX, t);
-                                    const float gv = __builtin_amdgcn_exp2f((float)(-(d * d) * k2d));
+                                    const float d = table_distance_f(rx, t);
+                                    const float gv = __builtin_amdgcn_exp2f(-(d * d) * k2);
                                     Tx[table_slot(A.tx, t)] = gv;
-                                    if (SIG) Tx2[table_slot(A.tx, t)] = gv * (float)(d * d) * is3;
+                                    if (SIG) Tx2[table_slot(A.tx, t)] = gv * (d * d) * is3;
                                 }
                                 for (int t = threadIdx.x; t < ntY; t += AJ_THREADS) {
-                                    const double d = table_distance(A.ty, Cy, AJ_TY, t);
-                                    const float gv = __builtin_amdgcn_exp2f((float)(-(d * d) * k2d));
+                                    const float d = table_distance_f(ry, t);
+                                    const float gv = __builtin_amdgcn_exp2f(-(d * d) * k2);
                                     Ty[table_slot(A.ty, t)] = gv;
-                                    if (SIG) Ty2[table_slot(A.ty, t)] = gv * (float)(d * d) * is3;
+                                    if (SIG) Ty2[table_slot(A.ty, t)] = gv * (d * d) * is3;
                                 }
                             }
                             __syncthreads();
