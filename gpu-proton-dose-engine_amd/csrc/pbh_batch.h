@@ -53,7 +53,8 @@ int pbh_make_adj_args(const PbhGrid* g, const PbhLattice* lat, const float* w, c
                       double* grad_sigma, int split, pbh::AdjArgs* out);
 int pbh_adj_default_split(const PbhGrid* g, const PbhLattice* lat);
 int pbh_launch_fwd_fp32_batch(const pbh::BatchFwdItem* items, const int* wg0, int n, int total, int scp,
-                              bool tab, hipStream_t s);
+                              bool tab, bool q5, hipStream_t s);
+bool fp32_q5(const pbh::FwdPlan& pl);
 int pbh_launch_fwd_mixed_batch(const pbh::BatchFwdItem* items, const int* wg0, int n, int total,
                                int precision, int kb, bool tab, bool q5, hipStream_t s);
 int pbh_launch_wsplit_batch(const pbh::BatchFwdItem* items, int n, int max_chunks, int precision, hipStream_t s);

@@ -240,7 +240,7 @@ extern "C" int pbh_batch_create(const PbhBatchItem* items, int n, int precision,
         f.r_peak = is_peak(precision) ? pbh_forward_ws_peaks(f.a, ws + L.fwd_ws[k]) : nullptr;
         hp[p] = BatchPeakItem{it.idd, it.lattice.n_layers, it.grid.nz, it.grid.oz, it.grid.dz, (float*)f.r_peak};
         keys[p] = key[k];
-        tabs[p] = (char)(mixed ? pbh_mixed_slot_mode(f.pl) : f.pl.table);
+        tabs[p] = (char)(mixed ? pbh_mixed_slot_mode(f.pl) : ((f.pl.table ? 1 : 0) | (fp32_q5(f.pl) ? 2 : 0)));
         const long long w = (long long)f.a.tiles_x * f.a.tiles_y * f.a.nz * f.pl.nchunk;
         if (w > 0x3fffffffLL) return PBH_ERR_UNSUPPORTED;
         wgs[p] = (int)w;
@@ -400,7 +400,7 @@ extern "C" int pbh_batch_forward_adjoint(const PbhBatch* b, void* stream)
     for (const Launch& g : b->fwd) {
         if (b->precision == PBH_PREC_FP32)
             rc = pbh_launch_fwd_fp32_batch(b->d_fwd + g.first, b->d_fwd_wg0 + g.wg0_off, g.count, g.total, g.key,
-                                           g.mode & 1, s);
+                                           g.mode & 1, (g.mode & 2) != 0, s);
         else
             rc = pbh_launch_fwd_mixed_batch(b->d_fwd + g.first, b->d_fwd_wg0 + g.wg0_off, g.count, g.total,
                                             b->precision, g.key, g.mode & 1, (g.mode & 2) != 0, s);

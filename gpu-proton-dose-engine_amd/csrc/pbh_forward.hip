@@ -43,7 +43,7 @@ namespace pbh {
 // Stage B: wave w owns the 32 x 32 dose block at columns 32 w.
 //
 // blk: the workgroup's index in the problem's own grid (chunk * tiles + tile).
-template <bool TAB, int SCP>
+template <bool TAB, int SCP, int QC>
 __device__ __forceinline__ void fwd_fp32_body(const FwdArgs& a, const FwdPlan& pl, int blk)
 {
     constexpr int KA = SCP / 4;            // stage A K steps
@@ -83,6 +83,11 @@ __device__ __forceinline__ void fwd_fp32_body(const FwdArgs& a, const FwdPlan& p
     const int tB0 = pl.tx.Q * (lane >> 5) - pl.tx.P * aB + pl.tx.P * (FT_TX - 1);
     const int ntY = pl.ty.P * (FT_TY - 1) + pl.ty.Q * (SCP - 1) + 1;
     const int ntX = pl.tx.P * (FT_TX - 1) + pl.tx.Q * (SCP - 1) + 1;
+    // QC > 0 (P = 2, Q = QC on both axes): an operand's entry t0 + 2 Q k keeps the parity
+    // of t0, so its parity-interleaved slot is slot(t0) + Q k: one base register plus
+    // compile-time offsets per gather
+    const int sA0 = TAB && QC ? table_slot(pl.ty, tA0) : 0;
+    const int sB0 = TAB && QC ? table_slot(pl.tx, tB0) : 0;
 
     const int ncx = chunk_count(a.nsx), ncy = chunk_count(a.nsy);
 
@@ -146,7 +151,7 @@ __device__ __forceinline__ void fwd_fp32_body(const FwdArgs& a, const FwdPlan& p
 #pragma unroll
                     for (int kk = 0; kk < KA; ++kk) {
                         if (TAB) {
-                            g[kk] = Ty[b][table_slot(pl.ty, tA0 + 4 * pl.ty.Q * kk)];
+                            g[kk] = QC ? Ty[b][sA0 + 2 * QC * kk] : Ty[b][table_slot(pl.ty, tA0 + 4 * pl.ty.Q * kk)];
                         } else {
                             const int j = 4 * kk + (lane >> 4);
                             const double d = yAc - (a.sy0 + (double)(j0 + j) * a.dsy);
@@ -177,7 +182,7 @@ __device__ __forceinline__ void fwd_fp32_body(const FwdArgs& a, const FwdPlan& p
 #pragma unroll
                     for (int kk = 0; kk < KB; ++kk) {
                         if (TAB) {
-                            gv[kk] = Tx[b][table_slot(pl.tx, tB0 + 2 * pl.tx.Q * kk)];
+                            gv[kk] = QC ? Tx[b][sB0 + QC * kk] : Tx[b][table_slot(pl.tx, tB0 + 2 * pl.tx.Q * kk)];
                         } else {
                             const int i = 2 * kk + (lane >> 5);
                             const double d = xBc - (a.sx0 + (double)(i0 + i) * a.dsx);
@@ -225,22 +230,22 @@ __device__ __forceinline__ void fwd_fp32_body(const FwdArgs& a, const FwdPlan& p
     store_block(a, pl, tile, ch, z, y0, x0, 0, cnB, acc, (size_t)tiles);
 }
 
-template <bool TAB, int SCP>
+template <bool TAB, int SCP, int QC>
 __global__ __launch_bounds__(FT_THREADS, 2)
 void fwd_fp32_kernel(FwdArgs a, FwdPlan pl)
 {
-    fwd_fp32_body<TAB, SCP>(a, pl, blockIdx.x);
+    fwd_fp32_body<TAB, SCP, QC>(a, pl, blockIdx.x);
 }
 
 // One launch over a batch of problems sharing the template parameters.
-template <bool TAB, int SCP>
+template <bool TAB, int SCP, int QC>
 __global__ __launch_bounds__(FT_THREADS, 2)
 void fwd_fp32_batch_kernel(const BatchFwdItem* __restrict__ items, const int* __restrict__ wg0, int n)
 {
     const int k = batch_item(wg0, n, blockIdx.x);
     const FwdArgs a = items[k].a;
     const FwdPlan pl = items[k].pl;
-    fwd_fp32_body<TAB, SCP>(a, pl, (int)blockIdx.x - wg0[k]);
+    fwd_fp32_body<TAB, SCP, QC>(a, pl, (int)blockIdx.x - wg0[k]);
 }
 
 __global__ __launch_bounds__(256) void combine_kernel(FwdArgs a, FwdPlan pl)
@@ -472,6 +477,13 @@ extern "C" size_t pbh_forward_workspace_bytes_tuned(const PbhGrid* g, const PbhL
     return pbh_forward_ws_bytes(a, layer_chunk > 0 && layer_chunk < a.L ? layer_chunk : (layer_chunk ? a.L : 0));
 }
 
+// Table geometry with P = 2, Q = 5 on both axes (2 mm voxels, 5 mm spots): the FP32
+// kernel's compile-time-slot instantiation.
+bool fp32_q5(const FwdPlan& pl)
+{
+    return pl.table && pl.tx.P == 2 && pl.tx.Q == 5 && pl.ty.P == 2 && pl.ty.Q == 5;
+}
+
 extern "C" int pbh_forward_choose_chunk(const PbhGrid* g, const PbhLattice* lat, int precision,
                                         const int32_t* active_per_z, int compute_units)
 {
@@ -519,11 +531,13 @@ extern "C" int pbh_forward_tuned(const PbhGrid* g, const PbhLattice* lat, const 
                           layer_chunk, &pl);
     if (rc != PBH_OK) return rc;
     const dim3 grid((unsigned)(tile_count(a) * pl.nchunk));
+    const bool q5 = fp32_q5(pl);
     switch (chunk_pad(a.nsx, a.nsy)) {
 #define PBH_FP32_CASE(N) \
     case N: \
-        if (pl.table) hipLaunchKernelGGL((fwd_fp32_kernel<true, N>), grid, dim3(FT_THREADS), 0, s, a, pl); \
-        else hipLaunchKernelGGL((fwd_fp32_kernel<false, N>), grid, dim3(FT_THREADS), 0, s, a, pl); \
+        if (q5) hipLaunchKernelGGL((fwd_fp32_kernel<true, N, 5>), grid, dim3(FT_THREADS), 0, s, a, pl); \
+        else if (pl.table) hipLaunchKernelGGL((fwd_fp32_kernel<true, N, 0>), grid, dim3(FT_THREADS), 0, s, a, pl); \
+        else hipLaunchKernelGGL((fwd_fp32_kernel<false, N, 0>), grid, dim3(FT_THREADS), 0, s, a, pl); \
         break;
     PBH_FP32_CASE(4) PBH_FP32_CASE(8) PBH_FP32_CASE(12) PBH_FP32_CASE(16) PBH_FP32_CASE(20)
     PBH_FP32_CASE(24) PBH_FP32_CASE(28) PBH_FP32_CASE(32) PBH_FP32_CASE(36) PBH_FP32_CASE(40)
@@ -546,14 +560,15 @@ extern "C" int pbh_forward(const PbhGrid* g, const PbhLattice* lat, const float*
 // Batched FP32 forward: items[0..n) share SCP (their padded chunk width) and the
 // table mode; total = wg0[n].
 int pbh_launch_fwd_fp32_batch(const BatchFwdItem* items, const int* wg0, int n, int total, int scp,
-                              bool tab, hipStream_t s)
+                              bool tab, bool q5, hipStream_t s)
 {
     const dim3 grid((unsigned)total), block(FT_THREADS);
     switch (scp) {
 #define PBH_FP32_BATCH_CASE(N) \
     case N: \
-        if (tab) hipLaunchKernelGGL((fwd_fp32_batch_kernel<true, N>), grid, block, 0, s, items, wg0, n); \
-        else hipLaunchKernelGGL((fwd_fp32_batch_kernel<false, N>), grid, block, 0, s, items, wg0, n); \
+        if (q5) hipLaunchKernelGGL((fwd_fp32_batch_kernel<true, N, 5>), grid, block, 0, s, items, wg0, n); \
+        else if (tab) hipLaunchKernelGGL((fwd_fp32_batch_kernel<true, N, 0>), grid, block, 0, s, items, wg0, n); \
+        else hipLaunchKernelGGL((fwd_fp32_batch_kernel<false, N, 0>), grid, block, 0, s, items, wg0, n); \
         break;
     PBH_FP32_BATCH_CASE(16) PBH_FP32_BATCH_CASE(24) PBH_FP32_BATCH_CASE(32)
     PBH_FP32_BATCH_CASE(40) PBH_FP32_BATCH_CASE(48)
