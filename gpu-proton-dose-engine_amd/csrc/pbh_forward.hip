@@ -339,11 +339,12 @@ static long long tile_count(const FwdArgs& a) { return (long long)a.tiles_x * a.
 
 constexpr int FT_MIN_CHUNK = 4;   // smallest automatic layer chunk
 
-// Layer chunk for about three workgroups per two resident slots (two per CU):
-// the chunks of shallow slices (many active layers) then spread over more
-// workgroups than those of deep slices, and the dispatcher balances them,
-// while few partial tiles have to be combined (measured on the 128^3 field:
-// two chunks per tile beat one and four).
+// Layer chunk when the active layers per slice are not known (pbh_forward reads
+// none of its device tables on the host): about three workgroups per two
+// resident slots (two per CU), so that the chunks of shallow slices (many
+// active layers) spread over more workgroups than those of deep slices while
+// few partial tiles have to be combined. pbh_forward_choose_chunk, given the
+// active-layer counts, replaces this rule.
 int pbh_auto_chunk(const FwdArgs& a)
 {
     const long long slots = 2LL * device_cus();
