@@ -281,42 +281,62 @@ void adj_fp32_batch_kernel(const AdjArgs* __restrict__ items, const int* __restr
 // grad_w[e] = sum_z part[z][e] in FP64, increasing z (8 loads in flight per thread)
 // grad_w[e] = sum_z part[z][e] in FP64 over the slices where element e's layer
 // has IDD != 0: elsewhere its c P partial is exactly zero, so the adjoint kernels
-// neither write nor zero it. Workgroup blk takes elements 64 blk .. 64 blk + 63;
-// wave w sums the slices z = w, w + 4, ... of one element per lane (coalesced
-// rows), and the four wave sums are added in wave order, so the result does not
-// depend on scheduling.
+// neither write nor read it. Workgroup blk takes 64 elements of one layer l
+// (blocks of a layer are consecutive). Per pass of 256 slices the workgroup
+// lists the active ones in increasing z (ballot compaction), and wave w adds the
+// partials of list entries w, w + 4, ... (coalesced rows, four loads in flight);
+// the four wave sums are then added in wave order, so the result does not depend
+// on scheduling.
 __device__ __forceinline__ void adj_reduce_body(const BatchRedItem& it, int blk)
 {
     __shared__ double ws[4][RED_ELEMS];
+    __shared__ int zl[256];
+    __shared__ int wcnt[4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const size_t e = (size_t)blk * RED_ELEMS + lane;
+    const int bpl = (it.per_layer + RED_ELEMS - 1) / RED_ELEMS;
+    const int l = blk / bpl;
+    const int el = (blk - l * bpl) * RED_ELEMS + lane;
+    const bool own = el < it.per_layer;
+    const size_t e = (size_t)l * it.per_layer + el;
+    const double* idd = it.idd + (size_t)l * it.nz;
     double acc = 0.0;
-    if (e < it.n) {
-        // Every slice's entry is loaded (no divergent loads); entries of inactive slices,
-        // which nothing wrote, are replaced by zero with a select, so that whatever the
-        // workspace held there cannot reach the sum.
-        const double* idd = it.idd + (e / (size_t)it.per_layer) * it.nz;
-        int z = wave;
-        for (; z + 12 < it.nz; z += 16) {
-            float v[4];
-            bool on[4];
+    for (int z0 = 0; z0 < it.nz; z0 += 256) {
+        const int z = z0 + (int)threadIdx.x;
+        const bool act = z < it.nz && idd[z] != 0.0;
+        const unsigned long long m = __ballot(act);
+        if (lane == 0) wcnt[wave] = __popcll(m);
+        __syncthreads();
+        int off = 0, cnt = 0;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                v[k] = it.part[(size_t)(z + 4 * k) * it.n + e];
-                on[k] = idd[z + 4 * k] != 0.0;
+        for (int w = 0; w < 4; ++w) {
+            off += w < wave ? wcnt[w] : 0;
+            cnt += wcnt[w];
+        }
+        if (act) zl[off + __popcll(m & ((1ull << lane) - 1ull))] = z;
+        __syncthreads();
+        if (own) {
+            int q = wave;
+            for (; q + 12 < cnt; q += 16) {
+                float v[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) v[k] = it.part[(size_t)zl[q + 4 * k] * it.n + e];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) acc += (double)v[k];
             }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) acc += on[k] ? (double)v[k] : 0.0;
+            for (; q < cnt; q += 4) acc += (double)it.part[(size_t)zl[q] * it.n + e];
         }
-        for (; z < it.nz; z += 4) {
-            const float v = it.part[(size_t)z * it.n + e];
-            acc += idd[z] != 0.0 ? (double)v : 0.0;
-        }
+        __syncthreads();
     }
     ws[wave][lane] = acc;
     __syncthreads();
-    if (wave == 0 && e < it.n)
+    if (wave == 0 && own)
         it.grad_w[e] = (float)(((ws[0][lane] + ws[1][lane]) + ws[2][lane]) + ws[3][lane]);
+}
+
+// Workgroups of the depth reduction for one problem: whole 64-element blocks per layer.
+__host__ __device__ inline int reduce_blocks(size_t n, int per_layer)
+{
+    return (int)(n / (size_t)per_layer) * ((per_layer + RED_ELEMS - 1) / RED_ELEMS);
 }
 
 __global__ __launch_bounds__(256) void adj_reduce_kernel(BatchRedItem it)
@@ -512,7 +532,7 @@ extern "C" int pbh_adjoint_tuned(const PbhGrid* g, const PbhLattice* lat, const 
     if (mixed) {
         if (!A.tab) return PBH_ERR_UNSUPPORTED;   // the BF16 adjoint gathers every factor from tables
         if ((rc = pbh_launch_adj_mixed(A, precision, g->nz, s)) != PBH_OK) return rc;
-        hipLaunchKernelGGL(adj_reduce_kernel, dim3((unsigned)((n + RED_ELEMS - 1) / RED_ELEMS)), dim3(256), 0,
+        hipLaunchKernelGGL(adj_reduce_kernel, dim3((unsigned)reduce_blocks(n, lat->n_sy * lat->n_sx)), dim3(256), 0,
                            s, BatchRedItem{A.part, idd, grad_w, n, g->nz, lat->n_sy * lat->n_sx});
         return launch_rc("adj_reduce_kernel");
     }
@@ -526,7 +546,7 @@ extern "C" int pbh_adjoint_tuned(const PbhGrid* g, const PbhLattice* lat, const 
         else hipLaunchKernelGGL((adj_fp32_kernel<false, false>), grid, block, 0, s, A);
     }
     if ((rc = launch_rc("adj_fp32_kernel")) != PBH_OK) return rc;
-    hipLaunchKernelGGL(adj_reduce_kernel, dim3((unsigned)((n + RED_ELEMS - 1) / RED_ELEMS)), dim3(256), 0, s,
+    hipLaunchKernelGGL(adj_reduce_kernel, dim3((unsigned)reduce_blocks(n, lat->n_sy * lat->n_sx)), dim3(256), 0, s,
                        BatchRedItem{A.part, idd, grad_w, n, g->nz, lat->n_sy * lat->n_sx});
     return launch_rc("adj_reduce_kernel");
 }

@@ -347,7 +347,8 @@ extern "C" int pbh_batch_create(const PbhBatchItem* items, int n, int precision,
         long long acc = 0;
         for (int q = 0; q < b->n_obj; ++q) {
             red_wg0[q] = (int)acc;
-            acc += (long long)cdiv((int)std::min<size_t>(hr[q].n, 0x7fffffff), RED_ELEMS);
+            acc += (long long)(hr[q].n / (size_t)hr[q].per_layer) *
+                   ((hr[q].per_layer + RED_ELEMS - 1) / RED_ELEMS);
         }
         red_wg0[b->n_obj] = (int)acc;
         if (acc > 0x7fffffffLL) {
