@@ -34,7 +34,7 @@ __device__ __forceinline__ float tab_get(const float* T, const AxisTable& g, int
 constexpr int AJ_LB = 64;   // layers of one workgroup handled per pass over the slice tiles
 
 // blk: the workgroup's index in the problem's own grid (slice * split + share).
-template <bool TAB, bool SIG>
+template <bool TAB, bool SIG, int QC>
 __device__ __forceinline__ void adj_fp32_body(const AdjArgs& A, int blk)
 {
     const FwdArgs& a = A.f;
@@ -50,6 +50,9 @@ __device__ __forceinline__ void adj_fp32_body(const AdjArgs& A, int blk)
     const int s = blk % A.split;
     const int z = blk / A.split;
     const int g = lane >> 4, c16 = lane & 15;
+    // QC > 0 (P = 2, Q = QC on both axes): slots of the lane's k = 0, block-0 entries
+    const int sgx = TAB && QC ? table_slot(A.tx, A.tx.Q * c16 - A.tx.P * g + A.tx.P * (AJ_TX - 1)) : 0;
+    const int sgy = TAB && QC ? table_slot(A.ty, A.ty.Q * c16 - A.ty.P * g + A.ty.P * (AJ_TY - 1)) : 0;
     const int tiles_x = cdiv(a.nx, AJ_TX), tiles_y = cdiv(a.ny, AJ_TY);
     const int ncx = chunk_count(a.nsx), ncy = chunk_count(a.nsy);
     const float* Rz = A.R + (size_t)z * a.ny * a.nx;
@@ -111,6 +114,13 @@ __device__ __forceinline__ void adj_fp32_body(const AdjArgs& A, int blk)
 
                             // Gaussian of voxel column xx (tile) and spot i (chunk): table or direct
                             auto gx = [&](int xx, int i, float* dg) -> float {
+                                if (TAB && QC) {
+                                    // P = 2: with xx = 4 k + g and i = 16 nb + c16 the entry keeps
+                                    // the parity of its k = nb = 0 value; slot = base + 8 Q nb - 4 k
+                                    const int sl = sgx + (i - c16) / 2 * QC - (xx - g);
+                                    if (SIG) *dg = Tx2[sl];
+                                    return Tx[sl];
+                                }
                                 if (TAB) {
                                     const int t = A.tx.Q * i - A.tx.P * xx + A.tx.P * (AJ_TX - 1);
                                     if (SIG) *dg = tab_get(Tx2, A.tx, t);
@@ -123,6 +133,11 @@ __device__ __forceinline__ void adj_fp32_body(const AdjArgs& A, int blk)
                                 return gv;
                             };
                             auto gy = [&](int yy, int j, float* dg) -> float {
+                                if (TAB && QC) {
+                                    const int sl = sgy + (j - c16) / 2 * QC - (yy - g);
+                                    if (SIG) *dg = Ty2[sl];
+                                    return Ty[sl];
+                                }
                                 if (TAB) {
                                     const int t = A.ty.Q * j - A.ty.P * yy + A.ty.P * (AJ_TY - 1);
                                     if (SIG) *dg = tab_get(Ty2, A.ty, t);
@@ -261,21 +276,21 @@ __device__ __forceinline__ void adj_fp32_body(const AdjArgs& A, int blk)
     }
 }
 
-template <bool TAB, bool SIG>
+template <bool TAB, bool SIG, int QC>
 __global__ __launch_bounds__(AJ_THREADS, 2)
 void adj_fp32_kernel(AdjArgs A)
 {
-    adj_fp32_body<TAB, SIG>(A, blockIdx.x);
+    adj_fp32_body<TAB, SIG, QC>(A, blockIdx.x);
 }
 
 // One launch over a batch of problems sharing the template parameters.
-template <bool TAB, bool SIG>
+template <bool TAB, bool SIG, int QC>
 __global__ __launch_bounds__(AJ_THREADS, 2)
 void adj_fp32_batch_kernel(const AdjArgs* __restrict__ items, const int* __restrict__ wg0, int n)
 {
     const int k = batch_item(wg0, n, blockIdx.x);
     const AdjArgs A = items[k];
-    adj_fp32_body<TAB, SIG>(A, (int)blockIdx.x - wg0[k]);
+    adj_fp32_body<TAB, SIG, QC>(A, (int)blockIdx.x - wg0[k]);
 }
 
 // grad_w[e] = sum_z part[z][e] in FP64, increasing z (8 loads in flight per thread)
@@ -538,12 +553,15 @@ extern "C" int pbh_adjoint_tuned(const PbhGrid* g, const PbhLattice* lat, const 
     }
     const dim3 grid((unsigned)(g->nz * A.split)), block(AJ_THREADS);
     const bool sig = grad_sigma != nullptr;
-    if (A.tab) {
-        if (sig) hipLaunchKernelGGL((adj_fp32_kernel<true, true>), grid, block, 0, s, A);
-        else hipLaunchKernelGGL((adj_fp32_kernel<true, false>), grid, block, 0, s, A);
+    if (pbh_adj_mixed_q5(A)) {
+        if (sig) hipLaunchKernelGGL((adj_fp32_kernel<true, true, 5>), grid, block, 0, s, A);
+        else hipLaunchKernelGGL((adj_fp32_kernel<true, false, 5>), grid, block, 0, s, A);
+    } else if (A.tab) {
+        if (sig) hipLaunchKernelGGL((adj_fp32_kernel<true, true, 0>), grid, block, 0, s, A);
+        else hipLaunchKernelGGL((adj_fp32_kernel<true, false, 0>), grid, block, 0, s, A);
     } else {
-        if (sig) hipLaunchKernelGGL((adj_fp32_kernel<false, true>), grid, block, 0, s, A);
-        else hipLaunchKernelGGL((adj_fp32_kernel<false, false>), grid, block, 0, s, A);
+        if (sig) hipLaunchKernelGGL((adj_fp32_kernel<false, true, 0>), grid, block, 0, s, A);
+        else hipLaunchKernelGGL((adj_fp32_kernel<false, false, 0>), grid, block, 0, s, A);
     }
     if ((rc = launch_rc("adj_fp32_kernel")) != PBH_OK) return rc;
     hipLaunchKernelGGL(adj_reduce_kernel, dim3((unsigned)reduce_blocks(n, lat->n_sy * lat->n_sx)), dim3(256), 0, s,
@@ -595,16 +613,19 @@ extern "C" int pbh_lut_chain(const double* grad_idd, const double* grad_sigma, c
 }
 
 // ---- batched launches (robust-scenario batches) ----
-int pbh_launch_adj_batch(const AdjArgs* items, const int* wg0, int n, int total, bool tab, bool sig,
+int pbh_launch_adj_batch(const AdjArgs* items, const int* wg0, int n, int total, bool tab, bool sig, bool q5,
                          hipStream_t s)
 {
     const dim3 grid((unsigned)total), block(AJ_THREADS);
-    if (tab) {
-        if (sig) hipLaunchKernelGGL((adj_fp32_batch_kernel<true, true>), grid, block, 0, s, items, wg0, n);
-        else hipLaunchKernelGGL((adj_fp32_batch_kernel<true, false>), grid, block, 0, s, items, wg0, n);
+    if (tab && q5) {
+        if (sig) hipLaunchKernelGGL((adj_fp32_batch_kernel<true, true, 5>), grid, block, 0, s, items, wg0, n);
+        else hipLaunchKernelGGL((adj_fp32_batch_kernel<true, false, 5>), grid, block, 0, s, items, wg0, n);
+    } else if (tab) {
+        if (sig) hipLaunchKernelGGL((adj_fp32_batch_kernel<true, true, 0>), grid, block, 0, s, items, wg0, n);
+        else hipLaunchKernelGGL((adj_fp32_batch_kernel<true, false, 0>), grid, block, 0, s, items, wg0, n);
     } else {
-        if (sig) hipLaunchKernelGGL((adj_fp32_batch_kernel<false, true>), grid, block, 0, s, items, wg0, n);
-        else hipLaunchKernelGGL((adj_fp32_batch_kernel<false, false>), grid, block, 0, s, items, wg0, n);
+        if (sig) hipLaunchKernelGGL((adj_fp32_batch_kernel<false, true, 0>), grid, block, 0, s, items, wg0, n);
+        else hipLaunchKernelGGL((adj_fp32_batch_kernel<false, false, 0>), grid, block, 0, s, items, wg0, n);
     }
     return launch_rc("adj_fp32_batch_kernel");
 }

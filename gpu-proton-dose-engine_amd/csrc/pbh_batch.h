@@ -67,6 +67,6 @@ int pbh_launch_adj_mixed_batch(const pbh::AdjArgs* items, const int* wg0, int n,
 int pbh_launch_combine_batch(const pbh::BatchFwdItem* items, const int* wg0, int n, int total, hipStream_t s);
 int pbh_launch_peak_batch(const pbh::BatchPeakItem* items, int n, hipStream_t s);
 int pbh_launch_adj_batch(const pbh::AdjArgs* items, const int* wg0, int n, int total, bool tab, bool sig,
-                         hipStream_t s);
+                         bool q5, hipStream_t s);
 int pbh_launch_obj_batch(const pbh::BatchObjItem* items, int n, hipStream_t s);
 int pbh_launch_reduce_batch(const pbh::BatchRedItem* items, const int* wg0, int n, int total, hipStream_t s);

@@ -310,7 +310,7 @@ extern "C" int pbh_batch_create(const PbhBatchItem* items, int n, int precision,
         // adjoint mode: bit 0 table factors, bit 1 BF16 kernel, bit 2 P = 2, Q = 5 slots
         const bool mix = b->adj_prec != PBH_PREC_FP32 && A.tab && !it.grad_idd && !it.grad_sigma &&
                          pbh_adj_mixed_fits(A);
-        atabs.push_back((char)(mix ? (3 | (pbh_adj_mixed_q5(A) ? 4 : 0)) : (A.tab ? 1 : 0)));
+        atabs.push_back((char)((mix ? 3 : (A.tab ? 1 : 0)) | (pbh_adj_mixed_q5(A) ? 4 : 0)));
         aw.push_back(it.grid.nz * A.split);
         const size_t ne = (size_t)it.lattice.n_layers * it.lattice.n_sy * it.lattice.n_sx;
         hr.push_back(BatchRedItem{part, it.idd, it.grad_w, ne, it.grid.nz,
@@ -417,7 +417,7 @@ extern "C" int pbh_batch_forward_adjoint(const PbhBatch* b, void* stream)
                                             b->adj_prec, (g.mode & 4) != 0, s);
         else
             rc = pbh_launch_adj_batch(b->d_adj + g.first, b->d_adj_wg0 + g.wg0_off, g.count, g.total, g.mode & 1,
-                                      b->sig, s);
+                                      b->sig, (g.mode & 4) != 0, s);
         if (rc != PBH_OK) return rc;
     }
     return pbh_launch_reduce_batch(b->d_red, b->d_red_wg0, b->n_obj, b->red_total, s);
