@@ -203,19 +203,34 @@ __device__ __forceinline__ void adj_mixed_body(const AdjArgs& A, int blk)
                     for (int u = 0; u < AM_NT; ++u)
                         rowy[u] = table_row(gy, (a.oy + (double)((tg + u) * AM_TY) * a.dy) -
                                                     (a.sy0 + (double)j0 * a.dsy), AM_TY);
+                    // QC > 0: this thread's table entries, squared distances formed once per group
+                    TabShare<QC ? 2 * (AM_TX - 1) + QC * (AM_SC - 1) + 1 : 1> shx;
+                    TabShare<QC ? 2 * (AM_TY - 1) + QC * (AM_SC - 1) + 1 : 1> shy[AM_NT];
+                    if (QC) {
+                        shx.init(gx, rowx);
+#pragma unroll
+                        for (int u = 0; u < AM_NT; ++u) shy[u].init(gy, rowy[u]);
+                    }
                     for (int q = 0; q < nmine; ++q) {
                         const float k2 = (float)lk2[q];
                         __syncthreads();        // previous layer's readers of the tables and Q are done
-                        for (int t = threadIdx.x; t < ntX; t += FT_THREADS) {
-                            const float d = table_distance_f(rowx, t);
-                            Tx[table_slot(gx, t)] = pack_split<T>(__builtin_amdgcn_exp2f(-(d * d) * k2));
-                        }
+                        if (QC) {
+                            shx.template build<T>(Tx, k2);
 #pragma unroll
-                        for (int u = 0; u < AM_NT; ++u) {
-                            if (u >= nt) break;
-                            for (int t = threadIdx.x; t < ntY; t += FT_THREADS) {
-                                const float d = table_distance_f(rowy[u], t);
-                                Ty[u][table_slot(gy, t)] = pack_split<T>(__builtin_amdgcn_exp2f(-(d * d) * k2));
+                            for (int u = 0; u < AM_NT; ++u)
+                                if (u < nt) shy[u].template build<T>(Ty[u], k2);
+                        } else {
+                            for (int t = threadIdx.x; t < ntX; t += FT_THREADS) {
+                                const float d = table_distance_f(rowx, t);
+                                Tx[table_slot(gx, t)] = pack_split<T>(__builtin_amdgcn_exp2f(-(d * d) * k2));
+                            }
+#pragma unroll
+                            for (int u = 0; u < AM_NT; ++u) {
+                                if (u >= nt) break;
+                                for (int t = threadIdx.x; t < ntY; t += FT_THREADS) {
+                                    const float d = table_distance_f(rowy[u], t);
+                                    Ty[u][table_slot(gy, t)] = pack_split<T>(__builtin_amdgcn_exp2f(-(d * d) * k2));
+                                }
                             }
                         }
                         __syncthreads();

@@ -174,6 +174,13 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                 const double Cx = (a.ox + (double)x0 * a.dx) - (a.sx0 + (double)i0 * a.dsx);
                 const TableRow rowx = TAB ? table_row(pl.tx, Cx, FT_TX) : TableRow{0, 0.0f, 0.0f};
                 const TableRow rowy = TAB ? table_row(pl.ty, Cy, FT_TY) : TableRow{0, 0.0f, 0.0f};
+                // QC > 0: this thread's table entries, squared distances formed once per chunk
+                TabShare<QC ? 2 * (FT_TX - 1) + QC * (KX - 1) + 1 : 1> shx;
+                TabShare<QC ? 2 * (FT_TY - 1) + QC * (KYP - 1) + 1 : 1> shy;
+                if (TAB && QC) {
+                    shx.init(pl.tx, rowx);
+                    shy.init(pl.ty, rowy);
+                }
                 // spots within reach of each wave's rows (stage A, K = y spots) and columns
                 // (stage B, K = x spots), in spot units of the chunk; read after the next barrier
                 for (int q = threadIdx.x; q < (m1 - m0) * 8; q += FT_THREADS) {
@@ -225,6 +232,10 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                 };
                 auto build_tx = [&](int m) {
                     if (!TAB) return;
+                    if (QC) {
+                        shx.template build<T>(Tx[m & 1], ll.k2[m]);
+                        return;
+                    }
                     const float k2 = ll.k2[m];
                     for (int t = threadIdx.x; t < ntX; t += FT_THREADS) {
                         const float d = table_distance_f(rowx, t);
@@ -233,6 +244,10 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                 };
                 auto build_ty = [&](int m) {
                     if (!TAB) return;
+                    if (QC) {
+                        shy.template build<T>(Ty[m & 1], ll.k2[m]);
+                        return;
+                    }
                     const float k2 = ll.k2[m];
                     for (int t = threadIdx.x; t < ntY; t += FT_THREADS) {
                         const float d = table_distance_f(rowy, t);

@@ -81,6 +81,61 @@ __device__ __forceinline__ unsigned pack_split(float x)
     return (unsigned)bits16(h) | ((unsigned)bits16(l) << 16);
 }
 
+// pack_split of two values, each half converted as one pair: w0 = pack_split(x0),
+// w1 = pack_split(x1), bit for bit.
+template <typename T>
+__device__ __forceinline__ void pack_split2(float x0, float x1, unsigned& w0, unsigned& w1)
+{
+    typedef T V2 __attribute__((ext_vector_type(2)));
+    typedef float F2 __attribute__((ext_vector_type(2)));
+    const V2 h = __builtin_convertvector((F2){x0, x1}, V2);
+    const V2 l = __builtin_convertvector((F2){x0 - (float)h[0], x1 - (float)h[1]}, V2);
+    const unsigned hb = __builtin_bit_cast(unsigned, h), lb = __builtin_bit_cast(unsigned, l);
+    w0 = __builtin_amdgcn_perm(lb, hb, 0x05040100u);
+    w1 = __builtin_amdgcn_perm(lb, hb, 0x07060302u);
+}
+
+// A thread's share of one parity-interleaved axis table (P = 2, Q = QC, half set): entries
+// t = threadIdx.x + FT_THREADS k (k < N) of the NT entries, slot(t) = slot(threadIdx.x)
+// + FT_THREADS / 2 k. The squared distances depend on the tile and spot chunk only,
+// so they are formed once per chunk; each layer then costs one product, one exp2
+// and half a pair conversion per entry. Entries equal the general build's
+// pack_split(exp2(-(d * d) * k2)).
+template <int NT>
+struct TabShare {
+    static constexpr int N = (NT + FT_THREADS - 1) / FT_THREADS;
+    float d2[N];
+    int s0;     // slot of entry threadIdx.x
+    __device__ __forceinline__ void init(const AxisTable& g, const TableRow& row)
+    {
+        s0 = table_slot(g, (int)threadIdx.x);
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const float d = table_distance_f(row, (int)threadIdx.x + FT_THREADS * k);
+            d2[k] = d * d;
+        }
+    }
+    template <typename T>
+    __device__ __forceinline__ void build(unsigned* tab, float k2) const
+    {
+        const float nk2 = -k2;
+        unsigned* dst = tab + s0;
+#pragma unroll
+        for (int k = 0; k < N; k += 2) {
+            const bool ok0 = FT_THREADS * (k + 1) <= NT || (int)threadIdx.x + FT_THREADS * k < NT;
+            if (k + 1 < N) {
+                const bool ok1 = FT_THREADS * (k + 2) <= NT || (int)threadIdx.x + FT_THREADS * (k + 1) < NT;
+                unsigned w0, w1;
+                pack_split2<T>(__builtin_amdgcn_exp2f(d2[k] * nk2), __builtin_amdgcn_exp2f(d2[k + 1] * nk2), w0, w1);
+                if (ok0) dst[FT_THREADS / 2 * k] = w0;
+                if (ok1) dst[FT_THREADS / 2 * (k + 1)] = w1;
+            } else if (ok0) {
+                dst[FT_THREADS / 2 * k] = pack_split<T>(__builtin_amdgcn_exp2f(d2[k] * nk2));
+            }
+        }
+    }
+};
+
 // Eight packed entries -> hi and lo operand vectors.
 template <typename V8>
 __device__ __forceinline__ void unpack8(const unsigned* t, V8& hi, V8& lo)
