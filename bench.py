@@ -13,7 +13,11 @@ residual kernel + adjoint + depth reduction; FP32 adjoint for the FP32 form,
 BF16 or BF16x3 adjoint for the 16-bit forms) and the error of each form
 against the FP64 oracle are reported beside it. ``roofline`` is for the forward
 call of the selected form (weight split, fused forward, layer-chunk combine),
-timed per call with HIP events on the launch stream. ``cpu_baseline`` times the host restatement of the
+timed per call with HIP events on the launch stream. Step and call times
+(``ms_per_step``, ``adjoint_ms``) are wall time over back-to-back calls with a
+synchronisation on both sides; the per-call events run in a pass of their own
+(``launch_ms``, ``adjoint_launch_ms``), since recording them between calls adds
+host time that the GPU would otherwise hide. ``cpu_baseline`` times the host restatement of the
 reference's PyTorch batched-matmul engine (oracle/torch_cpu.py, FP64, all host
 threads) on full forwards of the same field; its FP64 output is the reference
 for ``accuracy_vs_fp64`` (errors and the 2 %/2 mm gamma pass rate of the
@@ -82,21 +86,33 @@ def build_field(device):
     return prob, engines.DeviceProblem(prob, device, idd=idd, sigma=sig)
 
 
+def call_events(fn, steps):
+    """Mean duration of fn() from events recorded around each call on the launch
+    stream (a pass of its own: event records between calls add host time, so they
+    stay out of the timed steps)."""
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    for s, e in ev:
+        s.record()
+        fn()
+        e.record()
+    torch.cuda.synchronize()
+    return float(np.mean([s.elapsed_time(e) for s, e in ev]))
+
+
 def time_forward(dp, precision, steps, warmup):
+    """Returns (dose, wall ms per step over `steps` back-to-back steps, ms per call
+    from per-call events)."""
     out = torch.empty(dp.shape, dtype=torch.float32, device=dp.dev)
     for _ in range(warmup):
         dp.forward(precision, out=out)
     torch.cuda.synchronize()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(steps)]
     t0 = time.perf_counter()
-    for s, e in ev:
-        s.record()
+    for _ in range(steps):
         dp.forward(precision, out=out)
-        e.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    launch_ms = float(np.mean([s.elapsed_time(e) for s, e in ev]))
+    launch_ms = call_events(lambda: dp.forward(precision, out=out), steps)
     return out, wall * 1e3 / steps, launch_ms
 
 
@@ -136,23 +152,26 @@ def adjoint_form(precision):
 
 
 def time_adjoint(dp, D, steps, warmup, precision="fp32"):
-    """Objective residual + adjoint (dJ/dw) per call, events on the launch stream.
-    Returns (ms per call, dJ/dw of the last call)."""
+    """Objective residual + adjoint (dJ/dw) per call: wall ms per call over `steps`
+    back-to-back calls, ms per call from per-call events, dJ/dw of the last call."""
     from pbdose import planning
     Dp, wv = planning.central_cube(dp.problem.grid, device=dp.dev)
-    for _ in range(warmup):
-        _, R = planning.objective(dp, D, Dp, wv)
-        planning.adjoint(dp, R, precision=precision)
-    torch.cuda.synchronize()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(steps)]
-    for s, e in ev:
-        s.record()
+    g = None
+
+    def call():
+        nonlocal g
         _, R = planning.objective(dp, D, Dp, wv)
         g = planning.adjoint(dp, R, precision=precision)
-        e.record()
+
+    for _ in range(warmup):
+        call()
     torch.cuda.synchronize()
-    return float(np.mean([s.elapsed_time(e) for s, e in ev])), g
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        call()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    return wall, call_events(call, steps), g
 
 
 def time_robust(precision, steps, objective):
@@ -268,13 +287,13 @@ def main():
         D32, ms32, launch32 = time_forward(dp, "fp32", args.steps, args.warmup)
     else:
         D32, ms32, launch32 = D, ms_step, launch_ms
-    adj32_ms, g32 = time_adjoint(dp, D32, max(5, args.steps // 4), 3)
+    adj32_ms, adj32_ev, g32 = time_adjoint(dp, D32, max(5, args.steps // 4), 3)
     aform = adjoint_form(args.precision)
     if aform != "fp32":
-        adj_ms, gmix = time_adjoint(dp, D32, max(5, args.steps // 4), 3, aform)
+        adj_ms, adj_ev, gmix = time_adjoint(dp, D32, max(5, args.steps // 4), 3, aform)
         adj_err = float((gmix.double() - g32.double()).abs().max() / g32.double().abs().max())
     else:
-        adj_ms, adj_err = adj32_ms, 0.0
+        adj_ms, adj_ev, adj_err = adj32_ms, adj32_ev, 0.0
     pcie_ms = time_torch_separable(prob, args.precision, max(5, args.steps // 10))
     bt = base_type(args.precision)
     peak = PEAK[bt]
@@ -302,6 +321,8 @@ def main():
         "adjoint_form": aform,
         "adjoint_ms": adj_ms,
         "adjoint_ms_fp32": adj32_ms,
+        "adjoint_launch_ms": adj_ev,
+        "adjoint_launch_ms_fp32": adj32_ev,
         "adjoint_max_rel_diff_vs_fp32_adjoint": adj_err,
         "fwd_adj_ms": ms_step + adj_ms,
         "fwd_adj_ms_fp32": ms32 + adj32_ms,
