@@ -90,7 +90,7 @@ def objective(dp: DeviceProblem, dose: torch.Tensor, presc: torch.Tensor,
     L = _lib.lib()
     R = torch.empty_like(dose) if want_residual else None
     j = torch.empty(1, dtype=torch.float64, device=dp.dev)
-    ws = _lib.workspace(L.pbh_objective_workspace_bytes(dp.grid_s), dp.dev)
+    ws = dp.workspace("obj", 0)
     _lib.check(L.pbh_objective(dp.grid_s, _lib.ptr(dose), _lib.ptr(presc), _lib.ptr(vox_w), _lib.ptr(R),
                                _lib.ptr(j), _lib.ptr(ws), ws.numel(), _lib.stream_of(dp.dev)),
                "pbh_objective")

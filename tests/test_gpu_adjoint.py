@@ -86,6 +86,15 @@ def test_objective_kernel():
     d64 = D.double() - Dp.double()
     assert abs(float(J) - float((wv.double() * d64 * d64).sum())) <= 1e-9 * float(J)
     assert torch.allclose(R.double(), 2 * wv.double() * d64, rtol=1e-6, atol=1e-6 * float(R.abs().max()))
+    # the objective's workspace is kept with the problem: repeated calls give the same
+    # bits, and a call with other inputs its own J
+    for _ in range(3):
+        J2, _ = planning.objective(dp, D, Dp, wv)
+        assert float(J2) == float(J)
+    Dp3 = Dp * 0.5
+    J3, _ = planning.objective(dp, D, Dp3, None)
+    d3 = D.double() - Dp3.double()
+    assert abs(float(J3) - float((d3 * d3).sum())) <= 1e-9 * float(J3)
 
 
 def test_clinical_objective_gradients_vs_finite_differences():
