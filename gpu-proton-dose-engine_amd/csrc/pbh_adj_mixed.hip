@@ -14,7 +14,8 @@
 // (y): the wave's 16 rows of R of both are loaded once into registers and split
 // into BF16 (hi, lo). Per layer (x table once per group, y table per tile):
 //   stage 1  Q = R Gx      v_mfma_f32_16x16x32_bf16, A = R (registers), B = Gx
-//                          gathered from the packed (hi, lo) x table
+//                          gathered from the packed (hi, lo) x table, once per
+//                          (spot block, K step) for both tiles of the group
 //   Q^T is stored split in LDS, four consecutive rows per 8-byte write
 //   stage 2  P = Gy^T Q    v_mfma_f32_16x16x32_bf16, A = Gy^T from the y table,
 //                          B = Q^T rows (16-byte reads)
@@ -234,36 +235,44 @@ __device__ __forceinline__ void adj_mixed_body(const AdjArgs& A, int blk)
                             }
                         }
                         __syncthreads();
-                        // stage 1 of every tile of the group: Q[y][i], this wave's 16 rows, 3 spot blocks
-#pragma unroll
-                        for (int u = 0; u < AM_NT; ++u) {
-                            if (u >= nt) break;
+                        // stage 1 of both tiles of the group: Q[y][i], this wave's 16 rows of each,
+                        // 3 spot blocks; the Gx operand of a (spot block, K step) is gathered once
+                        // and used for both tiles
 #pragma unroll 1
-                            for (int nb = 0; nb < 3; ++nb) {
-                                const int wc = __builtin_amdgcn_readfirstlane((int)awin[par][q][nb]);
-                                const int k_lo = wc & 15, k_hi = wc >> 4;
-                                v4f acc1 = {};
+                        for (int nb = 0; nb < 3; ++nb) {
+                            const int wc = __builtin_amdgcn_readfirstlane((int)awin[par][q][nb]);
+                            const int k_lo = wc & 15, k_hi = wc >> 4;
+                            v4f acc1[AM_NT];
 #pragma unroll
-                                for (int ks = 0; ks < 4; ++ks) {
-                                    if (ks < k_lo || ks >= k_hi) continue;
-                                    unsigned t8[8];
+                            for (int u = 0; u < AM_NT; ++u) acc1[u] = v4f{};
 #pragma unroll
-                                    for (int e = 0; e < 8; ++e) t8[e] = Tx[slot1(nb, ks, e)];
-                                    V8 bh, bl;
-                                    unpack8(t8, bh, bl);
-                                    acc1 = mfma16(rh[u][ks], bh, acc1);
+                            for (int ks = 0; ks < 4; ++ks) {
+                                if (ks < k_lo || ks >= k_hi) continue;
+                                unsigned t8[8];
+#pragma unroll
+                                for (int e = 0; e < 8; ++e) t8[e] = Tx[slot1(nb, ks, e)];
+                                V8 bh, bl;
+                                unpack8(t8, bh, bl);
+#pragma unroll
+                                for (int u = 0; u < AM_NT; ++u) {
+                                    if (u >= nt) break;
+                                    acc1[u] = mfma16(rh[u][ks], bh, acc1[u]);
                                     if (SPLIT) {
-                                        acc1 = mfma16(rh[u][ks], bl, acc1);
-                                        acc1 = mfma16(rl[u][ks], bh, acc1);
+                                        acc1[u] = mfma16(rh[u][ks], bl, acc1[u]);
+                                        acc1[u] = mfma16(rl[u][ks], bh, acc1[u]);
                                     }
                                 }
-                                // C layout: rows y = 16 wave + 4 g + r, column i = 16 nb + c16
+                            }
+                            // C layout: rows y = 16 wave + 4 g + r, column i = 16 nb + c16
+#pragma unroll
+                            for (int u = 0; u < AM_NT; ++u) {
+                                if (u >= nt) break;
                                 u2v hv, lv;
 #pragma unroll
                                 for (int h2 = 0; h2 < 2; ++h2) {
                                     T a0, b0, a1, b1;
-                                    split<T>(acc1[2 * h2], a0, b0);
-                                    split<T>(acc1[2 * h2 + 1], a1, b1);
+                                    split<T>(acc1[u][2 * h2], a0, b0);
+                                    split<T>(acc1[u][2 * h2 + 1], a1, b1);
                                     hv[h2] = (unsigned)bits16(a0) | ((unsigned)bits16(a1) << 16);
                                     lv[h2] = (unsigned)bits16(b0) | ((unsigned)bits16(b1) << 16);
                                 }
