@@ -40,7 +40,7 @@ constexpr int AM_NT = 2;                                          // 64-row tile
 // the forward): entry t = Q i - P a + P (T - 1) for voxel a, spot i.
 template <int QC, int T, int K>
 struct AmTab {
-    static constexpr int N = QC ? 2 * ((2 * (T - 1) + QC * (K - 1) + 2) / 2) + 2 : FT_TAB;
+    static constexpr int N = QC ? table_slots(2 * (T - 1) + QC * (K - 1) + 1) : FT_TAB;
 };
 
 template <int MODE, int QC>

@@ -311,11 +311,12 @@ static bool axis_table(double dv, double ds, int T, int K, AxisTable* out)
         const int P = (int)llround(pr);
         if (P < 1 || P > 64) continue;
         if (fabs(pr - P) > 1e-9 * pr) continue;
-        if (P * (T - 1) + Q * (K - 1) + 2 > FT_TAB) return false;
+        const int n = P * (T - 1) + Q * (K - 1) + 1;      // table entries
+        if ((P == 2 ? table_slots(n) : n + 1) > FT_TAB) return false;
         out->P = P;
         out->Q = Q;
         out->u = dv / P;
-        out->half = (P == 2) ? (P * (T - 1) + Q * (K - 1) + 2) / 2 : 0;
+        out->half = (P == 2) ? table_half(n) : 0;
         return true;
     }
     return false;

@@ -26,10 +26,26 @@ struct AxisTable {
 
 // LDS slot of table entry t. With P == 2 the lanes of a half-wave read entries
 // of one parity at stride 2; storing even and odd entries in separate halves
-// makes those reads consecutive (no bank conflicts).
+// makes those reads consecutive.
 __device__ __forceinline__ int table_slot(const AxisTable& g, int t)
 {
     return g.half ? (t & 1) * g.half + (t >> 1) : t;
+}
+
+// Start of the odd entries of a parity-interleaved table of n entries: the first
+// slot at or after the (n + 1) / 2 even ones that is 14 mod 32. LDS serves a
+// wave's read in two 32-lane passes over 32 four-byte banks; where a pass reads
+// even and odd entries (the adjoint's gathers, the FP32 forward's y operand),
+// this offset puts its 32 dwords on 32 distinct banks.
+__host__ __device__ constexpr int table_half(int n)
+{
+    return (n + 1) / 2 + ((14 - (n + 1) / 2 % 32) % 32 + 32) % 32;
+}
+
+// LDS slots a parity-interleaved table of n entries spans.
+__host__ __device__ constexpr int table_slots(int n)
+{
+    return table_half(n) + n / 2 + 1;
 }
 
 // Host-side launch plan of a forward call.

@@ -114,9 +114,9 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
     __shared__ __attribute__((aligned(16))) T Wl[2][SPLIT ? SCH * WT_STRIDE : 8];
     __shared__ __attribute__((aligned(16))) T Uh[2][FT_TY * U_STRIDE];     // c U hi [y][i]
     __shared__ __attribute__((aligned(16))) T Ul[2][SPLIT ? FT_TY * U_STRIDE : 8];
-    // table slots: exact interleaved size on the P = 2, Q = QC geometry, FT_TAB otherwise
-    constexpr int TXN = !TAB ? 1 : QC ? 2 * ((2 * (FT_TX - 1) + QC * (KX - 1) + 2) / 2) + 2 : FT_TAB;
-    constexpr int TYN = !TAB ? 1 : QC ? 2 * ((2 * (FT_TY - 1) + QC * (KYP - 1) + 2) / 2) + 2 : FT_TAB;
+    // table slots: the interleaved span on the P = 2, Q = QC geometry, FT_TAB otherwise
+    constexpr int TXN = !TAB ? 1 : QC ? table_slots(2 * (FT_TX - 1) + QC * (KX - 1) + 1) : FT_TAB;
+    constexpr int TYN = !TAB ? 1 : QC ? table_slots(2 * (FT_TY - 1) + QC * (KYP - 1) + 1) : FT_TAB;
     __shared__ unsigned Tx[TAB ? 2 : 1][TXN];
     __shared__ unsigned Ty[TAB ? 2 : 1][TYN];
     __shared__ float uscale[2][2];
