@@ -216,10 +216,10 @@ __device__ __forceinline__ void adj_mixed_body(const AdjArgs& A, int blk)
                         const float k2 = (float)lk2[q];
                         __syncthreads();        // previous layer's readers of the tables and Q are done
                         if (QC) {
-                            shx.template build<T>(Tx, k2);
+                            build_packed<T>(shx, Tx, k2);
 #pragma unroll
                             for (int u = 0; u < AM_NT; ++u)
-                                if (u < nt) shy[u].template build<T>(Ty[u], k2);
+                                if (u < nt) build_packed<T>(shy[u], Ty[u], k2);
                         } else {
                             for (int t = threadIdx.x; t < ntX; t += FT_THREADS) {
                                 const float d = table_distance_f(rowx, t);

@@ -109,6 +109,13 @@ __device__ __forceinline__ void fwd_fp32_body(const FwdArgs& a, const FwdPlan& p
                 const double Cx = (a.ox + (double)x0 * a.dx) - (a.sx0 + (double)i0 * a.dsx);
                 const TableRow rowx = TAB ? table_row(pl.tx, Cx, FT_TX) : TableRow{0, 0.0f, 0.0f};
                 const TableRow rowy = TAB ? table_row(pl.ty, Cy, FT_TY) : TableRow{0, 0.0f, 0.0f};
+                // QC > 0: this thread's table entries, squared distances formed once per chunk
+                TabShare<QC ? 2 * (FT_TX - 1) + QC * (SCP - 1) + 1 : 1> shx;
+                TabShare<QC ? 2 * (FT_TY - 1) + QC * (SCP - 1) + 1 : 1> shy;
+                if (TAB && QC) {
+                    shx.init(pl.tx, rowx);
+                    shy.init(pl.ty, rowy);
+                }
 
                 auto load_w = [&](int m) {
                     const float* wl = a.w + (size_t)ll.lay[m] * a.nsy * a.nsx + (size_t)j0 * a.nsx + i0;
@@ -128,6 +135,10 @@ __device__ __forceinline__ void fwd_fp32_body(const FwdArgs& a, const FwdPlan& p
                 };
                 auto build_tx = [&](int m) {
                     if (!TAB) return;
+                    if (QC) {
+                        shx.build_f32(Tx[m & 1], ll.k2[m]);
+                        return;
+                    }
                     const float k2 = ll.k2[m];
                     for (int t = threadIdx.x; t < ntX; t += FT_THREADS) {
                         const float d = table_distance_f(rowx, t);
@@ -136,6 +147,10 @@ __device__ __forceinline__ void fwd_fp32_body(const FwdArgs& a, const FwdPlan& p
                 };
                 auto build_ty = [&](int m) {
                     if (!TAB) return;
+                    if (QC) {
+                        shy.build_f32(Ty[m & 1], ll.k2[m]);
+                        return;
+                    }
                     const float k2 = ll.k2[m];
                     for (int t = threadIdx.x; t < ntY; t += FT_THREADS) {
                         const float d = table_distance_f(rowy, t);

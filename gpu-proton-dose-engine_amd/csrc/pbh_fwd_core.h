@@ -97,6 +97,40 @@ __device__ __forceinline__ float table_distance_f(const TableRow& r, int t)
     return ((float)(r.k0 - t) + r.f) * r.u;
 }
 
+// A thread's share of one parity-interleaved axis table (P = 2, half set): entries
+// t = threadIdx.x + FT_THREADS k (k < N) of the NT entries, slot(t) = slot(threadIdx.x)
+// + FT_THREADS / 2 k. The squared distances depend on the tile and spot chunk only,
+// so they are formed once per chunk; each layer then costs one product and one
+// exp2 per entry. Entries equal the general build's exp2(-(d * d) * k2).
+template <int NT>
+struct TabShare {
+    static constexpr int N = (NT + FT_THREADS - 1) / FT_THREADS;
+    float d2[N];
+    int s0;     // slot of entry threadIdx.x
+    // entry threadIdx.x + FT_THREADS k exists
+    __device__ static __forceinline__ bool ok(int k)
+    {
+        return FT_THREADS * (k + 1) <= NT || (int)threadIdx.x + FT_THREADS * k < NT;
+    }
+    __device__ __forceinline__ void init(const AxisTable& g, const TableRow& row)
+    {
+        s0 = table_slot(g, (int)threadIdx.x);
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const float d = table_distance_f(row, (int)threadIdx.x + FT_THREADS * k);
+            d2[k] = d * d;
+        }
+    }
+    // FP32 entries
+    __device__ __forceinline__ void build_f32(float* tab, float k2) const
+    {
+        const float nk2 = -k2;
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+            if (ok(k)) tab[s0 + FT_THREADS / 2 * k] = __builtin_amdgcn_exp2f(d2[k] * nk2);
+    }
+};
+
 // Writes a 32 x 32 block of the dose tile held in the C layout of a 32x32 MFMA
 // (row 8 (r/4) + 4 (lane/32) + r%4, column lane%32) at (rm, cn) in the tile:
 // chunk 0 to the dose (rows and columns inside the grid), chunk c >= 1 to its

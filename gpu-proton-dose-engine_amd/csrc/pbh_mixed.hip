@@ -233,7 +233,7 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                 auto build_tx = [&](int m) {
                     if (!TAB) return;
                     if (QC) {
-                        shx.template build<T>(Tx[m & 1], ll.k2[m]);
+                        build_packed<T>(shx, Tx[m & 1], ll.k2[m]);
                         return;
                     }
                     const float k2 = ll.k2[m];
@@ -245,7 +245,7 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                 auto build_ty = [&](int m) {
                     if (!TAB) return;
                     if (QC) {
-                        shy.template build<T>(Ty[m & 1], ll.k2[m]);
+                        build_packed<T>(shy, Ty[m & 1], ll.k2[m]);
                         return;
                     }
                     const float k2 = ll.k2[m];

@@ -95,46 +95,25 @@ __device__ __forceinline__ void pack_split2(float x0, float x1, unsigned& w0, un
     w1 = __builtin_amdgcn_perm(lb, hb, 0x07060302u);
 }
 
-// A thread's share of one parity-interleaved axis table (P = 2, Q = QC, half set): entries
-// t = threadIdx.x + FT_THREADS k (k < N) of the NT entries, slot(t) = slot(threadIdx.x)
-// + FT_THREADS / 2 k. The squared distances depend on the tile and spot chunk only,
-// so they are formed once per chunk; each layer then costs one product, one exp2
-// and half a pair conversion per entry. Entries equal the general build's
-// pack_split(exp2(-(d * d) * k2)).
-template <int NT>
-struct TabShare {
-    static constexpr int N = (NT + FT_THREADS - 1) / FT_THREADS;
-    float d2[N];
-    int s0;     // slot of entry threadIdx.x
-    __device__ __forceinline__ void init(const AxisTable& g, const TableRow& row)
-    {
-        s0 = table_slot(g, (int)threadIdx.x);
+// Packed (hi, lo) entries of a thread's share of a table: entry t gets
+// pack_split(exp2(-(d * d) * k2)), two entries per pair conversion.
+template <typename T, int NT>
+__device__ __forceinline__ void build_packed(const TabShare<NT>& sh, unsigned* tab, float k2)
+{
+    const float nk2 = -k2;
+    unsigned* dst = tab + sh.s0;
 #pragma unroll
-        for (int k = 0; k < N; ++k) {
-            const float d = table_distance_f(row, (int)threadIdx.x + FT_THREADS * k);
-            d2[k] = d * d;
+    for (int k = 0; k < TabShare<NT>::N; k += 2) {
+        if (k + 1 < TabShare<NT>::N) {
+            unsigned w0, w1;
+            pack_split2<T>(__builtin_amdgcn_exp2f(sh.d2[k] * nk2), __builtin_amdgcn_exp2f(sh.d2[k + 1] * nk2), w0, w1);
+            if (TabShare<NT>::ok(k)) dst[FT_THREADS / 2 * k] = w0;
+            if (TabShare<NT>::ok(k + 1)) dst[FT_THREADS / 2 * (k + 1)] = w1;
+        } else if (TabShare<NT>::ok(k)) {
+            dst[FT_THREADS / 2 * k] = pack_split<T>(__builtin_amdgcn_exp2f(sh.d2[k] * nk2));
         }
     }
-    template <typename T>
-    __device__ __forceinline__ void build(unsigned* tab, float k2) const
-    {
-        const float nk2 = -k2;
-        unsigned* dst = tab + s0;
-#pragma unroll
-        for (int k = 0; k < N; k += 2) {
-            const bool ok0 = FT_THREADS * (k + 1) <= NT || (int)threadIdx.x + FT_THREADS * k < NT;
-            if (k + 1 < N) {
-                const bool ok1 = FT_THREADS * (k + 2) <= NT || (int)threadIdx.x + FT_THREADS * (k + 1) < NT;
-                unsigned w0, w1;
-                pack_split2<T>(__builtin_amdgcn_exp2f(d2[k] * nk2), __builtin_amdgcn_exp2f(d2[k + 1] * nk2), w0, w1);
-                if (ok0) dst[FT_THREADS / 2 * k] = w0;
-                if (ok1) dst[FT_THREADS / 2 * (k + 1)] = w1;
-            } else if (ok0) {
-                dst[FT_THREADS / 2 * k] = pack_split<T>(__builtin_amdgcn_exp2f(d2[k] * nk2));
-            }
-        }
-    }
-};
+}
 
 // Eight packed entries -> hi and lo operand vectors.
 template <typename V8>
