@@ -80,6 +80,13 @@ __device__ __forceinline__ void adj_fp32_body(const AdjArgs& A, int blk)
                         }
                         const double Cx = (a.ox + (double)x0 * a.dx) - (a.sx0 + (double)i0 * a.dsx);
                         const double Cy = (a.oy + (double)y0 * a.dy) - (a.sy0 + (double)j0 * a.dsy);
+                        // QC > 0: this thread's table entries, squared distances formed once per tile
+                        TabShare<QC ? 2 * (AJ_TX - 1) + QC * (AJ_SC - 1) + 1 : 1> shx;
+                        TabShare<QC ? 2 * (AJ_TY - 1) + QC * (AJ_SC - 1) + 1 : 1> shy;
+                        if (TAB && QC) {
+                            shx.init(A.tx, table_row(A.tx, Cx, AJ_TX));
+                            shy.init(A.ty, table_row(A.ty, Cy, AJ_TY));
+                        }
 
                         for (int q = 0; q < nl; ++q) {
                             const int l = lb + q * A.split;
@@ -93,7 +100,21 @@ __device__ __forceinline__ void adj_fp32_body(const AdjArgs& A, int blk)
                             const float k2 = (float)k2d;
                             const float is3 = (float)(1.0 / (sd * sd * sd));   // dG/ds = G d^2 / s^3
                             const double cn = iddv / (2.0 * kPi * sd * sd);
-                            if (TAB) {
+                            if (TAB && QC) {
+                                // G and, for dJ/dsigma, G d^2 / s^3 from the per-tile squared distances
+                                auto build = [&](const auto& sh, float* t1, float* t2) {
+#pragma unroll
+                                    for (int k = 0; k < sh.N; ++k) {
+                                        if (!sh.ok(k)) continue;
+                                        const int sl = sh.s0 + AJ_THREADS / 2 * k;
+                                        const float gv = __builtin_amdgcn_exp2f(sh.d2[k] * -k2);
+                                        t1[sl] = gv;
+                                        if (SIG) t2[sl] = gv * sh.d2[k] * is3;
+                                    }
+                                };
+                                build(shx, Tx, Tx2);
+                                build(shy, Ty, Ty2);
+                            } else if (TAB) {
                                 const int ntX = A.tx.P * (AJ_TX - 1) + A.tx.Q * (AJ_SC - 1) + 1;
                                 const int ntY = A.ty.P * (AJ_TY - 1) + A.ty.Q * (AJ_SC - 1) + 1;
                                 const TableRow rx = table_row(A.tx, Cx, AJ_TX), ry = table_row(A.ty, Cy, AJ_TY);
