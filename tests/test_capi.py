@@ -226,3 +226,40 @@ def test_forward_chunk_chooser():
     assert L.pbh_forward_choose_chunk(gs, ls, 0, act.ctypes.data, 256) == 26     # FP32
     for prec in (1, 3):                                                          # BF16, BF16x3
         assert 16 <= L.pbh_forward_choose_chunk(gs, ls, prec, act.ctypes.data, 256) <= 20
+
+
+def test_objective_and_adjoint_reject_bad_arguments_without_launching():
+    """Argument checks of pbh_objective and pbh_adjoint_tuned return their codes
+    before any device call (the pointers below are never dereferenced)."""
+    L = _lib.lib()
+    g = _lib.PbhGrid(128, 128, 128, 0, 0.2, 0.2, 0.2, -12.7, -12.7, 0.0)
+    lat = _lib.PbhLattice(41, 41, 40, 0, 0.5, 0.5, 0.0, 0.0)
+    nul, one = C.c_void_p(0), C.c_void_p(256)
+    ob = L.pbh_objective_workspace_bytes(g)
+    assert ob >= 1024 * 8                                  # FP64 partial sums of J
+    assert L.pbh_objective(g, nul, one, nul, one, one, one, ob, nul) == -1          # no dose
+    assert L.pbh_objective(g, one, one, nul, one, nul, one, ob, nul) == -1          # no J output
+    bad = _lib.PbhGrid(0, 128, 128, 0, 0.2, 0.2, 0.2, 0, 0, 0)
+    assert L.pbh_objective(bad, one, one, nul, one, one, one, ob, nul) == -2
+    assert L.pbh_objective(g, one, one, nul, one, one, one, ob - 1, nul) == -7
+    P = _lib.PREC
+    ab = L.pbh_adjoint_workspace_bytes(g, lat, P["bf16x3"])
+    assert ab >= 128 * 40 * 41 * 41 * 4                    # per-slice FP32 partials
+    args = lambda prec, gi, split, nbytes: (g, lat, one, one, one, one, one, gi, nul, prec, split, one, nbytes, nul)
+    assert L.pbh_adjoint_tuned(g, lat, nul, one, one, one, one, nul, nul, 0, 0, one, ab, nul) == -1   # no weights
+    assert L.pbh_adjoint_tuned(*args(99, nul, 0, ab)) == -5                        # unknown precision
+    assert L.pbh_adjoint_tuned(*args(P["fp16"], nul, 0, ab)) == -5                 # no FP16 adjoint
+    assert L.pbh_adjoint_tuned(*args(P["bf16x3"], one, 0, ab)) == -5               # table gradients: FP32 only
+    assert L.pbh_adjoint_tuned(*args(P["bf16x3"], nul, -1, ab)) == -2              # negative split
+    assert L.pbh_adjoint_tuned(*args(P["bf16x3"], nul, 0, ab - 1)) == -7           # workspace too small
+
+
+def test_bench_runs_on_one_gpu_only():
+    """bench.py measures one GPU: --gpus above 1 exits with status 2 and says why,
+    before touching a device."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
