@@ -429,8 +429,8 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
     store_block(a, pl, tile, ch, z, y0, x0, 0, cnB, acc, (size_t)tiles);
 }
 
-// Workgroups per CU: on the P = 2, Q = 5 geometry (exact-size tables) three for
-// the split forms (LDS near 51 KB) and four for the single-product forms (30 KB,
+// Workgroups per CU: on the P = 2, Q = 5 geometry (tables of their interleaved span)
+// three for the split forms (LDS near 52 KB) and four for the single-product forms (31 KB,
 // VGPRs held to 128); two otherwise.
 template <int MODE, int QC> struct MixedOcc {
     static constexpr int value = !QC ? 2 : (MODE == MODE_PLAIN ? 4 : 3);
