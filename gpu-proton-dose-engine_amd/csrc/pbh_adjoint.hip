@@ -19,6 +19,10 @@
 // the workgroup in a fixed order.
 // Kernel 2 sums the per-slice FP32 partials over z in FP64 in increasing z: the
 // weight gradient is deterministic (no atomics).
+#include <algorithm>
+#include <functional>
+#include <vector>
+
 #include "pbh_batch.h"
 
 namespace pbh {
@@ -31,6 +35,7 @@ constexpr int AJ_QS = AJ_SC + 1;                 // Q row stride
 // Slot of table entry t for axes of AJ_TX / AJ_TY voxels (same layout rules as the forward).
 __device__ __forceinline__ float tab_get(const float* T, const AxisTable& g, int t) { return T[table_slot(g, t)]; }
 
+constexpr int AJ_MAX_SPLIT = 16;  // workgroups per depth slice, at most
 constexpr int AJ_LB = 64;   // layers of one workgroup handled per pass over the slice tiles
 
 // blk: the workgroup's index in the problem's own grid (slice * split + share).
@@ -493,6 +498,71 @@ using namespace pbh;
 
 extern "C" int pbh_validate(const PbhGrid* g, const PbhLattice* lat);
 
+// Layer split of the adjoint from the active-layer count of every depth slice:
+// simulates the dispatch of the workgroups (slice-major, split s of slice z is
+// block z split + s) onto `slots` resident slots, each taking the next workgroup
+// when it frees. Workgroup s of slice z takes the active layers whose index is s
+// modulo split; it costs its layers plus AJ_WG_COST layers of fixed work (R tiles,
+// windows), or AJ_WG_EMPTY when it has none. Returns the split (1 to AJ_MAX_SPLIT,
+// at least min_split) with the shortest makespan, the smaller on ties.
+// On the 128^3 / 41 x 41 x 40-layer field it picks 7 (BF16x3, 3 slots per CU),
+// 10 (BF16, 4 slots) and 5 (FP32, 2 slots); a measured sweep of splits 2-16 had
+// the same bests, 7 %, 5 % and 12 % ahead of a split of 8.
+constexpr double AJ_WG_COST = 2.5, AJ_WG_EMPTY = 0.3;
+
+static bool adj_axis(double dv, double ds, int T, int K, AxisTable* out);
+
+static int model_split(int nz, int L, const int32_t* active, long long slots, int min_split)
+{
+    if (slots < 1) slots = 1;
+    if (slots > (1 << 20)) slots = 1 << 20;
+    int best = std::max(1, min_split);
+    double best_t = -1.0;
+    std::vector<double> heap;
+    for (int sp = std::max(1, min_split); sp <= AJ_MAX_SPLIT; ++sp) {
+        if ((long long)nz * sp > (1LL << 24)) break;                  // keep the model cheap
+        heap.assign((size_t)slots, 0.0);                                // min-heap of slot finish times
+        for (int z = 0; z < nz; ++z) {
+            const int act = active[z] < 0 ? 0 : (active[z] > L ? L : active[z]);
+            for (int s = 0; s < sp; ++s) {
+                const int n = act > s ? (act - s + sp - 1) / sp : 0;
+                const double c = n > 0 ? n + AJ_WG_COST : AJ_WG_EMPTY;
+                std::pop_heap(heap.begin(), heap.end(), std::greater<double>());
+                heap.back() += c;
+                std::push_heap(heap.begin(), heap.end(), std::greater<double>());
+            }
+        }
+        const double t = *std::max_element(heap.begin(), heap.end());
+        if (best_t < 0.0 || t < best_t) {
+            best_t = t;
+            best = sp;
+        }
+    }
+    return best;
+}
+
+extern "C" int pbh_adjoint_choose_split(const PbhGrid* g, const PbhLattice* lat, int precision,
+                                        const int32_t* active_per_z, int compute_units)
+{
+    int rc = pbh_validate(g, lat);
+    if (rc != PBH_OK) return rc;
+    if (!active_per_z) return PBH_ERR_NULL_ARG;
+    const bool mixed = precision == PBH_PREC_BF16 || precision == PBH_PREC_BF16X3;
+    if (precision != PBH_PREC_FP32 && !mixed) return PBH_ERR_UNSUPPORTED;
+    // resident workgroups per CU: the FP32 kernel 2; the mixed kernels on the P = 2, Q = 5
+    // table geometry 3 (split products, 152 VGPRs) or 4 (single products, 112 VGPRs), else 2
+    int per_cu = 2;
+    AxisTable tx, ty;
+    if (mixed && adj_axis(g->dx, lat->dx_spot, AJ_TX, AJ_SC, &tx) && adj_axis(g->dy, lat->dy_spot, AJ_TY, AJ_SC, &ty) &&
+        tx.P == 2 && tx.Q == 5 && ty.P == 2 && ty.Q == 5)
+        per_cu = precision == PBH_PREC_BF16X3 ? 3 : 4;
+    // the mixed kernel keeps at most AM_MAXK (128) layers of a slice per workgroup
+    const int min_split = mixed ? (lat->n_layers + 127) / 128 : 1;
+    if (min_split > AJ_MAX_SPLIT) return PBH_ERR_UNSUPPORTED;
+    const int cus = compute_units > 0 ? compute_units : pbh_device_cus();
+    return model_split(g->nz, lat->n_layers, active_per_z, (long long)per_cu * cus, min_split);
+}
+
 int pbh_adj_default_split(const PbhGrid* g, const PbhLattice* lat)
 {
     int s = 1;
@@ -531,7 +601,7 @@ int pbh_make_adj_args(const PbhGrid* g, const PbhLattice* lat, const float* w, c
                       double* grad_sigma, int split, AdjArgs* out)
 {
     if (split == 0) split = pbh_adj_default_split(g, lat);
-    if (split != 1 && split != 2 && split != 4 && split != 8 && split != 16) return PBH_ERR_UNSUPPORTED;
+    if (split < 1 || split > AJ_MAX_SPLIT) return PBH_ERR_UNSUPPORTED;
     if ((long long)g->nz * split > 0x7fffffffLL) return PBH_ERR_UNSUPPORTED;
     AdjArgs& A = *out;
     A.f = make_fwd_args(g, lat, w, idd, sigma, nullptr);

@@ -95,8 +95,7 @@ int check_items(const PbhBatchItem* items, int n, int precision)
 int check_tuning(const PbhBatchTuning& t)
 {
     if (t.fwd_chunk < 0) return PBH_ERR_UNSUPPORTED;
-    if (t.adj_split != 0 && t.adj_split != 1 && t.adj_split != 2 && t.adj_split != 4 && t.adj_split != 8 &&
-        t.adj_split != 16)
+    if (t.adj_split < 0 || t.adj_split > 16)
         return PBH_ERR_UNSUPPORTED;
     if (t.group != 0 && t.group != 1) return PBH_ERR_UNSUPPORTED;
     if (t.order != 0 && t.order != 1) return PBH_ERR_UNSUPPORTED;

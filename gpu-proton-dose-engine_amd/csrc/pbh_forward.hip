@@ -337,7 +337,8 @@ static bool axis_table(double dv, double ds, int T, int K, AxisTable* out)
     return false;
 }
 
-static int device_cus()
+// Compute units of the current device (256 when the query fails), cached per device.
+int pbh_device_cus()
 {
     static int cus[64];
     int dev = 0;
@@ -363,7 +364,7 @@ constexpr int FT_MIN_CHUNK = 4;   // smallest automatic layer chunk
 // active-layer counts, replaces this rule.
 int pbh_auto_chunk(const FwdArgs& a)
 {
-    const long long slots = 2LL * device_cus();
+    const long long slots = 2LL * pbh_device_cus();
     const long long per_tile = (3 * slots + 2 * tile_count(a) - 1) / (2 * tile_count(a));
     if (per_tile <= 1) return a.L;
     int chunk = (int)((a.L + per_tile - 1) / per_tile);
@@ -521,7 +522,7 @@ extern "C" int pbh_forward_choose_chunk(const PbhGrid* g, const PbhLattice* lat,
             pl.table && pl.tx.P == 2 && pl.tx.Q == 5 && pl.ty.P == 2 && pl.ty.Q == 5)
             per_cu = 3;
     }
-    const int cus = compute_units > 0 ? compute_units : device_cus();
+    const int cus = compute_units > 0 ? compute_units : pbh_device_cus();
     return model_chunk(a, active_per_z, (long long)per_cu * cus);
 }
 

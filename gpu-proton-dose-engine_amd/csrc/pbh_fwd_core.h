@@ -359,3 +359,4 @@ int pbh_plan_forward(const pbh::FwdArgs& a, int kx_pad, int ky_pad, void* ws, si
 size_t pbh_forward_ws_bytes(const pbh::FwdArgs& a, int layer_chunk);
 int pbh_auto_chunk(const pbh::FwdArgs& a);
 int pbh_launch_combine(const pbh::FwdArgs& a, const pbh::FwdPlan& pl, hipStream_t s);
+int pbh_device_cus();

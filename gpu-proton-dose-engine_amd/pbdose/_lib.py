@@ -74,6 +74,7 @@ SIGNATURES = {
     "pbh_adjoint_workspace_bytes": (C.c_size_t, [_G, _L, C.c_int]),
     "pbh_adjoint": (C.c_int, [_G, _L, _P, _P, _P, _P, _P, _P, _P, C.c_int, _P, C.c_size_t, _P]),
     "pbh_adjoint_tuned": (C.c_int, [_G, _L, _P, _P, _P, _P, _P, _P, _P, C.c_int, C.c_int, _P, C.c_size_t, _P]),
+    "pbh_adjoint_choose_split": (C.c_int, [_G, _L, C.c_int, _P, C.c_int]),
     "pbh_objective_workspace_bytes": (C.c_size_t, [_G]),
     "pbh_objective": (C.c_int, [_G, _P, _P, _P, _P, _P, _P, C.c_size_t, _P]),
     "pbh_lut_chain": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_int, _P, _P]),

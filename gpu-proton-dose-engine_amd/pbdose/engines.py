@@ -80,6 +80,25 @@ class DeviceProblem:
             self._chunk[precision] = int(c)
         return self._chunk[precision]
 
+    def auto_split(self, precision: int) -> int:
+        """Layer split of the automatic adjoint for this problem's active layers
+        (pbh_adjoint_choose_split; cached like auto_chunk)."""
+        key = ("split", precision)
+        if key not in self._chunk:
+            g, lat = self.problem.grid, self.problem.lattice
+            ckey = ("split", g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, lat.n_sx, lat.n_sy, lat.n_layers,
+                    lat.dx_spot, lat.dy_spot, self.dev.index, precision, self._active.tobytes())
+            c = _CHUNK_CACHE.get(ckey)
+            if c is None:
+                c = _lib.lib().pbh_adjoint_choose_split(self.grid_s, self.lat_s, precision,
+                                                        self._active.ctypes.data, 0)
+                _lib.check(min(c, 0), "pbh_adjoint_choose_split")
+                if len(_CHUNK_CACHE) > 4096:
+                    _CHUNK_CACHE.clear()
+                _CHUNK_CACHE[ckey] = c
+            self._chunk[key] = int(c)
+        return self._chunk[key]
+
     def workspace(self, kind: str, precision: int, layer_chunk: int = 0) -> torch.Tensor:
         key = (kind, precision, layer_chunk)
         if key not in self._ws:

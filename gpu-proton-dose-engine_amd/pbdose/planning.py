@@ -102,7 +102,9 @@ def adjoint(dp: DeviceProblem, R: torch.Tensor, w: Optional[torch.Tensor] = None
     """dJ/dw (L, nsy, nsx) FP32 for the dose cotangent R; with ``lut_grads`` also
     dJ/dIDD and dJ/dsigma (L, nz) FP64 (FP32 arithmetic only). ``split``:
     workgroups per depth slice sharing its layers (0 = automatic).
-    ``precision``: 'fp32', or 'bf16' / 'bf16x3' (BF16 MFMA, dJ/dw only)."""
+    ``precision``: 'fp32', or 'bf16' / 'bf16x3' (BF16 MFMA, dJ/dw only). Without table
+    gradients, ``split`` 0 takes the split that pbh_adjoint_choose_split models fastest
+    for this problem's active layers per slice."""
     if precision not in ("fp32", "bf16", "bf16x3"):
         raise ValueError(f"adjoint precision {precision!r}: 'fp32', 'bf16' or 'bf16x3'")
     if lut_grads and precision != "fp32":
@@ -116,6 +118,8 @@ def adjoint(dp: DeviceProblem, R: torch.Tensor, w: Optional[torch.Tensor] = None
     gi = torch.empty_like(dp.idd) if lut_grads else None
     gs = torch.empty_like(dp.sigma) if lut_grads else None
     ws = dp.workspace("adj", 0)
+    if int(split) == 0 and not lut_grads:    # table gradients take every layer of a slice
+        split = dp.auto_split(_lib.PREC[precision])
     _lib.check(L.pbh_adjoint_tuned(dp.grid_s, dp.lat_s, _lib.ptr(wt), _lib.ptr(dp.idd), _lib.ptr(dp.sigma),
                                    _lib.ptr(R), _lib.ptr(gw), _lib.ptr(gi), _lib.ptr(gs), _lib.PREC[precision],
                                    int(split),

@@ -200,12 +200,22 @@ int pbh_adjoint(const PbhGrid* g, const PbhLattice* lat,
                 int precision, void* workspace, size_t workspace_bytes, void* stream);
 
 /* pbh_adjoint with an explicit layer split: `split` workgroups share the layers
- * of each depth slice (1, 2, 4, 8 or 16; 0 = automatic). Results do not depend
- * on the split. */
+ * of each depth slice (1 to 16; 0 = automatic from the geometry). Results do not
+ * depend on the split. */
 int pbh_adjoint_tuned(const PbhGrid* g, const PbhLattice* lat,
                       const float* w, const double* idd, const double* sigma,
                       const float* R, float* grad_w, double* grad_idd, double* grad_sigma,
                       int precision, int split, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Layer split for pbh_adjoint_tuned from the number of active layers (IDD != 0)
+ * of every depth slice, active_per_z[nz] (host memory): simulates the dispatch
+ * of the adjoint's workgroups onto the resident slots (compute_units CUs, <= 0:
+ * the current device's) and returns the split (1 to 16) with the shortest
+ * makespan, or a negative PBH_ERR_* code. precision: PBH_PREC_FP32, PBH_PREC_BF16
+ * or PBH_PREC_BF16X3. Host only; no counterpart in the reference (its engines are
+ * forward-only). */
+int pbh_adjoint_choose_split(const PbhGrid* g, const PbhLattice* lat, int precision,
+                             const int32_t* active_per_z, int compute_units);
 
 /* R = 2 wv (dose - presc) and J = sum wv (dose - presc)^2 (J written to
  * j_out[0] as FP64). Fused residual of the weighted-quadratic objective. */
@@ -248,7 +258,7 @@ typedef struct PbhBatchItem {
 typedef struct PbhBatchTuning {
     int32_t fwd_chunk;       /* forward layer chunk: at most this many active layers per
                                 workgroup (0 = automatic per item) */
-    int32_t adj_split;       /* adjoint layer split 1, 2, 4, 8 or 16 (0 = per-item default) */
+    int32_t adj_split;       /* adjoint layer split 1 to 16 (0 = per-item default) */
     int32_t group;           /* 0: one forward launch at the widest spot chunk;
                                 1: one forward launch per chunk width */
     int32_t order;           /* 0: items in the given order; 1: largest work first */

@@ -130,7 +130,7 @@ def test_batch_workspace_query_and_argument_checks():
     assert L.pbh_batch_workspace_bytes(items, 3, 3, None) > 0          # default tuning
     assert L.pbh_batch_workspace_bytes(items, 0, 3, None) == 0          # empty batch
     assert L.pbh_batch_workspace_bytes(items, 3, 99, None) == 0         # bad precision
-    bad = _lib.PbhBatchTuning(0, 3, 0, 0)                               # adjoint split 3
+    bad = _lib.PbhBatchTuning(0, 17, 0, 0)                              # adjoint split above 16
     assert L.pbh_batch_workspace_bytes(items, 3, 3, C.byref(bad)) == 0
     bad = _lib.PbhBatchTuning(-1, 0, 0, 0)                              # negative layer chunk
     assert L.pbh_batch_workspace_bytes(items, 3, 3, C.byref(bad)) == 0
@@ -263,3 +263,28 @@ def test_bench_runs_on_one_gpu_only():
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_adjoint_split_chooser():
+    """Host-only layer-split model of the adjoint: argument checks, and the splits
+    measured fastest on the 128^3 / 41x41 x 40-layer field (BF16x3 7, BF16 10,
+    FP32 5 of splits 2-16)."""
+    import numpy as np
+    from oracle import pb_oracle as O
+    from pbdose import model
+    L = _lib.lib()
+    g = model.DoseGrid(128, 128, 128, 0.2, 0.2, 0.2, -12.7, -12.7, 0.0)
+    lat = model.SpotLattice(41, 41, 40, 0.5, 0.5)
+    gs, ls = _lib.grid_struct(g), _lib.lattice_struct(lat)
+    idd, _ = O.layer_luts(g.z, np.linspace(70, 180, 40))
+    act = np.ascontiguousarray((idd != 0).sum(0).astype(np.int32))
+    P = _lib.PREC
+    assert L.pbh_adjoint_choose_split(gs, ls, P["bf16x3"], None, 256) == -1
+    assert L.pbh_adjoint_choose_split(gs, ls, P["fp16"], act.ctypes.data, 256) == -5
+    assert L.pbh_adjoint_choose_split(gs, ls, P["bf16x3"], act.ctypes.data, 256) == 7
+    assert L.pbh_adjoint_choose_split(gs, ls, P["bf16"], act.ctypes.data, 256) == 10
+    assert L.pbh_adjoint_choose_split(gs, ls, P["fp32"], act.ctypes.data, 256) == 5
+    # every slice with all layers active on 8 CUs: more workgroups per slice only add
+    # fixed work once the slots are full
+    full = np.full(128, 40, np.int32)
+    assert 1 <= L.pbh_adjoint_choose_split(gs, ls, P["fp32"], full.ctypes.data, 8) <= 2

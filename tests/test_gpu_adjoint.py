@@ -236,3 +236,29 @@ def test_mixed_adjoint_clinical_field_and_unsupported_cases():
     dq = engines.DeviceProblem(p)
     with pytest.raises(RuntimeError, match="unsupported"):
         planning.adjoint(dq, torch.zeros(dq.shape, dtype=torch.float32, device=dq.dev), precision="bf16x3")
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16"])
+def test_adjoint_does_not_depend_on_the_split(precision):
+    """dJ/dw is the same, bit for bit, for every layer split 1-16 (powers of two or
+    not: each (slice, layer) partial is computed by one workgroup in a fixed order,
+    and the depth reduction adds them in increasing z), and the automatic split is
+    the chooser's."""
+    from pbdose import _lib, engines, planning
+    print(f"case adjoint split independence {precision}", flush=True)
+    p = small_problem(nx=64, ny=48, nz=60, d=0.2, nsx=13, nsy=11, ds=0.5,
+                      energies=tuple(np.linspace(70.0, 150.0, 12)))
+    dp = engines.DeviceProblem(p)
+    R = torch.as_tensor(np.random.default_rng(4).standard_normal(p.grid.shape), dtype=torch.float32,
+                        device=dp.dev)
+    ref = planning.adjoint(dp, R, split=1, precision=precision)
+    for split in (2, 3, 5, 7, 8, 10, 16):
+        g = planning.adjoint(dp, R, split=split, precision=precision)
+        assert torch.equal(g, ref), f"split {split}"
+    auto = dp.auto_split(_lib.PREC[precision])
+    assert 1 <= auto <= 16
+    assert torch.equal(planning.adjoint(dp, R, precision=precision), ref)
+    if precision == "fp32":
+        gw1, gi1, gs1 = planning.adjoint(dp, R, split=1, lut_grads=True)
+        gw5, gi5, gs5 = planning.adjoint(dp, R, split=5, lut_grads=True)
+        assert torch.equal(gw5, gw1) and torch.equal(gi5, gi1) and torch.equal(gs5, gs1)
