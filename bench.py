@@ -152,16 +152,17 @@ def adjoint_form(precision):
 
 
 def time_adjoint(dp, D, steps, warmup, precision="fp32"):
-    """Objective residual + adjoint (dJ/dw) per call: wall ms per call over `steps`
-    back-to-back calls, ms per call from per-call events, dJ/dw of the last call."""
+    """J and dJ/dw of the weighted-quadratic objective per call (objective residual,
+    adjoint and depth reduction in one pbh_objective_adjoint call): wall ms per call
+    over `steps` back-to-back calls, ms per call from per-call events, dJ/dw of the
+    last call."""
     from pbdose import planning
     Dp, wv = planning.central_cube(dp.problem.grid, device=dp.dev)
     g = None
 
     def call():
         nonlocal g
-        _, R = planning.objective(dp, D, Dp, wv)
-        g = planning.adjoint(dp, R, precision=precision)
+        _, g = planning.objective_gradient(dp, D, Dp, wv, precision=precision)
 
     for _ in range(warmup):
         call()
@@ -243,8 +244,7 @@ def profile_kernels(forms, reps=5):
     D = dp.forward("fp32")
     for aform in sorted({adjoint_form(f) for f in forms} | {"fp32"}):
         for _ in range(reps):
-            _, R = planning.objective(dp, D, Dp, wv)
-            planning.adjoint(dp, R, precision=aform)
+            planning.objective_gradient(dp, D, Dp, wv, precision=aform)
         torch.cuda.synchronize()
         print(f"objective + {aform} adjoint: {reps} calls", flush=True)
     a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)

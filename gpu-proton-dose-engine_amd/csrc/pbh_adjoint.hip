@@ -320,6 +320,29 @@ void adj_fp32_batch_kernel(const AdjArgs* __restrict__ items, const int* __restr
 }
 
 // grad_w[e] = sum_z part[z][e] in FP64, increasing z (8 loads in flight per thread)
+constexpr int OBJ_THREADS = 256;
+
+// Workgroups of the depth reduction for one problem: whole 64-element blocks per layer.
+__host__ __device__ inline int reduce_blocks(size_t n, int per_layer)
+{
+    return (int)(n / (size_t)per_layer) * ((per_layer + RED_ELEMS - 1) / RED_ELEMS);
+}
+
+// out[0] = sum of n partials: each thread sums a fixed stride, then a fixed tree
+__device__ __forceinline__ void sum_partials_body(const double* partial, int n, double* out)
+{
+    __shared__ double red[OBJ_THREADS];
+    double t = 0.0;
+    for (int i = threadIdx.x; i < n; i += OBJ_THREADS) t += partial[i];
+    red[threadIdx.x] = t;
+    __syncthreads();
+    for (int w = OBJ_THREADS / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
 // grad_w[e] = sum_z part[z][e] in FP64 over the slices where element e's layer
 // has IDD != 0: elsewhere its c P partial is exactly zero, so the adjoint kernels
 // neither write nor read it. Workgroup blk takes 64 elements of one layer l
@@ -330,6 +353,10 @@ void adj_fp32_batch_kernel(const AdjArgs* __restrict__ items, const int* __restr
 // on scheduling.
 __device__ __forceinline__ void adj_reduce_body(const BatchRedItem& it, int blk)
 {
+    if (it.j_out && blk == reduce_blocks(it.n, it.per_layer)) {     // the objective's J (uniform)
+        sum_partials_body(it.jpart, it.njpart, it.j_out);
+        return;
+    }
     __shared__ double ws[4][RED_ELEMS];
     __shared__ int zl[256];
     __shared__ int wcnt[4];
@@ -374,12 +401,6 @@ __device__ __forceinline__ void adj_reduce_body(const BatchRedItem& it, int blk)
         it.grad_w[e] = (float)(((ws[0][lane] + ws[1][lane]) + ws[2][lane]) + ws[3][lane]);
 }
 
-// Workgroups of the depth reduction for one problem: whole 64-element blocks per layer.
-__host__ __device__ inline int reduce_blocks(size_t n, int per_layer)
-{
-    return (int)(n / (size_t)per_layer) * ((per_layer + RED_ELEMS - 1) / RED_ELEMS);
-}
-
 __global__ __launch_bounds__(256) void adj_reduce_kernel(BatchRedItem it)
 {
     adj_reduce_body(it, blockIdx.x);
@@ -393,7 +414,6 @@ __global__ __launch_bounds__(256) void adj_reduce_batch_kernel(const BatchRedIte
 }
 
 // ---- weighted quadratic objective: R = 2 wv (D - Dp), J = sum wv (D - Dp)^2 ----
-constexpr int OBJ_THREADS = 256;
 
 // Workgroup blk of nblk: grid-stride loop over the voxels, partial[blk] = its sum.
 __device__ __forceinline__ void objective_body(const float* D, const float* Dp, const float* wv, size_t n,
@@ -440,21 +460,6 @@ __global__ __launch_bounds__(OBJ_THREADS) void objective_kernel(const float* D, 
                                                                 size_t n, float* R, double* partial)
 {
     objective_body(D, Dp, wv, n, R, partial, blockIdx.x, gridDim.x);
-}
-
-// out[0] = sum of n partials: each thread sums a fixed stride, then a fixed tree
-__device__ __forceinline__ void sum_partials_body(const double* partial, int n, double* out)
-{
-    __shared__ double red[OBJ_THREADS];
-    double t = 0.0;
-    for (int i = threadIdx.x; i < n; i += OBJ_THREADS) t += partial[i];
-    red[threadIdx.x] = t;
-    __syncthreads();
-    for (int w = OBJ_THREADS / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[0] = red[0];
 }
 
 __global__ __launch_bounds__(OBJ_THREADS) void sum_partials_kernel(const double* partial, int n, double* out)
@@ -618,6 +623,54 @@ int pbh_make_adj_args(const PbhGrid* g, const PbhLattice* lat, const float* w, c
     return PBH_OK;
 }
 
+// Adjoint arguments of one problem with every check that precedes a launch: the
+// split, and for the BF16 kernels the table geometry and the layers per workgroup.
+static int prepare_adjoint(const PbhGrid* g, const PbhLattice* lat, const float* w, const double* idd,
+                           const double* sigma, const float* R, float* part, double* grad_idd,
+                           double* grad_sigma, int precision, int split, AdjArgs* A)
+{
+    int rc = pbh_make_adj_args(g, lat, w, idd, sigma, R, part, grad_idd, grad_sigma, split, A);
+    if (rc != PBH_OK) return rc;
+    const bool mixed = precision == PBH_PREC_BF16 || precision == PBH_PREC_BF16X3;
+    // the BF16 adjoint gathers every factor from tables and keeps at most 128 layers per workgroup
+    if (mixed && (!A->tab || !pbh_adj_mixed_fits(*A))) return PBH_ERR_UNSUPPORTED;
+    return PBH_OK;
+}
+
+// The adjoint kernels and the depth reduction of one problem (arguments from
+// prepare_adjoint); with j_out, the reduction's launch also sums the njpart
+// objective partials.
+static int adjoint_launch(const PbhGrid* g, const PbhLattice* lat, const AdjArgs& A, const double* idd,
+                          float* grad_w, double* grad_sigma, int precision, const double* jpart, int njpart,
+                          double* j_out, hipStream_t s)
+{
+    const bool mixed = precision == PBH_PREC_BF16 || precision == PBH_PREC_BF16X3;
+    int rc;
+    const size_t n = (size_t)lat->n_layers * lat->n_sy * lat->n_sx;
+    if (mixed) {
+        if ((rc = pbh_launch_adj_mixed(A, precision, g->nz, s)) != PBH_OK) return rc;
+    } else {
+        const dim3 grid((unsigned)(g->nz * A.split)), block(AJ_THREADS);
+        const bool sig = grad_sigma != nullptr;
+        if (pbh_adj_mixed_q5(A)) {
+            if (sig) hipLaunchKernelGGL((adj_fp32_kernel<true, true, 5>), grid, block, 0, s, A);
+            else hipLaunchKernelGGL((adj_fp32_kernel<true, false, 5>), grid, block, 0, s, A);
+        } else if (A.tab) {
+            if (sig) hipLaunchKernelGGL((adj_fp32_kernel<true, true, 0>), grid, block, 0, s, A);
+            else hipLaunchKernelGGL((adj_fp32_kernel<true, false, 0>), grid, block, 0, s, A);
+        } else {
+            if (sig) hipLaunchKernelGGL((adj_fp32_kernel<false, true, 0>), grid, block, 0, s, A);
+            else hipLaunchKernelGGL((adj_fp32_kernel<false, false, 0>), grid, block, 0, s, A);
+        }
+        if ((rc = launch_rc("adj_fp32_kernel")) != PBH_OK) return rc;
+    }
+    const int per_layer = lat->n_sy * lat->n_sx;
+    const int blocks = reduce_blocks(n, per_layer) + (j_out ? 1 : 0);
+    hipLaunchKernelGGL(adj_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, s,
+                       BatchRedItem{A.part, idd, grad_w, n, g->nz, per_layer, jpart, njpart, j_out});
+    return launch_rc("adj_reduce_kernel");
+}
+
 extern "C" int pbh_adjoint_tuned(const PbhGrid* g, const PbhLattice* lat, const float* w, const double* idd,
                                  const double* sigma, const float* R, float* grad_w, double* grad_idd,
                                  double* grad_sigma, int precision, int split, void* workspace,
@@ -631,34 +684,11 @@ extern "C" int pbh_adjoint_tuned(const PbhGrid* g, const PbhLattice* lat, const 
     if (mixed && (grad_idd || grad_sigma)) return PBH_ERR_UNSUPPORTED;   // table gradients: FP32 only
     if (split < 0) return PBH_ERR_BAD_DIM;
     if (workspace_bytes < pbh_adjoint_workspace_bytes(g, lat, precision)) return PBH_ERR_WORKSPACE;
-    hipStream_t s = (hipStream_t)stream;
     AdjArgs A;
-    rc = pbh_make_adj_args(g, lat, w, idd, sigma, R, (float*)workspace, grad_idd, grad_sigma, split, &A);
-    if (rc != PBH_OK) return rc;
-    const size_t n = (size_t)lat->n_layers * lat->n_sy * lat->n_sx;
-    if (mixed) {
-        if (!A.tab) return PBH_ERR_UNSUPPORTED;   // the BF16 adjoint gathers every factor from tables
-        if ((rc = pbh_launch_adj_mixed(A, precision, g->nz, s)) != PBH_OK) return rc;
-        hipLaunchKernelGGL(adj_reduce_kernel, dim3((unsigned)reduce_blocks(n, lat->n_sy * lat->n_sx)), dim3(256), 0,
-                           s, BatchRedItem{A.part, idd, grad_w, n, g->nz, lat->n_sy * lat->n_sx});
-        return launch_rc("adj_reduce_kernel");
-    }
-    const dim3 grid((unsigned)(g->nz * A.split)), block(AJ_THREADS);
-    const bool sig = grad_sigma != nullptr;
-    if (pbh_adj_mixed_q5(A)) {
-        if (sig) hipLaunchKernelGGL((adj_fp32_kernel<true, true, 5>), grid, block, 0, s, A);
-        else hipLaunchKernelGGL((adj_fp32_kernel<true, false, 5>), grid, block, 0, s, A);
-    } else if (A.tab) {
-        if (sig) hipLaunchKernelGGL((adj_fp32_kernel<true, true, 0>), grid, block, 0, s, A);
-        else hipLaunchKernelGGL((adj_fp32_kernel<true, false, 0>), grid, block, 0, s, A);
-    } else {
-        if (sig) hipLaunchKernelGGL((adj_fp32_kernel<false, true, 0>), grid, block, 0, s, A);
-        else hipLaunchKernelGGL((adj_fp32_kernel<false, false, 0>), grid, block, 0, s, A);
-    }
-    if ((rc = launch_rc("adj_fp32_kernel")) != PBH_OK) return rc;
-    hipLaunchKernelGGL(adj_reduce_kernel, dim3((unsigned)reduce_blocks(n, lat->n_sy * lat->n_sx)), dim3(256), 0, s,
-                       BatchRedItem{A.part, idd, grad_w, n, g->nz, lat->n_sy * lat->n_sx});
-    return launch_rc("adj_reduce_kernel");
+    if ((rc = prepare_adjoint(g, lat, w, idd, sigma, R, (float*)workspace, grad_idd, grad_sigma, precision, split,
+                              &A)) != PBH_OK)
+        return rc;
+    return adjoint_launch(g, lat, A, idd, grad_w, grad_sigma, precision, nullptr, 0, nullptr, (hipStream_t)stream);
 }
 
 extern "C" int pbh_adjoint(const PbhGrid* g, const PbhLattice* lat, const float* w, const double* idd,
@@ -692,6 +722,43 @@ extern "C" int pbh_objective(const PbhGrid* g, const float* dose, const float* p
     if (rc != PBH_OK) return rc;
     hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(OBJ_THREADS), 0, s, (const double*)part, OBJ_BLOCKS, j_out);
     return launch_rc("sum_partials_kernel");
+}
+
+// Workspace of pbh_objective_adjoint: the adjoint's per-slice partials, then R
+// (nz ny nx FP32), then the OBJ_BLOCKS partial sums of J.
+static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+extern "C" size_t pbh_objective_adjoint_workspace_bytes(const PbhGrid* g, const PbhLattice* lat, int precision)
+{
+    const size_t part = pbh_adjoint_workspace_bytes(g, lat, precision);
+    if (part == 0) return 0;
+    return align_up(part) + align_up((size_t)g->nx * g->ny * g->nz * sizeof(float)) + OBJ_BLOCKS * sizeof(double);
+}
+
+extern "C" int pbh_objective_adjoint(const PbhGrid* g, const PbhLattice* lat, const float* w, const double* idd,
+                                     const double* sigma, const float* dose, const float* presc,
+                                     const float* vox_w, float* grad_w, double* j_out, int precision, int split,
+                                     void* workspace, size_t workspace_bytes, void* stream)
+{
+    int rc = pbh_validate(g, lat);
+    if (rc != PBH_OK) return rc;
+    if (!w || !idd || !sigma || !dose || !presc || !grad_w || !j_out || !workspace) return PBH_ERR_NULL_ARG;
+    const bool mixed = precision == PBH_PREC_BF16 || precision == PBH_PREC_BF16X3;
+    if (precision != PBH_PREC_FP32 && !mixed) return PBH_ERR_UNSUPPORTED;
+    if (split < 0) return PBH_ERR_BAD_DIM;
+    if (workspace_bytes < pbh_objective_adjoint_workspace_bytes(g, lat, precision)) return PBH_ERR_WORKSPACE;
+    hipStream_t s = (hipStream_t)stream;
+    char* ws = (char*)workspace;
+    float* part = (float*)ws;
+    float* R = (float*)(ws + align_up(pbh_adjoint_workspace_bytes(g, lat, precision)));
+    const size_t nv = (size_t)g->nx * g->ny * g->nz;
+    double* jpart = (double*)((char*)R + align_up(nv * sizeof(float)));
+    AdjArgs A;
+    if ((rc = prepare_adjoint(g, lat, w, idd, sigma, R, part, nullptr, nullptr, precision, split, &A)) != PBH_OK)
+        return rc;
+    hipLaunchKernelGGL(objective_kernel, dim3(OBJ_BLOCKS), dim3(OBJ_THREADS), 0, s, dose, presc, vox_w, nv, R, jpart);
+    if ((rc = launch_rc("objective_kernel")) != PBH_OK) return rc;
+    return adjoint_launch(g, lat, A, idd, grad_w, nullptr, precision, jpart, OBJ_BLOCKS, j_out, s);
 }
 
 extern "C" int pbh_lut_chain(const double* grad_idd, const double* grad_sigma, const double* didd,

@@ -38,6 +38,11 @@ struct BatchRedItem {
     size_t n;
     int nz;
     int per_layer;         // nsy * nsx
+    // optional: one more workgroup (after the reduction's own) writes j_out[0] = the
+    // fixed-order sum of the njpart objective partials jpart (null j_out: none)
+    const double* jpart;
+    int njpart;
+    double* j_out;
 };
 
 constexpr int OBJ_BATCH_BLOCKS = 256;   // objective workgroups per item

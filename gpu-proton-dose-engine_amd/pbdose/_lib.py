@@ -77,6 +77,9 @@ SIGNATURES = {
     "pbh_adjoint_choose_split": (C.c_int, [_G, _L, C.c_int, _P, C.c_int]),
     "pbh_objective_workspace_bytes": (C.c_size_t, [_G]),
     "pbh_objective": (C.c_int, [_G, _P, _P, _P, _P, _P, _P, C.c_size_t, _P]),
+    "pbh_objective_adjoint_workspace_bytes": (C.c_size_t, [_G, _L, C.c_int]),
+    "pbh_objective_adjoint": (C.c_int, [_G, _L, _P, _P, _P, _P, _P, _P, _P, _P, C.c_int, C.c_int, _P,
+                                        C.c_size_t, _P]),
     "pbh_lut_chain": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_int, _P, _P]),
     "pbh_batch_workspace_bytes": (C.c_size_t, [C.POINTER(PbhBatchItem), C.c_int, C.c_int,
                                                C.POINTER(PbhBatchTuning)]),

@@ -107,6 +107,8 @@ class DeviceProblem:
                 n = L.pbh_forward_workspace_bytes_tuned(self.grid_s, self.lat_s, precision, layer_chunk)
             elif kind == "obj":
                 n = L.pbh_objective_workspace_bytes(self.grid_s)
+            elif kind == "objadj":
+                n = L.pbh_objective_adjoint_workspace_bytes(self.grid_s, self.lat_s, precision)
             else:
                 n = L.pbh_adjoint_workspace_bytes(self.grid_s, self.lat_s, precision)
             self._ws[key] = _lib.workspace(n, self.dev)

@@ -224,6 +224,19 @@ int pbh_objective(const PbhGrid* g, const float* dose, const float* presc,
                   const float* vox_w, float* R, double* j_out,
                   void* workspace, size_t workspace_bytes, void* stream);
 
+/* pbh_objective followed by pbh_adjoint_tuned without table gradients, in one
+ * call: J = sum wv (dose - presc)^2 into j_out[0] (FP64) and grad_w = dJ/dw.
+ * The residual R stays in the workspace, and J is summed by one more workgroup
+ * of the adjoint's depth-reduction launch, one kernel launch fewer than the two
+ * calls. Results equal the two calls' bit for bit. No counterpart in the
+ * reference (its engines are forward-only). */
+size_t pbh_objective_adjoint_workspace_bytes(const PbhGrid* g, const PbhLattice* lat, int precision);
+int pbh_objective_adjoint(const PbhGrid* g, const PbhLattice* lat,
+                          const float* w, const double* idd, const double* sigma,
+                          const float* dose, const float* presc, const float* vox_w,
+                          float* grad_w, double* j_out, int precision, int split,
+                          void* workspace, size_t workspace_bytes, void* stream);
+
 /* out[l] = sum_z (grad_idd[l,z] didd[l,z] + grad_sigma[l,z] dsigma[l,z]) for
  * (L, nz) FP64 arrays: the chain rule from the look-up tables to a per-layer
  * parameter (e.g. a range shift, with didd/dsigma the tables' derivatives). */

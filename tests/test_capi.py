@@ -288,3 +288,21 @@ def test_adjoint_split_chooser():
     # fixed work once the slots are full
     full = np.full(128, 40, np.int32)
     assert 1 <= L.pbh_adjoint_choose_split(gs, ls, P["fp32"], full.ctypes.data, 8) <= 2
+
+
+def test_objective_adjoint_rejects_bad_arguments_without_launching():
+    L = _lib.lib()
+    g = _lib.PbhGrid(128, 128, 128, 0, 0.2, 0.2, 0.2, -12.7, -12.7, 0.0)
+    lat = _lib.PbhLattice(41, 41, 40, 0, 0.5, 0.5, 0.0, 0.0)
+    nul, one = C.c_void_p(0), C.c_void_p(256)
+    P = _lib.PREC
+    nb = L.pbh_objective_adjoint_workspace_bytes(g, lat, P["bf16x3"])
+    # the adjoint's partials, R and the partial sums of J
+    assert nb >= L.pbh_adjoint_workspace_bytes(g, lat, P["bf16x3"]) + 128 ** 3 * 4 + 1024 * 8
+    args = lambda jout, prec, split, nbytes: (g, lat, one, one, one, one, one, nul, one, jout, prec, split,
+                                              one, nbytes, nul)
+    assert L.pbh_objective_adjoint(*args(nul, P["bf16x3"], 0, nb)) == -1          # no J output
+    assert L.pbh_objective_adjoint(*args(one, P["fp16"], 0, nb)) == -5            # no FP16 adjoint
+    assert L.pbh_objective_adjoint(*args(one, P["bf16x3"], -1, nb)) == -2         # negative split
+    assert L.pbh_objective_adjoint(*args(one, P["bf16x3"], 0, nb - 1)) == -7      # workspace too small
+    assert L.pbh_objective_adjoint(*args(one, P["bf16x3"], 17, nb)) == -5         # split above 16

@@ -262,3 +262,24 @@ def test_adjoint_does_not_depend_on_the_split(precision):
         gw1, gi1, gs1 = planning.adjoint(dp, R, split=1, lut_grads=True)
         gw5, gi5, gs5 = planning.adjoint(dp, R, split=5, lut_grads=True)
         assert torch.equal(gw5, gw1) and torch.equal(gi5, gi1) and torch.equal(gs5, gs1)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_objective_gradient_equals_objective_then_adjoint(precision):
+    """pbh_objective_adjoint gives J and dJ/dw bit for bit as pbh_objective followed
+    by pbh_adjoint_tuned, on repeated calls (the workspace is reused), with and
+    without voxel weights."""
+    from pbdose import engines, planning
+    print(f"case fused objective + adjoint {precision}", flush=True)
+    p = small_problem(nx=64, ny=48, nz=40, d=0.2, nsx=13, nsy=11, ds=0.5,
+                      energies=tuple(np.linspace(70.0, 130.0, 8)))
+    dp = engines.DeviceProblem(p)
+    D = dp.forward()
+    Dp, wv = planning.central_cube(p.grid, half=1.5, dose_value=float(D.max()) * 0.5)
+    for vw in (wv, None):
+        J0, R = planning.objective(dp, D, Dp, vw)
+        g0 = planning.adjoint(dp, R, precision=precision)
+        for _ in range(2):
+            J1, g1 = planning.objective_gradient(dp, D, Dp, vw, precision=precision)
+            assert float(J1) == float(J0)
+            assert torch.equal(g1, g0)
