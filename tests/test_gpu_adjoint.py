@@ -64,6 +64,33 @@ def test_adjoint_matches_oracle(shape):
     assert ew <= 2e-5 and ei <= 2e-5 and es <= 5e-5
 
 
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_adjoint_two_to_three_pitch(precision):
+    """2 mm voxels and 3 mm spots: parity-interleaved tables with P = 2, Q = 3 (run-time
+    slots, odd entries at the bank offset). FP32 with table gradients; BF16x3 dJ/dw
+    within the mixed adjoint's bound."""
+    from pbdose import engines, planning
+    print(f"case adjoint 2 mm / 3 mm {precision}", flush=True)
+    p = small_problem(nx=80, ny=56, nz=20, d=0.2, nsx=23, nsy=15, ds=0.3)
+    dp = engines.DeviceProblem(p)
+    R = np.random.default_rng(6).standard_normal(p.grid.shape)
+    Rt = torch.as_tensor(R, dtype=torch.float32, device=dp.dev)
+    x, y, sx, sy = p.grid.x, p.grid.y, p.lattice.spot_x, p.lattice.spot_y
+    Rf = R.astype(np.float32).astype(np.float64)
+    ref_w = O.adjoint_w(x, y, sx, sy, p.idd, p.sigma, Rf)
+    if precision == "fp32":
+        gw, gi, gs = planning.adjoint(dp, Rt, lut_grads=True)
+        ref_i, ref_s = O.adjoint_lut(x, y, sx, sy, p.lattice.weights, p.idd, p.sigma, Rf)
+        ew, ei, es = rel(gw.cpu(), ref_w), rel(gi.cpu(), ref_i), rel(gs.cpu(), ref_s)
+        print(f"  dJ/dw {ew:.2e}  dJ/dIDD {ei:.2e}  dJ/dsigma {es:.2e}", flush=True)
+        assert ew <= 2e-5 and ei <= 2e-5 and es <= 5e-5
+    else:
+        gw = planning.adjoint(dp, Rt, precision=precision)
+        ew = rel(gw.cpu(), ref_w)
+        print(f"  dJ/dw {ew:.2e}", flush=True)
+        assert ew <= MIXED_ADJ_TOL[precision]
+
+
 def test_adjoint_direct_factor_path():
     from pbdose import engines, planning
     print("case adjoint with direct factors (0.37 cm spots)", flush=True)

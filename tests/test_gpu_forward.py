@@ -159,6 +159,14 @@ def test_direct_factor_path(precision):
     check(p, tol=TOL_FP32 if precision == "fp32" else MIXED_TOL[precision], precision=precision)
 
 
+def test_two_to_three_pitch_grid():
+    # 2 mm voxels, 3 mm spots: parity-interleaved tables with P = 2, Q = 3 (run-time
+    # slots), several spot chunks in x
+    print("case 2 mm voxels / 3 mm spots, FP32", flush=True)
+    p = make_problem(160, 72, 6, 0.2, 61, 21, 0.3, [85.0, 120.0, 150.0], seed=41)
+    check(p)
+
+
 def test_quarter_pitch_grid():
     # 2.5 mm voxels, 5 mm spots: table with P = 1, Q = 2
     print("case 2.5 mm voxels / 5 mm spots", flush=True)
