@@ -1,7 +1,7 @@
 """Host-CPU restatement of the reference's PyTorch batched-matmul engine.
 
-TEST / BASELINE INFRASTRUCTURE ONLY: bench.py times it as the reported CPU
-baseline ("port"); the shipped package never imports it.
+TEST INFRASTRUCTURE ONLY: tests use it as a checker, and bench.py times it as the
+reported CPU comparison ("port"); the shipped package never imports it.
 
 Same op sequence as engines.py:256-274 (per_layer=True): broadcast exp
 factors Gx, Gy for every (layer, depth), A = IDD x w, T = bmm(A^T, Gy^T),
