@@ -428,10 +428,15 @@ static size_t ws_wsplit(const FwdArgs& a)
     return align256((size_t)a.L * chunk_count(a.nsy) * chunk_count(a.nsx) * FT_WSPLIT * sizeof(unsigned short));
 }
 
+static size_t ws_layers(const FwdArgs& a)
+{
+    return align256((size_t)a.nz * sizeof(int)) + align256((size_t)a.nz * a.L * sizeof(LayerEntry));
+}
+
 size_t pbh_forward_ws_bytes(const FwdArgs& a, int layer_chunk)
 {
     const int chunk = layer_chunk > 0 ? layer_chunk : pbh_auto_chunk(a);
-    size_t n = ws_peaks(a) + ws_wsplit(a);
+    size_t n = ws_peaks(a) + ws_wsplit(a) + ws_layers(a);
     n += (size_t)(nchunks(a, chunk) - 1) * tile_count(a) * FT_TY * FT_TX * sizeof(float);
     return n;
 }
@@ -457,11 +462,16 @@ int pbh_plan_forward(const FwdArgs& a, int kx_pad, int ky_pad, void* ws, size_t 
     }
     plan->part = nullptr;
     plan->wsplit = nullptr;
+    plan->layers = nullptr;
+    plan->nlayers = nullptr;
     const size_t need = pbh_forward_ws_bytes(a, chunk);
     if (ws && ws_bytes >= need) {
         char* p = (char*)ws;
         plan->wsplit = (unsigned short*)(p + ws_peaks(a));
-        if (plan->nchunk > 1) plan->part = (float*)(p + ws_peaks(a) + ws_wsplit(a));
+        char* lp = p + ws_peaks(a) + ws_wsplit(a);
+        plan->nlayers = (int*)lp;
+        plan->layers = (LayerEntry*)(lp + align256((size_t)a.nz * sizeof(int)));
+        if (plan->nchunk > 1) plan->part = (float*)(lp + ws_layers(a));
     }
     if (plan->nchunk > 1) {
         if (!ws) return PBH_ERR_NULL_ARG;

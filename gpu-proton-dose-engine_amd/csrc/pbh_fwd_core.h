@@ -57,6 +57,13 @@ __host__ __device__ constexpr int table_slots(int n)
 // partials to the dose in chunk order, so results are deterministic and no
 // workgroup waits for another. Chunks keep every workgroup to a bounded amount
 // of work although shallow slices have many more active layers than deep ones.
+// One active layer of a depth slice, as the mixed forward uses it: layer index,
+// c = IDD / (2 pi s^2), k2 = log2(e) / (2 s^2), window half-width FT_WIN_SIGMAS s (cm).
+struct LayerEntry {
+    int lay;
+    float c, k2, cut;
+};
+
 struct FwdPlan {
     int chunk;          // at most this many active layers per workgroup (>= 1)
     int nchunk;         // workgroups per dose tile: ceil(L / chunk)
@@ -64,6 +71,10 @@ struct FwdPlan {
     AxisTable tx, ty;
     float* part;        // [nchunk - 1][tiles][FT_TY * FT_TX] partial tiles (nchunk > 1)
     unsigned short* wsplit;   // mixed forms: [L][chunk y][chunk x][FT_WSPLIT] split weights
+    // mixed forms: the active layers of every slice in layer order, [nz][L] (first
+    // nlayers[z] of row z), written by the weight-split launch
+    LayerEntry* layers;
+    int* nlayers;
 };
 
 // Table entry t of an axis (t = Q i - P a + P (T - 1)) for base offset C:
@@ -307,6 +318,26 @@ __device__ inline int collect_layers(const FwdArgs& a, int lb, int z, float zc,
     }
     __syncthreads();
     return total;
+}
+
+// Active layers lb .. lb + FT_LB - 1 (in active order) of depth slice z from the
+// precomputed lists into ll; returns their count. Contains one barrier: every
+// thread of the workgroup must call it.
+__device__ inline int copy_layers(const FwdPlan& pl, int L, int z, int lb, float zc, const float* r_peak,
+                                  LayerList& ll)
+{
+    const int n = min(FT_LB, pl.nlayers[z] - lb);
+    const int t = threadIdx.x;
+    if (t < n) {
+        const LayerEntry e = pl.layers[(size_t)z * L + lb + t];
+        ll.lay[t] = e.lay;
+        ll.c[t] = e.c;
+        ll.k2[t] = e.k2;
+        ll.cut[t] = e.cut;
+        ll.zpk[t] = r_peak ? fabsf(zc - r_peak[e.lay]) : 0.0f;
+    }
+    __syncthreads();
+    return n;
 }
 
 // Padded spot-chunk width (a multiple of 4, at most PBH_SPOT_CHUNK) for a lattice.

@@ -79,17 +79,51 @@ __device__ __forceinline__ void wsplit_body(const FwdArgs& a, unsigned short* ds
     lo[p] = (unsigned)bits16(r0) | ((unsigned)bits16(r1) << 16);
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void wsplit_kernel(FwdArgs a, unsigned short* dst)
+// The active layers of every depth slice, in layer order, with their scalars
+// (the values collect_layers forms): one wave per slice, its lanes taking 64
+// layers at a time (ballot compaction); slices 4 blk + wave with a stride of 4 nblk.
+__device__ __forceinline__ void layer_lists_body(const FwdArgs& a, const FwdPlan& pl, int blk, int nblk)
 {
-    wsplit_body<T>(a, dst, blockIdx.x, blockIdx.y);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int z = 4 * blk + wave; z < a.nz; z += 4 * nblk) {
+        LayerEntry* row = pl.layers + (size_t)z * a.L;
+        int n = 0;
+        for (int lb = 0; lb < a.L; lb += 64) {
+            const int l = lb + lane;
+            double iddv = 0.0, sg = 0.0;
+            if (l < a.L) {
+                iddv = a.idd[(size_t)l * a.nz + z];
+                sg = a.sigma[(size_t)l * a.nz + z];
+            }
+            const bool act = iddv != 0.0;
+            const unsigned long long m = __ballot(act);
+            if (act) {
+                const LayerScalars sc = layer_scalars(iddv, sg);
+                const double sd = sg > kSigmaFloor ? sg : kSigmaFloor;
+                row[n + __popcll(m & ((1ull << lane) - 1ull))] = LayerEntry{l, sc.c, sc.k2, (float)(FT_WIN_SIGMAS * sd)};
+            }
+            n += __popcll(m);
+        }
+        if (lane == 0) pl.nlayers[z] = n;
+    }
+}
+
+// Grid (chunks, WS_PARTS + 1): rows 0 .. WS_PARTS-1 split the weights, the last
+// row writes the layer lists of the slices.
+template <typename T>
+__global__ __launch_bounds__(256) void wsplit_kernel(FwdArgs a, FwdPlan pl)
+{
+    if (blockIdx.y == WS_PARTS) layer_lists_body(a, pl, blockIdx.x, gridDim.x);
+    else wsplit_body<T>(a, pl.wsplit, blockIdx.x, blockIdx.y);
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void wsplit_batch_kernel(const BatchFwdItem* __restrict__ items)
 {
     const FwdArgs a = items[blockIdx.z].a;
-    wsplit_body<T>(a, items[blockIdx.z].pl.wsplit, blockIdx.x, blockIdx.y);
+    const FwdPlan& pl = items[blockIdx.z].pl;
+    if (blockIdx.y == WS_PARTS) layer_lists_body(a, pl, blockIdx.x, gridDim.x);
+    else wsplit_body<T>(a, pl.wsplit, blockIdx.x, blockIdx.y);
 }
 
 // 4 waves (two workgroups per CU), one-barrier software pipeline over the
@@ -161,8 +195,12 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
 
     const int lo = ch * pl.chunk, hi = lo + pl.chunk;   // this workgroup's active layers
     int base = 0;                                       // active layers of earlier passes
-    for (int lb = 0; lb < a.L && base < hi; lb += FT_LB) {
-        const int nact = collect_layers(a, lb, z, zc, r_peak, ll);
+    // passes over 64 layers (their active ones), or over 64 active layers of the
+    // slice's precomputed list
+    const int npass = pl.layers ? pl.nlayers[z] : a.L;
+    for (int lb = 0; lb < npass && base < hi; lb += FT_LB) {
+        const int nact = pl.layers ? copy_layers(pl, a.L, z, lb, zc, r_peak, ll)
+                                   : collect_layers(a, lb, z, zc, r_peak, ll);
         const int m0 = max(0, lo - base), m1 = min(nact, hi - base);
         base += nact;
         if (m1 <= m0) continue;
@@ -573,11 +611,12 @@ int pbh_forward_mixed(const FwdArgs& a, double oz, double dz, int precision, int
                            a.idd, a.L, a.nz, oz, dz, r_peak);
         if ((rc = launch_rc("peak_depth_f32_kernel")) != PBH_OK) return rc;
     }
-    const dim3 wgrid((unsigned)(a.L * chunk_count(a.nsy) * chunk_count(a.nsx)), WS_PARTS);
+    if (!pl.wsplit || !pl.layers) return PBH_ERR_WORKSPACE;
+    const dim3 wgrid((unsigned)(a.L * chunk_count(a.nsy) * chunk_count(a.nsx)), WS_PARTS + 1);
     if (is_f16(precision))
-        hipLaunchKernelGGL(wsplit_kernel<_Float16>, wgrid, dim3(256), 0, s, a, pl.wsplit);
+        hipLaunchKernelGGL(wsplit_kernel<_Float16>, wgrid, dim3(256), 0, s, a, pl);
     else
-        hipLaunchKernelGGL(wsplit_kernel<__bf16>, wgrid, dim3(256), 0, s, a, pl.wsplit);
+        hipLaunchKernelGGL(wsplit_kernel<__bf16>, wgrid, dim3(256), 0, s, a, pl);
     if ((rc = launch_rc("wsplit_kernel")) != PBH_OK) return rc;
     if ((rc = dispatch_precision(precision, mixed_key(pl, kb), SingleLaunch{a, pl, oz, dz, r_peak, s})) != PBH_OK)
         return rc;
@@ -593,7 +632,7 @@ int pbh_launch_peak_batch(const BatchPeakItem* items, int n, hipStream_t s)
 // Split weights of every item of a batch (grid: max chunks x items).
 int pbh_launch_wsplit_batch(const BatchFwdItem* items, int n, int max_chunks, int precision, hipStream_t s)
 {
-    const dim3 grid((unsigned)max_chunks, WS_PARTS, (unsigned)n);
+    const dim3 grid((unsigned)max_chunks, WS_PARTS + 1, (unsigned)n);
     if (is_f16(precision))
         hipLaunchKernelGGL(wsplit_batch_kernel<_Float16>, grid, dim3(256), 0, s, items);
     else
