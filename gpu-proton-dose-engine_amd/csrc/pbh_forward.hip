@@ -558,6 +558,8 @@ extern "C" int pbh_forward_tuned(const PbhGrid* g, const PbhLattice* lat, const 
     rc = pbh_plan_forward(a, chunk_pad(a.nsx, a.nsy), chunk_pad(a.nsx, a.nsy), workspace, workspace_bytes,
                           layer_chunk, &pl);
     if (rc != PBH_OK) return rc;
+    pl.layers = nullptr;            // the layer lists are written by the mixed forms' split launch only
+    pl.nlayers = nullptr;
     const dim3 grid((unsigned)(tile_count(a) * pl.nchunk));
     const bool q5 = fp32_q5(pl);
     switch (chunk_pad(a.nsx, a.nsy)) {

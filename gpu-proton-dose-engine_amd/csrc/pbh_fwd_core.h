@@ -193,7 +193,7 @@ __device__ inline void combine_body(const FwdArgs& a, const FwdPlan& pl, int til
     const int tx = tile % a.tiles_x;
     const int ty = (tile / a.tiles_x) % a.tiles_y;
     const int z = tile / (a.tiles_x * a.tiles_y);
-    const int nact = count_active(a, z, cnt);
+    const int nact = pl.nlayers ? pl.nlayers[z] : count_active(a, z, cnt);
     const int nch = nact > pl.chunk ? (nact + pl.chunk - 1) / pl.chunk : 1;
     if (nch <= 1) return;
     const float* part0 = pl.part + (size_t)tile * (FT_TY * FT_TX);

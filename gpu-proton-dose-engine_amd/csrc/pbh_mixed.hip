@@ -121,9 +121,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void wsplit_batch_kernel(const BatchFwdItem* __restrict__ items)
 {
     const FwdArgs a = items[blockIdx.z].a;
-    const FwdPlan& pl = items[blockIdx.z].pl;
-    if (blockIdx.y == WS_PARTS) layer_lists_body(a, pl, blockIdx.x, gridDim.x);
-    else wsplit_body<T>(a, pl.wsplit, blockIdx.x, blockIdx.y);
+    wsplit_body<T>(a, items[blockIdx.z].pl.wsplit, blockIdx.x, blockIdx.y);
 }
 
 // 4 waves (two workgroups per CU), one-barrier software pipeline over the
@@ -133,7 +131,7 @@ __global__ __launch_bounds__(256) void wsplit_batch_kernel(const BatchFwdItem* _
 // w owns the 32 x 32 dose block at columns 32 w; KB K steps of 16 spots.
 //
 // blk: the workgroup's index in the problem's own grid (chunk * tiles + tile).
-template <typename T, int MODE, bool TAB, int KB, int QC>
+template <typename T, int MODE, bool TAB, int KB, int QC, bool LISTS>
 __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& pl, double oz, double dz,
                                                const float* r_peak, int blk)
 {
@@ -195,12 +193,12 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
 
     const int lo = ch * pl.chunk, hi = lo + pl.chunk;   // this workgroup's active layers
     int base = 0;                                       // active layers of earlier passes
-    // passes over 64 layers (their active ones), or over 64 active layers of the
-    // slice's precomputed list
-    const int npass = pl.layers ? pl.nlayers[z] : a.L;
+    // LISTS: passes over 64 active layers of the slice's precomputed list (written by
+    // the weight-split launch); otherwise passes over 64 layers, scanned here
+    const int npass = LISTS ? pl.nlayers[z] : a.L;
     for (int lb = 0; lb < npass && base < hi; lb += FT_LB) {
-        const int nact = pl.layers ? copy_layers(pl, a.L, z, lb, zc, r_peak, ll)
-                                   : collect_layers(a, lb, z, zc, r_peak, ll);
+        const int nact = LISTS ? copy_layers(pl, a.L, z, lb, zc, r_peak, ll)
+                               : collect_layers(a, lb, z, zc, r_peak, ll);
         const int m0 = max(0, lo - base), m1 = min(nact, hi - base);
         base += nact;
         if (m1 <= m0) continue;
@@ -478,10 +476,13 @@ template <typename T, int MODE, bool TAB, int KB, int QC>
 __global__ __launch_bounds__(FT_THREADS, (MixedOcc<MODE, QC>::value))
 void fwd_mixed_kernel(FwdArgs a, FwdPlan pl, double oz, double dz, const float* r_peak)
 {
-    fwd_mixed_body<T, MODE, TAB, KB, QC>(a, pl, oz, dz, r_peak, blockIdx.x);
+    fwd_mixed_body<T, MODE, TAB, KB, QC, true>(a, pl, oz, dz, r_peak, blockIdx.x);
 }
 
-// One launch over a batch of problems sharing the template parameters.
+// One launch over a batch of problems sharing the template parameters. Its
+// workgroups scan their slices' layers (no precomputed lists: with both the item
+// lookup and the copy path the kernel needed more than the 168 VGPRs of 3 workgroups
+// per CU and spilled).
 template <typename T, int MODE, bool TAB, int KB, int QC>
 __global__ __launch_bounds__(FT_THREADS, (MixedOcc<MODE, QC>::value))
 void fwd_mixed_batch_kernel(const BatchFwdItem* __restrict__ items, const int* __restrict__ wg0, int n)
@@ -489,7 +490,7 @@ void fwd_mixed_batch_kernel(const BatchFwdItem* __restrict__ items, const int* _
     const int k = batch_item(wg0, n, blockIdx.x);
     const FwdArgs a = items[k].a;
     const FwdPlan pl = items[k].pl;
-    fwd_mixed_body<T, MODE, TAB, KB, QC>(a, pl, items[k].oz, items[k].dz, items[k].r_peak,
+    fwd_mixed_body<T, MODE, TAB, KB, QC, false>(a, pl, items[k].oz, items[k].dz, items[k].r_peak,
                                          (int)blockIdx.x - wg0[k]);
 }
 
@@ -632,7 +633,7 @@ int pbh_launch_peak_batch(const BatchPeakItem* items, int n, hipStream_t s)
 // Split weights of every item of a batch (grid: max chunks x items).
 int pbh_launch_wsplit_batch(const BatchFwdItem* items, int n, int max_chunks, int precision, hipStream_t s)
 {
-    const dim3 grid((unsigned)max_chunks, WS_PARTS + 1, (unsigned)n);
+    const dim3 grid((unsigned)max_chunks, WS_PARTS, (unsigned)n);
     if (is_f16(precision))
         hipLaunchKernelGGL(wsplit_batch_kernel<_Float16>, grid, dim3(256), 0, s, items);
     else
