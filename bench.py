@@ -47,7 +47,7 @@ import torch  # noqa: E402
 # WRITE_SIZE (KiB per dispatch, summed over the call's kernels) scaled by the same
 # counters on a 256 MiB copy (fetch x 2.000, write x 1.000):
 # profiles/counters_forward_adjoint_128cube_67240spots.md
-TRAFFIC_KIB = {"fp32": (5187.5, 15232.0), "bf16x3": (8813.8, 20896.0), "bf16": (11330.7, 33567.4)}
+TRAFFIC_KIB = {"fp32": (5185.5, 15232.0), "bf16x3": (8600.5, 20943.6), "bf16": (8121.8, 27087.6)}
 CALL_KERNELS = {"fp32": "fwd_fp32_kernel + combine_kernel"}
 FETCH_SCALE, WRITE_SCALE = 2.000, 1.000
 
