@@ -337,7 +337,7 @@ __global__ __launch_bounds__(LUT_THREADS) void idd_prep_kernel(LayerLutArgs a)
     const double h = lin_at(0.0, top, n, 1) - lin_at(0.0, top, n, 0);
     double* dc = a.dcsda + (size_t)l * n;
     double* gk = a.gk + (size_t)l * n;
-    for (int i = threadIdx.x; i < n; i += LUT_THREADS) {
+    for (int i = blockIdx.y * LUT_THREADS + threadIdx.x; i < n; i += gridDim.y * LUT_THREADS) {
         // bin edges zb = linspace(0, top, n + 1); Eb = E(R0 - zb) inside the range, else 0
         const double zb0 = lin_at(0.0, top, n + 1, i), zb1 = lin_at(0.0, top, n + 1, i + 1);
         const double eb0 = zb0 < R0 ? energy_from_range(a.E, a.R, (R0 - zb0) > 0.0 ? R0 - zb0 : 0.0) : 0.0;
@@ -379,10 +379,14 @@ __global__ __launch_bounds__(LUT_THREADS) void idd_prep_kernel(LayerLutArgs a)
 }
 
 // Convolved fine-grid dose at index j: sum_i dcsda[i] g[|j - i|] (one wave).
-__device__ inline double conv_at(const double* dc, const double* gk, int n, int j)
+// Terms with |j - i| > M (M covering 10 sR: the Gaussian below e^-50 of its peak)
+// are dropped. The loop starts at a multiple of 64, so every lane keeps its residue
+// class and the kept terms are added in the same order as over the whole range.
+__device__ inline double conv_at(const double* dc, const double* gk, int n, int j, int M)
 {
+    const int lo = (j - M > 0 ? j - M : 0) & ~63, hi = j + M + 1 < n ? j + M + 1 : n;
     double s = 0.0;
-    for (int i = (threadIdx.x & 63); i < n; i += 64) {
+    for (int i = lo + (threadIdx.x & 63); i < hi; i += 64) {
         const int m = j > i ? j - i : i - j;
         s += dc[i] * gk[m];
     }
@@ -427,7 +431,8 @@ __global__ __launch_bounds__(LUT_THREADS) void idd_eval_kernel(LayerLutArgs a)
     const bool depth = a.m.straggling_depth != 0;
     const double* amp = depth ? a.amp + (size_t)l * n : nullptr;
     const int W = (int)ceil(12.0 * sR / h) + 2;
-    auto conv = [&](int j) { return depth ? conv_depth_at(dc, amp, gk, n, j, W, h) : conv_at(dc, gk, n, j); };
+    const int M = sR > 0.0 ? (int)ceil(10.0 * sR / h) + 1 : 0;
+    auto conv = [&](int j) { return depth ? conv_depth_at(dc, amp, gk, n, j, W, h) : conv_at(dc, gk, n, j, M); };
     const int wv = threadIdx.x >> 6;
     for (int q = wv; q < Z_PER_BLOCK; q += LUT_THREADS / 64) {
         const int k = blockIdx.x * Z_PER_BLOCK + q;
@@ -670,7 +675,8 @@ extern "C" int pbh_build_lut(const PbhBeamModel* m, int nz, double z0, double dz
         hipLaunchKernelGGL(strag_cum_kernel, dim3(n_layers), dim3(LUT_THREADS), 0, s, a);
         if ((rc = launch_rc("strag_cum_kernel")) != PBH_OK) return rc;
     }
-    hipLaunchKernelGGL(idd_prep_kernel, dim3(n_layers), dim3(LUT_THREADS), 0, s, a);
+    // one workgroup per layer and 256 fine bins: every bin is independent
+    hipLaunchKernelGGL(idd_prep_kernel, dim3(n_layers, cdiv(a.n_fine, LUT_THREADS)), dim3(LUT_THREADS), 0, s, a);
     if ((rc = launch_rc("idd_prep_kernel")) != PBH_OK) return rc;
     const dim3 eval_grid(cdiv(nz, Z_PER_BLOCK), n_layers);
     hipLaunchKernelGGL(idd_eval_kernel, eval_grid, dim3(LUT_THREADS), 0, s, a);
