@@ -99,3 +99,8 @@ def test_every_form_passes_2pct_2mm_against_fp64(precision):
     print(f"  pass {r['pass_rate']:.4f}% of {r['n_eval']} voxels, gamma_max {r['gamma_max']:.3e}, "
           f"max rel err {e['max_rel_err_global']:.2e}", flush=True)
     assert r["pass_rate"] == 100.0
+    # error bar of the split-precision forms at this field size: 1e-5 max global
+    # relative error (FP32: 1e-6); the peak-only forms are bounded by gamma alone
+    bar = {"fp32": 1e-6, "bf16x3": 1e-5, "fp16x3": 1e-5}.get(precision)
+    if bar is not None:
+        assert e["max_rel_err_global"] <= bar
