@@ -411,7 +411,10 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                             }
                         }
                     }
-                    v16f p = {};
+                    // without a per-layer scale the products accumulate straight into acc
+                    v16f p;
+                    if (F16) p = v16f{};
+                    else p = acc;
 #pragma unroll
                     for (int ks = 0; ks < KB; ++ks) {
                         if (ks >= nkB) break;
@@ -429,7 +432,7 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
 #pragma unroll
                         for (int r = 0; r < 16; ++r) acc[r] += (r < 8 ? s0 : s1) * p[r];
                     } else {
-                        acc += p;
+                        acc = p;
                     }
                 };
 
