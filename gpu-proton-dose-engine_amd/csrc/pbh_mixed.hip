@@ -22,11 +22,11 @@
 // lane's operands are a base register plus compile-time offsets (QC = 5); other
 // geometries compute the slots at run time (QC = 0).
 //
-// Each layer's contribution is summed in fresh FP32 accumulators and added to
-// the running dose once. FP16 has a narrow exponent range, so U is scaled per
-// row block by a power of two (largest magnitude into [1, 2)) and the inverse
-// scale is applied to the layer's partial when it joins the running dose. BF16
-// shares FP32's exponent range and needs no scaling.
+// BF16 shares FP32's exponent range: stage B accumulates every layer's products
+// straight into the running dose accumulators. FP16 has a narrow exponent range,
+// so U is scaled per row block by a power of two (largest magnitude into [1, 2));
+// each layer's contribution is then summed in fresh FP32 accumulators and joins
+// the running dose with the inverse scale applied.
 #include "pbh_mixed_core.h"
 
 namespace pbh {
