@@ -310,3 +310,28 @@ def test_objective_gradient_equals_objective_then_adjoint(precision):
             J1, g1 = planning.objective_gradient(dp, D, Dp, vw, precision=precision)
             assert float(J1) == float(J0)
             assert torch.equal(g1, g0)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("shape", [((1, 1, 1), (1, 1), (120.0,)), ((48, 40, 1), (9, 7), (90.0, 120.0, 150.0)),
+                                   ((1, 40, 6), (5, 9), (100.0, 130.0)), ((40, 1, 6), (9, 5), (100.0, 130.0))])
+def test_adjoint_degenerate_shapes(shape, precision):
+    """One voxel / spot / layer, one depth slice, one-voxel-wide grids: dJ/dw of a
+    random residual vs the FP64 oracle adjoint."""
+    from pbdose import engines, model, planning
+    (nx, ny, nz), (nsx, nsy), E = shape
+    print(f"case adjoint degenerate {shape[0]} {precision}", flush=True)
+    p = small_problem(seed=6, nx=nx, ny=ny, nz=max(nz, 2), nsx=nsx, nsy=nsy, energies=E)
+    if nz == 1:
+        # the oracle's moments need two depths: tables of the first slice of two
+        g = p.grid
+        g1 = model.DoseGrid(g.nx, g.ny, 1, g.dx, g.dy, g.dz, g.ox, g.oy, g.oz)
+        p = model.PBProblem(g1, p.lattice, p.idd[:, :1].copy(), p.sigma[:, :1].copy())
+    dp = engines.DeviceProblem(p)
+    R = np.random.default_rng(9).standard_normal(dp.shape).astype(np.float32)
+    ref = O.adjoint_w(p.grid.x, p.grid.y, p.lattice.spot_x, p.lattice.spot_y, p.idd, p.sigma,
+                      R.astype(np.float64))
+    g = planning.adjoint(dp, torch.as_tensor(R, device=dp.dev), precision=precision)
+    e = rel(g.cpu(), ref)
+    print(f"  vs oracle {e:.2e}", flush=True)
+    assert e <= (2e-5 if precision == "fp32" else MIXED_ADJ_TOL[precision])

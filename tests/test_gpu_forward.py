@@ -96,6 +96,39 @@ def test_zero_idd_layers_and_zero_weights():
     check(p)
 
 
+# Degenerate shapes at the 2.5 mm voxel / 5 mm spot pitch (the tabulated geometry of
+# the mixed forms): one voxel, spot and layer; one depth slice; one-voxel-wide grids.
+DEGENERATE = {
+    "one_voxel_spot_layer": ((1, 1, 1), (1, 1), [120.0]),
+    "one_slice": ((48, 40, 1), (9, 7), [90.0, 120.0, 150.0]),
+    "one_column": ((1, 40, 6), (5, 9), [100.0, 130.0]),
+    "one_row": ((40, 1, 6), (9, 5), [100.0, 130.0]),
+}
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "bf16x3"])
+@pytest.mark.parametrize("case", list(DEGENERATE))
+def test_degenerate_shapes(case, precision):
+    from pbdose import model
+    (nx, ny, nz), (nsx, nsy), E = DEGENERATE[case]
+    print(f"case degenerate {case} {precision}", flush=True)
+    p = make_problem(nx, ny, max(nz, 2), 0.25, nsx, nsy, 0.5, E, seed=4)
+    if nz == 1:
+        # the oracle's moments need two depths: tables of the first slice of two
+        g = p.grid
+        g1 = model.DoseGrid(g.nx, g.ny, 1, g.dx, g.dy, g.dz, g.ox, g.oy, g.oz)
+        p = model.PBProblem(g1, p.lattice, p.idd[:, :1].copy(), p.sigma[:, :1].copy())
+    check(p, tol=TOL_FP32 if precision == "fp32" else MIXED_TOL[precision], precision=precision)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "bf16x3"])
+def test_all_zero_weights_give_zero_dose(precision):
+    from pbdose import engines
+    print(f"case all-zero weights {precision}", flush=True)
+    p = make_problem(32, 32, 8, 0.25, 7, 7, 0.5, [100.0, 140.0], w=np.zeros((2, 7, 7)))
+    D = engines.torch_separable(p, "cuda", precision=precision)
+    assert int(torch.count_nonzero(D)) == 0
+
 def test_linearity_and_determinism():
     from pbdose import engines
     p = make_problem(64, 64, 16, 0.2, 21, 21, 0.5, np.linspace(100, 150, 6), seed=11)
