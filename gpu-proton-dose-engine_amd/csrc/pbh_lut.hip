@@ -177,9 +177,23 @@ __global__ __launch_bounds__(LUT_THREADS) void range_table_kernel(double* E, dou
     }
     __syncthreads();
     if (threadIdx.x == 0) {   // sequential, as np.cumsum
+        // segments read 16 at a time ahead of the dependent additions (same order);
+        // the chain of FP64 additions, not the reads, sets this loop's time
+        constexpr int U = 16;
         double acc = 0.0;
         R[0] = 0.0;
-        for (int k = 0; k < N_TABLE - 1; ++k) { acc += seg[k]; R[k + 1] = acc; }
+        int k = 0;
+        for (; k + U <= N_TABLE - 1; k += U) {
+            double v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = seg[k + u];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                acc += v[u];
+                R[k + u + 1] = acc;
+            }
+        }
+        for (; k < N_TABLE - 1; ++k) { acc += seg[k]; R[k + 1] = acc; }
     }
 }
 
