@@ -244,19 +244,22 @@ __global__ __launch_bounds__(LUT_THREADS) void layer_scalars_kernel(LayerLutArgs
     double s_strag = 0.0;
     if (a.m.straggling == 0) {
         // sigma^2 = trapezoid of BOHR / (beta^2 S^3) on linspace(0.5, E0, 4000). physics.py:168-174
-        const int n = 4000;
+        constexpr int n = 4000;
         double part = 0.0;
         const double last = lin_at(ELO, E0, n, n - 1), first = lin_at(ELO, E0, n, 0);
         if (last > first) {
-            for (int i = threadIdx.x; i < n - 1; i += LUT_THREADS) {
-                const double e0 = lin_at(ELO, E0, n, i), e1 = lin_at(ELO, E0, n, i + 1);
-                double b0 = beta2_of(e0, nullptr), b1 = beta2_of(e1, nullptr);
-                b0 = b0 < 1e-12 ? 1e-12 : b0;
-                b1 = b1 < 1e-12 ? 1e-12 : b1;
-                const double S0 = stopping_power(e0), S1 = stopping_power(e1);
-                const double f0 = BOHR / (b0 * (S0 * S0 * S0));
-                const double f1 = BOHR / (b1 * (S1 * S1 * S1));
-                part += (e1 - e0) * (f1 + f0) / 2.0;
+            // a contiguous run of cells per thread: each integrand value is evaluated
+            // once and serves the two cells it bounds
+            constexpr int PER = (n - 1 + LUT_THREADS - 1) / LUT_THREADS;
+            const int c0 = threadIdx.x * PER, c1 = c0 + PER < n - 1 ? c0 + PER : n - 1;
+            if (c0 < c1) {
+                double e0 = lin_at(ELO, E0, n, c0), f0 = bohr_integrand(e0);
+                for (int i = c0; i < c1; ++i) {
+                    const double e1 = lin_at(ELO, E0, n, i + 1), f1 = bohr_integrand(e1);
+                    part += (e1 - e0) * (f1 + f0) / 2.0;
+                    e0 = e1;
+                    f0 = f1;
+                }
             }
         }
         s_strag = sqrt(block_sum(part, red));
@@ -429,13 +432,13 @@ __device__ inline double conv_depth_at(const double* dc, const double* amp, cons
     return wave_sum(s);
 }
 
-constexpr int Z_PER_BLOCK = 16;
+constexpr int Z_PER_BLOCK = 4;        // one output depth per wave
 
 // ---- IDD, step 2: evaluate the convolution at the bracketing fine points and
 // interpolate onto z (left = right = 0). One wave per depth. ----
 __global__ __launch_bounds__(LUT_THREADS) void idd_eval_kernel(LayerLutArgs a)
 {
-    const int l = blockIdx.y;
+    const int l = blockIdx.x;
     const int n = a.n_fine;
     const double R0 = a.R0[l], sR = a.sR[l];
     const double top = R0 + 8.0 * sR + 0.2;
@@ -449,7 +452,7 @@ __global__ __launch_bounds__(LUT_THREADS) void idd_eval_kernel(LayerLutArgs a)
     auto conv = [&](int j) { return depth ? conv_depth_at(dc, amp, gk, n, j, W, h) : conv_at(dc, gk, n, j, M); };
     const int wv = threadIdx.x >> 6;
     for (int q = wv; q < Z_PER_BLOCK; q += LUT_THREADS / 64) {
-        const int k = blockIdx.x * Z_PER_BLOCK + q;
+        const int k = blockIdx.y * Z_PER_BLOCK + q;
         if (k >= a.nz) break;
         const double z = a.z0 + (double)k * a.dz;
         double out;
@@ -516,13 +519,13 @@ __device__ inline double bsum_at(const double* T, const double* hf, double zlo, 
 // ---- Fermi-Eyges, step 2: B interpolated onto z, sigma = sqrt(var). ----
 __global__ __launch_bounds__(LUT_THREADS) void fe_eval_kernel(LayerLutArgs a)
 {
-    const int l = blockIdx.y;
+    const int l = blockIdx.x;
     const int nf = a.nf;
     const double zlo = a.z0, zhi = a.z0 + (double)(a.nz - 1) * a.dz;
     const double* T = a.T + (size_t)l * nf;
     const int wv = threadIdx.x >> 6;
     for (int q = wv; q < Z_PER_BLOCK; q += LUT_THREADS / 64) {
-        const int k = blockIdx.x * Z_PER_BLOCK + q;
+        const int k = blockIdx.y * Z_PER_BLOCK + q;
         if (k >= a.nz) break;
         const double z = a.z0 + (double)k * a.dz;
         double B;
@@ -554,19 +557,31 @@ __global__ __launch_bounds__(LUT_THREADS) void fe_eval_kernel(LayerLutArgs a)
 }
 
 // Per-layer summary: R0, sigma_R, depth and value of the first IDD maximum (np.argmax).
-__global__ void layer_info_kernel(const double* idd, const double* R0, const double* sR,
+__global__ __launch_bounds__(64) void layer_info_kernel(const double* idd, const double* R0, const double* sR,
                                   int L, int nz, double z0, double dz, double* info)
 {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    // one wave per layer: np.argmax (first maximum) over lane-strided depths, then
+    // across lanes, the lower index winning ties
+    const int l = blockIdx.x, lane = threadIdx.x;
     if (l >= L) return;
     const double* row = idd + (size_t)l * nz;
-    int best = 0;
-    for (int k = 1; k < nz; ++k)
-        if (row[k] > row[best]) best = k;
-    info[4 * l + 0] = R0[l];
-    info[4 * l + 1] = sR[l];
-    info[4 * l + 2] = z0 + (double)best * dz;
-    info[4 * l + 3] = row[best];
+    double bv = 0.0;
+    int bk = -1;
+    for (int k = lane; k < nz; k += 64) {
+        const double v = row[k];
+        if (bk < 0 || v > bv) { bv = v; bk = k; }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(bv, o, 64);
+        const int ok = __shfl_xor(bk, o, 64);
+        if (ok >= 0 && (bk < 0 || ov > bv || (ov == bv && ok < bk))) { bv = ov; bk = ok; }
+    }
+    if (lane == 0) {
+        info[4 * l + 0] = R0[l];
+        info[4 * l + 1] = sR[l];
+        info[4 * l + 2] = z0 + (double)bk * dz;
+        info[4 * l + 3] = bv;
+    }
 }
 
 // Fine-grid size of the moment integrals (physics.py:256-258), evaluated on the
@@ -692,7 +707,7 @@ extern "C" int pbh_build_lut(const PbhBeamModel* m, int nz, double z0, double dz
     // one workgroup per layer and 256 fine bins: every bin is independent
     hipLaunchKernelGGL(idd_prep_kernel, dim3(n_layers, cdiv(a.n_fine, LUT_THREADS)), dim3(LUT_THREADS), 0, s, a);
     if ((rc = launch_rc("idd_prep_kernel")) != PBH_OK) return rc;
-    const dim3 eval_grid(cdiv(nz, Z_PER_BLOCK), n_layers);
+    const dim3 eval_grid(n_layers, cdiv(nz, Z_PER_BLOCK));   // layers in x: up to 2^20 of them
     hipLaunchKernelGGL(idd_eval_kernel, eval_grid, dim3(LUT_THREADS), 0, s, a);
     if ((rc = launch_rc("idd_eval_kernel")) != PBH_OK) return rc;
     hipLaunchKernelGGL(fe_prep_kernel, dim3(n_layers), dim3(LUT_THREADS), 0, s, a);
@@ -700,7 +715,7 @@ extern "C" int pbh_build_lut(const PbhBeamModel* m, int nz, double z0, double dz
     hipLaunchKernelGGL(fe_eval_kernel, eval_grid, dim3(LUT_THREADS), 0, s, a);
     if ((rc = launch_rc("fe_eval_kernel")) != PBH_OK) return rc;
     if (layer_info) {
-        hipLaunchKernelGGL(layer_info_kernel, dim3(cdiv(n_layers, 64)), dim3(64), 0, s,
+        hipLaunchKernelGGL(layer_info_kernel, dim3(n_layers), dim3(64), 0, s,
                            (const double*)idd, (const double*)a.R0, (const double*)a.sR,
                            n_layers, nz, z0, dz, layer_info);
         if ((rc = launch_rc("layer_info_kernel")) != PBH_OK) return rc;
