@@ -37,16 +37,34 @@ constexpr int WT_STRIDE = KYP + 8;        // W^T rows in halves
 constexpr int U_STRIDE = SCH + 8;         // U rows in halves
 constexpr int WV = SCH * KYP / 8;         // 16-byte vectors per split half of a weight chunk
 
-// r_peak[l] = depth of the first IDD maximum of layer l (np.argmax semantics).
-__global__ void peak_depth_f32_kernel(const double* idd, int L, int nz, double oz, double dz, float* r_peak)
+// Index of the first maximum of row[0 .. nz) (np.argmax semantics), by one wave:
+// lane-strided scans, then a reduction across lanes in which the lower index wins
+// ties. Every lane returns the result.
+__device__ inline int first_max_wave(const double* row, int nz)
 {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    double bv = 0.0;
+    int bk = -1;
+    for (int k = lane; k < nz; k += 64) {
+        const double v = row[k];
+        if (bk < 0 || v > bv) { bv = v; bk = k; }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(bv, o, 64);
+        const int ok = __shfl_xor(bk, o, 64);
+        if (ok >= 0 && (bk < 0 || ov > bv || (ov == bv && ok < bk))) { bv = ov; bk = ok; }
+    }
+    return bk;
+}
+
+// r_peak[l] = depth of the first IDD maximum of layer l: one wave per layer.
+__global__ __launch_bounds__(64) void peak_depth_f32_kernel(const double* idd, int L, int nz, double oz, double dz,
+                                                            float* r_peak)
+{
+    const int l = blockIdx.x;
     if (l >= L) return;
-    const double* row = idd + (size_t)l * nz;
-    int best = 0;
-    for (int k = 1; k < nz; ++k)
-        if (row[k] > row[best]) best = k;
-    r_peak[l] = (float)(oz + (double)best * dz);
+    const int best = first_max_wave(idd + (size_t)l * nz, nz);
+    if (threadIdx.x == 0) r_peak[l] = (float)(oz + (double)best * dz);
 }
 
 // Split 16-bit weights of one (layer, spot chunk): hi and lo at [i][j] (i < 48,
@@ -497,16 +515,13 @@ void fwd_mixed_batch_kernel(const BatchFwdItem* __restrict__ items, const int* _
                                          (int)blockIdx.x - wg0[k]);
 }
 
-// r_peak of every layer of every item (one workgroup per item)
-__global__ void peak_depth_batch_kernel(const BatchPeakItem* __restrict__ items)
+// r_peak of every layer of every item: one workgroup per item, one wave per layer
+__global__ __launch_bounds__(256) void peak_depth_batch_kernel(const BatchPeakItem* __restrict__ items)
 {
     const BatchPeakItem it = items[blockIdx.x];
-    for (int l = threadIdx.x; l < it.L; l += blockDim.x) {
-        const double* row = it.idd + (size_t)l * it.nz;
-        int best = 0;
-        for (int k = 1; k < it.nz; ++k)
-            if (row[k] > row[best]) best = k;
-        it.r_peak[l] = (float)(it.oz + (double)best * it.dz);
+    for (int l = threadIdx.x >> 6; l < it.L; l += blockDim.x >> 6) {
+        const int best = first_max_wave(it.idd + (size_t)l * it.nz, it.nz);
+        if ((threadIdx.x & 63) == 0) it.r_peak[l] = (float)(it.oz + (double)best * it.dz);
     }
 }
 
@@ -611,7 +626,7 @@ int pbh_forward_mixed(const FwdArgs& a, double oz, double dz, int precision, int
     float* r_peak = nullptr;
     if (precision == PBH_PREC_BF16X3_PEAK || precision == PBH_PREC_FP16X3_PEAK) {
         r_peak = pbh_forward_ws_peaks(a, ws);
-        hipLaunchKernelGGL(peak_depth_f32_kernel, dim3(cdiv(a.L, 64)), dim3(64), 0, s,
+        hipLaunchKernelGGL(peak_depth_f32_kernel, dim3(a.L), dim3(64), 0, s,
                            a.idd, a.L, a.nz, oz, dz, r_peak);
         if ((rc = launch_rc("peak_depth_f32_kernel")) != PBH_OK) return rc;
     }
@@ -629,7 +644,7 @@ int pbh_forward_mixed(const FwdArgs& a, double oz, double dz, int precision, int
 
 int pbh_launch_peak_batch(const BatchPeakItem* items, int n, hipStream_t s)
 {
-    hipLaunchKernelGGL(peak_depth_batch_kernel, dim3((unsigned)n), dim3(64), 0, s, items);
+    hipLaunchKernelGGL(peak_depth_batch_kernel, dim3((unsigned)n), dim3(256), 0, s, items);
     return launch_rc("peak_depth_batch_kernel");
 }
 
