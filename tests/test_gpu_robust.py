@@ -89,6 +89,12 @@ def test_small_batch_matches_oracle_and_single_path(precision, tuning):
     b = robust.RobustBatch(fields, scen, precision=precision, tuning=tuning)
     b.run_eager()
     torch.cuda.synchronize()
+    check_items(b, fields, scen, precision)
+
+
+def check_items(b, fields, scen, precision):
+    """Every item of a run batch vs the FP64 oracle (dose, dJ/dw, J) and vs the
+    single-problem path."""
     tol = 1e-6 if precision == "fp32" else 2e-5
     worst = [0.0] * 5
     for k, (fi, si) in enumerate(b.items):
@@ -112,6 +118,28 @@ def test_small_batch_matches_oracle_and_single_path(precision, tuning):
     print(f"  {len(b.items)} items, {b.launch_count} launches: dose vs oracle {worst[0]:.2e}, "
           f"dJ/dw vs oracle {worst[1]:.2e}, J {worst[2]:.1e}, vs single path {worst[3]:.1e} / {worst[4]:.1e}",
           flush=True)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_degenerate_fields_in_a_batch(precision):
+    """Fields on a 1 x 1 x 2 grid (one spot, one layer), a one-voxel-wide column and a
+    one-voxel-wide row, at 2.5 mm voxels and 5 mm spots, in one batch with two scenarios."""
+    from pbdose import model, robust
+    print(f"case degenerate fields in a batch {precision}", flush=True)
+    rng = np.random.default_rng(5)
+
+    def field(nx, ny, nz, nsx, nsy, E):
+        g = model.DoseGrid(nx, ny, nz, 0.25, 0.25, 0.25, -(nx - 1) * 0.25 / 2, -(ny - 1) * 0.25 / 2, 0.0)
+        w = rng.lognormal(0.0, 0.5, (len(E), nsy, nsx))
+        return robust.Field(g, model.SpotLattice(nsx, nsy, len(E), 0.5, 0.5, energies=np.array(E), weights=w))
+
+    fields = [field(1, 1, 2, 1, 1, [120.0]), field(1, 40, 6, 5, 9, [100.0, 130.0]),
+              field(40, 1, 6, 9, 5, [100.0, 130.0])]
+    scen = small_scenarios()[:2]
+    b = robust.RobustBatch(fields, scen, precision=precision, tuning=robust.DEFAULT_TUNING)
+    b.run_eager()
+    torch.cuda.synchronize()
+    check_items(b, fields, scen, precision)
 
 
 def test_forward_only_batch_and_lut_gradients():
