@@ -57,16 +57,6 @@ __device__ inline int first_max_wave(const double* row, int nz)
     return bk;
 }
 
-// r_peak[l] = depth of the first IDD maximum of layer l: one wave per layer.
-__global__ __launch_bounds__(64) void peak_depth_f32_kernel(const double* idd, int L, int nz, double oz, double dz,
-                                                            float* r_peak)
-{
-    const int l = blockIdx.x;
-    if (l >= L) return;
-    const int best = first_max_wave(idd + (size_t)l * nz, nz);
-    if (threadIdx.x == 0) r_peak[l] = (float)(oz + (double)best * dz);
-}
-
 // Split 16-bit weights of one (layer, spot chunk): hi and lo at [i][j] (i < 48,
 // j < 64), zero outside the chunk. blk = (layer * chunks_y + chunk_y) *
 // chunks_x + chunk_x; part (0 .. WS_PARTS-1) selects a slice of the chunk's
@@ -127,12 +117,22 @@ __device__ __forceinline__ void layer_lists_body(const FwdArgs& a, const FwdPlan
 }
 
 // Grid (chunks, WS_PARTS + 1): rows 0 .. WS_PARTS-1 split the weights, the last
-// row writes the layer lists of the slices.
+// row writes the layer lists of the slices and, for the "_peak" forms (r_peak not
+// null), r_peak[l] = depth of the first IDD maximum of layer l, one wave per layer.
 template <typename T>
-__global__ __launch_bounds__(256) void wsplit_kernel(FwdArgs a, FwdPlan pl)
+__global__ __launch_bounds__(256) void wsplit_kernel(FwdArgs a, FwdPlan pl, double oz, double dz, float* r_peak)
 {
-    if (blockIdx.y == WS_PARTS) layer_lists_body(a, pl, blockIdx.x, gridDim.x);
-    else wsplit_body<T>(a, pl.wsplit, blockIdx.x, blockIdx.y);
+    if (blockIdx.y != WS_PARTS) {
+        wsplit_body<T>(a, pl.wsplit, blockIdx.x, blockIdx.y);
+        return;
+    }
+    layer_lists_body(a, pl, blockIdx.x, gridDim.x);
+    if (!r_peak) return;
+    const int wave = threadIdx.x >> 6;
+    for (int l = 4 * blockIdx.x + wave; l < a.L; l += 4 * gridDim.x) {
+        const int best = first_max_wave(a.idd + (size_t)l * a.nz, a.nz);
+        if ((threadIdx.x & 63) == 0) r_peak[l] = (float)(oz + (double)best * dz);
+    }
 }
 
 template <typename T>
@@ -625,17 +625,14 @@ int pbh_forward_mixed(const FwdArgs& a, double oz, double dz, int precision, int
     if (rc != PBH_OK) return rc;
     float* r_peak = nullptr;
     if (precision == PBH_PREC_BF16X3_PEAK || precision == PBH_PREC_FP16X3_PEAK) {
-        r_peak = pbh_forward_ws_peaks(a, ws);
-        hipLaunchKernelGGL(peak_depth_f32_kernel, dim3(a.L), dim3(64), 0, s,
-                           a.idd, a.L, a.nz, oz, dz, r_peak);
-        if ((rc = launch_rc("peak_depth_f32_kernel")) != PBH_OK) return rc;
+        r_peak = pbh_forward_ws_peaks(a, ws);   // written by the weight-split launch
     }
     if (!pl.wsplit || !pl.layers) return PBH_ERR_WORKSPACE;
     const dim3 wgrid((unsigned)(a.L * chunk_count(a.nsy) * chunk_count(a.nsx)), WS_PARTS + 1);
     if (is_f16(precision))
-        hipLaunchKernelGGL(wsplit_kernel<_Float16>, wgrid, dim3(256), 0, s, a, pl);
+        hipLaunchKernelGGL(wsplit_kernel<_Float16>, wgrid, dim3(256), 0, s, a, pl, oz, dz, r_peak);
     else
-        hipLaunchKernelGGL(wsplit_kernel<__bf16>, wgrid, dim3(256), 0, s, a, pl);
+        hipLaunchKernelGGL(wsplit_kernel<__bf16>, wgrid, dim3(256), 0, s, a, pl, oz, dz, r_peak);
     if ((rc = launch_rc("wsplit_kernel")) != PBH_OK) return rc;
     if ((rc = dispatch_precision(precision, mixed_key(pl, kb), SingleLaunch{a, pl, oz, dz, r_peak, s})) != PBH_OK)
         return rc;
