@@ -63,9 +63,13 @@ SIGNATURES = {
     "pbh_range_table_size": (C.c_int, []),
     "pbh_range_table": (C.c_int, [_P, _P, _P]),
     "pbh_interp_table": (C.c_int, [_P, _P, C.c_int, _P, _P, C.c_int, _P]),
-    "pbh_lut_workspace_bytes": (C.c_size_t, [_M, C.c_int, C.c_int]),
-    "pbh_build_lut": (C.c_int, [_M, C.c_int, C.c_double, C.c_double, C.c_int, _P, _P, _P, _P,
+    "pbh_interp_table_slope": (C.c_int, [_P, _P, C.c_int, _P, _P, _P, C.c_int, _P]),
+    "pbh_lut_workspace_bytes": (C.c_size_t, [_M, C.c_int, C.c_double, C.c_double, C.c_int]),
+    "pbh_build_lut": (C.c_int, [_M, C.c_int, C.c_double, C.c_double, C.c_int, _P, _P, _P, _P, _P, _P,
                                 _P, C.c_size_t, _P]),
+    "pbh_lut_tangent_workspace_bytes": (C.c_size_t, [_M, C.c_int, C.c_double, C.c_double, C.c_int]),
+    "pbh_build_lut_tangent": (C.c_int, [_M, C.c_int, C.c_double, C.c_double, C.c_int, _P, _P, _P, _P,
+                                        _P, _P, _P, _P, _P, _P, C.c_size_t, _P]),
     "pbh_forward_workspace_bytes": (C.c_size_t, [_G, _L, C.c_int]),
     "pbh_forward_workspace_bytes_tuned": (C.c_size_t, [_G, _L, C.c_int, C.c_int]),
     "pbh_forward": (C.c_int, [_G, _L, _P, _P, _P, _P, C.c_int, _P, C.c_size_t, _P]),

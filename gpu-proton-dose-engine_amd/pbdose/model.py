@@ -139,10 +139,32 @@ class PBProblem:
         return "\n".join(lines)
 
 
-def _tables(grid: DoseGrid, energies, idd_model, sigma_x0, sigma_theta0, device):
-    idd, sig, _ = ph.build_luts(grid.nz, grid.oz, grid.dz, energies, idd_model=idd_model,
-                                sigma_x0=sigma_x0, sigma_theta0=sigma_theta0, device=device)
-    return idd.cpu().numpy(), sig.cpu().numpy()
+def _tables(grid: DoseGrid, energies, idd_model, sigma_x0, sigma_theta0, device,
+            fluence_normalized: bool = True):
+    """(idd, sigma) host arrays of every layer, from one device call. With
+    fluence_normalized=False each layer's IDD is divided by its maximum on the
+    grid, as idd_pristine(normalize=True) does (physics.py:380-382)."""
+    idd, sig, info = ph.build_luts(grid.nz, grid.oz, grid.dz, energies, idd_model=idd_model,
+                                   sigma_x0=sigma_x0, sigma_theta0=sigma_theta0, device=device)
+    idd = idd.cpu().numpy()
+    if not fluence_normalized:
+        pk = info[:, 3].cpu().numpy()
+        idd = idd / np.where(pk > 0, pk, 1.0)[:, None]
+    return idd, sig.cpu().numpy()
+
+
+def gaussian_field_weights(n_layers: int, n_spot: int = 41, spot_spacing: float = 0.5,
+                           spot_sigma: float = 0.28, seed: int = 0) -> np.ndarray:
+    """Weights (L, n_spot, n_spot) of gaussian_impt_field: a Gaussian lateral
+    envelope (peak 1) with 25 % per-spot jitter from default_rng(seed), clipped
+    at 0. The draws are taken in the reference's order (model.py:207-220), so
+    that the same seed gives the same weights bit for bit."""
+    rng = np.random.default_rng(seed)
+    s = (np.arange(n_spot) - (n_spot - 1) / 2.0) * spot_spacing
+    X, Y = np.meshgrid(s, s)
+    env = np.exp(-(X**2 + Y**2) / (2.0 * spot_sigma**2))
+    env = env / env.max()
+    return np.clip(env[None] * (1.0 + 0.25 * rng.standard_normal((n_layers, n_spot, n_spot))), 0.0, None)
 
 
 def gaussian_impt_field(energies, grid: DoseGrid | None = None, *, n_spot: int = 41,
@@ -154,19 +176,14 @@ def gaussian_impt_field(energies, grid: DoseGrid | None = None, *, n_spot: int =
     grid = grid or DoseGrid.centered(128, 0.2)
     energies = np.asarray(energies, dtype=np.float64)
     L = len(energies)
-    rng = np.random.default_rng(seed)
-    s = (np.arange(n_spot) - (n_spot - 1) / 2.0) * spot_spacing
-    X, Y = np.meshgrid(s, s)
-    env = np.exp(-(X**2 + Y**2) / (2.0 * spot_sigma**2))
-    env = env / env.max()
-    w = np.clip(env[None] * (1.0 + 0.25 * rng.standard_normal((L, n_spot, n_spot))), 0.0, None)
+    w = gaussian_field_weights(L, n_spot, spot_spacing, spot_sigma, seed)
     lat = SpotLattice(n_spot, n_spot, L, spot_spacing, spot_spacing, energies=energies, weights=w)
-    if not fluence_normalized:
-        raise NotImplementedError("peak-normalised IDDs are not supported; use fluence_normalized=True")
-    idd, sig = _tables(grid, energies, idd_model, beam_sigma_x0, beam_sigma_theta0, device)
+    idd, sig = _tables(grid, energies, idd_model, beam_sigma_x0, beam_sigma_theta0, device,
+                       fluence_normalized)
     meta = {"weights": f"Gaussian envelope sigma={spot_sigma * 10:.1f} mm, seed {seed}",
-            "beam sigma_x0": f"{beam_sigma_x0 * 10:.1f} mm"}
-    return PBProblem(grid, lat, idd, sig, meta=meta, fluence_normalized=True)
+            "beam sigma_x0": f"{beam_sigma_x0 * 10:.1f} mm",
+            "IDD": "fluence-normalised" if fluence_normalized else "peak-normalised per layer"}
+    return PBProblem(grid, lat, idd, sig, meta=meta, fluence_normalized=fluence_normalized)
 
 
 def clinical_layer_energies(n_layers: int = 40, r_min: float = 3.0, r_max: float = 25.5,
@@ -207,11 +224,11 @@ def flat_impt_field(energies, grid: DoseGrid | None = None, *, n_spot: int = 41,
     w = flat_field_weights(n_spot, n_spot, L, field_half_width=field_half_width,
                            spot_spacing=spot_spacing, seed=seed)
     lat = SpotLattice(n_spot, n_spot, L, spot_spacing, spot_spacing, energies=energies, weights=w)
-    if not fluence_normalized:
-        raise NotImplementedError("peak-normalised IDDs are not supported; use fluence_normalized=True")
-    idd, sig = _tables(grid, energies, idd_model, beam_sigma_x0, beam_sigma_theta0, device)
-    meta = {"cross-section": f"flat {2 * field_half_width:.1f} x {2 * field_half_width:.1f} cm"}
-    return PBProblem(grid, lat, idd, sig, meta=meta, fluence_normalized=True)
+    idd, sig = _tables(grid, energies, idd_model, beam_sigma_x0, beam_sigma_theta0, device,
+                       fluence_normalized)
+    meta = {"cross-section": f"flat {2 * field_half_width:.1f} x {2 * field_half_width:.1f} cm",
+            "IDD": "fluence-normalised" if fluence_normalized else "peak-normalised per layer"}
+    return PBProblem(grid, lat, idd, sig, meta=meta, fluence_normalized=fluence_normalized)
 
 
 def water_field(grid: DoseGrid, n_sx: int, n_sy: int, spacing: float, energies, weights,

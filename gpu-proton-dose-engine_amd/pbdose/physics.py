@@ -112,44 +112,84 @@ def energy_from_range(R_cm, device="cuda"):
 
 
 def _uniform_axis(z):
+    """(nz, z0, dz) of a uniform depth grid; a single depth has nz = 1 (dz unused)."""
     z = np.asarray(z, dtype=np.float64).reshape(-1)
-    if z.size < 2:
-        raise ValueError("depth grid needs at least two points")
+    if z.size == 0:
+        raise ValueError("depth grid needs at least one point")
+    if z.size == 1:
+        return 1, float(z[0]), 1.0
     dz = (z[-1] - z[0]) / (z.size - 1)
     if not np.allclose(z, z[0] + np.arange(z.size) * dz, rtol=0, atol=1e-12 * max(1.0, abs(z[-1]))):
         raise ValueError("depth grid must be uniform (z0 + k dz)")
     return z.size, float(z[0]), float(dz)
 
 
+def _energies(energies, dev):
+    if torch.is_tensor(energies):
+        return energies.to(device=dev, dtype=torch.float64).reshape(-1).contiguous()
+    return torch.as_tensor(np.asarray(energies, dtype=np.float64), device=dev).reshape(-1).contiguous()
+
+
 def build_luts(nz: int, z0: float, dz: float, energies, *, idd_model: IDDModel = DEFAULT_IDD,
                sigma_x0: float = 0.30, sigma_theta0: float = 0.0, rho0: float = 0.0,
-               device="cuda"):
+               device="cuda", denergies=None):
     """Per-layer (idd, sigma, info) on depths z0 + k dz as FP64 device tensors.
 
     idd (L, nz) is the fluence-normalised IDD (idd_pristine(normalize=False),
     physics.py:328-387); sigma (L, nz) is fermi_eyges_moments(...)['sigma_x']
-    (physics.py:221-286); info (L, 4) = R0, sigma_R, depth and value of the IDD
-    maximum. One device call for all layers (model.py:225-233 loops on the host).
+    (physics.py:221-286), with a single depth (nz = 1) integrated over [0, z0]
+    as physics.py:247-249 does; info (L, 4) = R0, sigma_R, depth and value of
+    the IDD maximum. One device call for all layers (model.py:225-233 loops on
+    the host), reading the range table built once per device.
+
+    With ``denergies`` (one value per layer) the call also returns the tables'
+    derivatives along that direction of the layer energies,
+    (idd, sigma, info, didd, dsigma), from pbh_build_lut_tangent: the same
+    tables bit for bit, with forward-mode FP64 derivatives of every kernel.
     """
     dev = _lib.require_cuda(device)
     L = _lib.lib()
-    if torch.is_tensor(energies):
-        en = energies.to(device=dev, dtype=torch.float64).reshape(-1).contiguous()
-    else:
-        en = torch.as_tensor(np.asarray(energies, dtype=np.float64), device=dev).reshape(-1).contiguous()
+    en = _energies(energies, dev)
     nl = en.numel()
     m = beam_model(idd_model, sigma_x0, sigma_theta0, rho0)
-    nbytes = L.pbh_lut_workspace_bytes(m, int(nz), nl)
+    tangent = denergies is not None
+    query = L.pbh_lut_tangent_workspace_bytes if tangent else L.pbh_lut_workspace_bytes
+    nbytes = query(m, int(nz), float(z0), float(dz), nl)
     if nbytes == 0:
         raise ValueError("invalid beam model or grid for the LUT builder")
     ws = _lib.workspace(nbytes, dev)
+    tE, tR = range_table(dev)
     idd = torch.empty((nl, nz), dtype=torch.float64, device=dev)
     sig = torch.empty((nl, nz), dtype=torch.float64, device=dev)
     info = torch.empty((nl, 4), dtype=torch.float64, device=dev)
-    _lib.check(L.pbh_build_lut(m, int(nz), float(z0), float(dz), nl, _lib.ptr(en), _lib.ptr(idd),
-                               _lib.ptr(sig), _lib.ptr(info), _lib.ptr(ws), ws.numel(),
-                               _lib.stream_of(dev)), "pbh_build_lut")
-    return idd, sig, info
+    if not tangent:
+        _lib.check(L.pbh_build_lut(m, int(nz), float(z0), float(dz), nl, _lib.ptr(en), _lib.ptr(tE),
+                                   _lib.ptr(tR), _lib.ptr(idd), _lib.ptr(sig), _lib.ptr(info),
+                                   _lib.ptr(ws), ws.numel(), _lib.stream_of(dev)), "pbh_build_lut")
+        return idd, sig, info
+    den = _energies(denergies, dev)
+    if den.numel() != nl:
+        raise ValueError("denergies needs one value per layer")
+    didd = torch.empty_like(idd)
+    dsig = torch.empty_like(sig)
+    _lib.check(L.pbh_build_lut_tangent(m, int(nz), float(z0), float(dz), nl, _lib.ptr(en), _lib.ptr(den),
+                                       _lib.ptr(tE), _lib.ptr(tR), _lib.ptr(idd), _lib.ptr(sig),
+                                       _lib.ptr(didd), _lib.ptr(dsig), _lib.ptr(info), _lib.ptr(ws),
+                                       ws.numel(), _lib.stream_of(dev)), "pbh_build_lut_tangent")
+    return idd, sig, info, didd, dsig
+
+
+def energy_from_range_slope(R_cm, device="cuda"):
+    """(E(R), dE/dR) as FP64 device tensors: energy_from_range (physics.py:136-139)
+    and the slope of the table segment it interpolates (0 where R is clamped)."""
+    dev = _lib.require_cuda(device)
+    E, R = range_table(dev)
+    x = _energies(R_cm, dev)
+    out, dout = torch.empty_like(x), torch.empty_like(x)
+    _lib.check(_lib.lib().pbh_interp_table_slope(_lib.ptr(R), _lib.ptr(E), R.numel(), _lib.ptr(x),
+                                                 _lib.ptr(out), _lib.ptr(dout), x.numel(),
+                                                 _lib.stream_of(dev)), "pbh_interp_table_slope")
+    return out, dout
 
 
 def idd_pristine(z, E0, model: IDDModel = DEFAULT_IDD, normalize: bool = True, device="cuda"):

@@ -9,10 +9,11 @@ layer is rebuilt at energy E(R0_l + delta_l) by the on-device FP64 LUT builder.
 The gradients come from the adjoint kernels:
 
     dJ/dw      pbh_adjoint (deterministic sum over depth)
-    dJ/ddelta  pbh_adjoint's dJ/dIDD and dJ/dsigma, contracted over depth with the
-               FP64 derivatives of the look-up tables with respect to delta
-               (central difference of the LUT builder with step ``lut_step``)
-               by pbh_lut_chain.
+    dJ/ddelta  pbh_adjoint's dJ/dIDD and dJ/dsigma, contracted over depth by
+               pbh_lut_chain with the exact FP64 derivatives of the look-up tables
+               with respect to delta: pbh_build_lut_tangent differentiates every
+               kernel of the LUT builder in forward mode along dE/dR at R0 + delta
+               (pbh_interp_table_slope), in the same pass that builds the tables.
 
 No counterpart exists in the reference (its engines are forward-only,
 engines.py:208-283); the objective follows the workload definition used for
@@ -35,7 +36,8 @@ class FieldModel:
     """A field whose per-layer tables are rebuilt on the device for range shifts.
 
     grid, lattice (with energies), beam/IDD model. ``luts(delta)`` returns FP64
-    (idd, sigma) device tensors for the energies E(R0 + delta)."""
+    (idd, sigma) device tensors for the energies E(R0 + delta);
+    ``luts_and_derivatives(delta)`` adds their exact derivatives in delta."""
 
     def __init__(self, grid: DoseGrid, lattice: SpotLattice, *, idd_model=physics.DEFAULT_IDD,
                  sigma_x0: float = 0.30, sigma_theta0: float = 0.0, rho0: float = 0.0,
@@ -49,7 +51,6 @@ class FieldModel:
         self.E0 = torch.as_tensor(np.asarray(lattice.energies, dtype=np.float64), device=self.dev)
         E, R = physics.range_table(self.dev)
         self.R0 = self._interp(E, R, self.E0)
-        self._cache = {}
         idd, sig = self.luts(None)
         self.problem = PBProblem(grid, lattice, idd.cpu().numpy(), sig.cpu().numpy())
         self.dp = DeviceProblem(self.problem, self.dev, idd=idd, sigma=sig)
@@ -62,11 +63,14 @@ class FieldModel:
                    "pbh_interp_table")
         return out
 
+    def _delta(self, delta):
+        return torch.zeros_like(self.R0) if delta is None else delta.to(self.dev, torch.float64)
+
     def energies(self, delta: Optional[torch.Tensor]) -> torch.Tensor:
         if delta is None:
             return self.E0
         E, R = physics.range_table(self.dev)
-        return self._interp(R, E, self.R0 + delta.to(torch.float64))
+        return self._interp(R, E, self.R0 + self._delta(delta))
 
     def luts(self, delta: Optional[torch.Tensor]):
         g = self.grid
@@ -74,11 +78,27 @@ class FieldModel:
                                          **self.kw)
         return idd, sig
 
-    def lut_derivatives(self, delta: Optional[torch.Tensor]):
-        """d(idd)/d(delta_l), d(sigma)/d(delta_l) (L, nz), FP64 central difference."""
+    def luts_and_derivatives(self, delta: Optional[torch.Tensor]):
+        """(idd, sigma, d idd/d delta_l, d sigma/d delta_l), all (L, nz) FP64: one
+        tangent build at E(R0 + delta) with direction dE/dR there."""
+        g = self.grid
+        E, dE = physics.energy_from_range_slope(self.R0 + self._delta(delta), device=self.dev)
+        if delta is None:
+            E = self.E0
+        idd, sig, _, didd, dsig = physics.build_luts(g.nz, g.oz, g.dz, E, device=self.dev,
+                                                     denergies=dE, **self.kw)
+        return idd, sig, didd, dsig
+
+    def lut_derivatives(self, delta: Optional[torch.Tensor], method: str = "analytic"):
+        """d(idd)/d(delta_l), d(sigma)/d(delta_l) (L, nz) FP64. ``method``
+        'analytic' (forward-mode derivative of the LUT builder) or 'fd' (central
+        difference with step ``lut_step``, two more LUT builds)."""
+        if method == "analytic":
+            return self.luts_and_derivatives(delta)[2:]
+        if method != "fd":
+            raise ValueError(f"method {method!r}: 'analytic' or 'fd'")
         h = self.lut_step
-        d0 = torch.zeros(self.lattice.n_layers, dtype=torch.float64, device=self.dev) if delta is None \
-            else delta.to(torch.float64)
+        d0 = self._delta(delta)
         ip, sp = self.luts(d0 + h)
         im, sm = self.luts(d0 - h)
         return (ip - im) / (2.0 * h), (sp - sm) / (2.0 * h)
@@ -167,8 +187,14 @@ class _Dose(torch.autograd.Function):
     @staticmethod
     def forward(ctx, w, delta, model: FieldModel, precision: str):
         dp = model.dp
+        ctx.dtab = None
         if delta is not None:
-            idd, sig = model.luts(delta.detach())
+            if ctx.needs_input_grad[1]:
+                # the tables and their derivatives in one tangent build
+                idd, sig, didd, dsig = model.luts_and_derivatives(delta.detach())
+                ctx.dtab = (didd, dsig)
+            else:
+                idd, sig = model.luts(delta.detach())
             dp = DeviceProblem(model.problem, model.dev, weights=w.detach(), idd=idd, sigma=sig)
         D = dp.forward(precision, w=w.detach().contiguous())
         ctx.model, ctx.dp = model, dp
@@ -183,7 +209,7 @@ class _Dose(torch.autograd.Function):
         out = adjoint(ctx.dp, gD.contiguous().float(), w=w.detach(), lut_grads=need_delta)
         if need_delta:
             gw, gi, gs = out
-            didd, dsig = ctx.model.lut_derivatives(delta.detach())
+            didd, dsig = ctx.dtab if ctx.dtab is not None else ctx.model.lut_derivatives(delta.detach())
             gdelta = lut_chain(gi, gs, didd, dsig).to(delta.dtype)
         else:
             gw, gdelta = out, None

@@ -125,22 +125,58 @@ int pbh_range_table(double* E, double* R, void* stream);
 int pbh_interp_table(const double* xp, const double* fp, int n_table,
                      const double* x, double* out, int n, void* stream);
 
-/* Workspace for pbh_build_lut with nz depths and n_layers layers. */
-size_t pbh_lut_workspace_bytes(const PbhBeamModel* m, int nz, int n_layers);
+/* out[i] and dout[i] = d out[i] / d x[i]: pbh_interp_table and the slope of the
+ * bracketing segment (0 where x is clamped to the table's ends). With
+ * (xp, fp) = (R, E) this is dE/dR at R, the energy change of a range shift.
+ * physics.py:136-139 and its derivative; no counterpart in the reference. */
+int pbh_interp_table_slope(const double* xp, const double* fp, int n_table,
+                           const double* x, double* out, double* dout, int n, void* stream);
+
+/* Workspace for pbh_build_lut on the depths z0 + k dz, k < nz (nz >= 1), with
+ * n_layers layers; 0 for invalid input. */
+size_t pbh_lut_workspace_bytes(const PbhBeamModel* m, int nz, double z0, double dz, int n_layers);
 
 /* Per-layer IDD (fluence-normalised, MeV/cm) and lateral sigma (cm) on the
  * depths z_k = z0 + k dz, k < nz, for layer energies energies[L] (device).
+ * A single depth (nz = 1) follows physics.py:247-249: sigma is that of the
+ * grid [0, z0] ([0, 1e-6] when z0 <= 0) at its end, the IDD is read at z0.
  * With m->straggling_depth = 1 the IDD uses the depth-dependent range spread
  * described at PbhBeamModel; sigma_R in layer_info is the spread at the end
  * of range, which both models share.
+ * range_E, range_R: the CSDA range table from pbh_range_table (device), so
+ * that it is built once; both NULL: the call builds it in its workspace.
  * Writes idd[L*nz], sigma[L*nz] (FP64) and, if layer_info != NULL,
  * layer_info[L][4] = {CSDA range R0, range spread sigma_R, depth of the IDD
  * maximum on the grid (first maximum), IDD maximum}.
  * physics.py:221-286, :328-387; model.py:225-233 */
 int pbh_build_lut(const PbhBeamModel* m, int nz, double z0, double dz,
                   int n_layers, const double* energies,
+                  const double* range_E, const double* range_R,
                   double* idd, double* sigma, double* layer_info,
                   void* workspace, size_t workspace_bytes, void* stream);
+
+/* Workspace for pbh_build_lut_tangent; 0 for invalid input. */
+size_t pbh_lut_tangent_workspace_bytes(const PbhBeamModel* m, int nz, double z0, double dz,
+                                       int n_layers);
+
+/* pbh_build_lut together with the tables' derivatives along a direction of the
+ * layer energies: didd[l][k] = (d idd[l][k] / d E_l) denergies[l] (each
+ * layer's tables depend on its own energy only), likewise dsigma. The tables equal
+ * pbh_build_lut's bit for bit; the derivatives are those of the same discrete
+ * algorithm (FP64 forward-mode differentiation of every kernel: range
+ * straggling, CSDA binning on the energy-dependent fine grid, the Gaussian
+ * convolution, the interpolation onto z, the scattering power inside the
+ * Fermi-Eyges double sum), exact wherever the algorithm is differentiable
+ * (not where a bin edge or fine-grid point crosses the range R0). For a range
+ * shift d of layer l, E_l = E(R0_l + d) and denergies[l] = dE/dR there
+ * (pbh_interp_table_slope). physics.py:111-139, :151-174, :221-286, :328-387
+ * differentiated; no counterpart in the reference. */
+int pbh_build_lut_tangent(const PbhBeamModel* m, int nz, double z0, double dz,
+                          int n_layers, const double* energies, const double* denergies,
+                          const double* range_E, const double* range_R,
+                          double* idd, double* sigma, double* didd, double* dsigma,
+                          double* layer_info, void* workspace, size_t workspace_bytes,
+                          void* stream);
 
 /* ---------------------------------------------------------------------------
  * Forward dose.

@@ -19,10 +19,11 @@ range_table / csda_range        reference golden ranges 200/250/300 MeV; the
 energy_from_range               16-digit clinical_layer_energies endpoints;
                                 round trip R(E(R)) = R; reference fixtures
 straggling_bohr / sigma_range   published sigma_R at 100/150/200/230 MeV;
-                                reference fixtures
+                                reference fixtures of all three models
+                                (bohr, icru49, none)
 fe_moments                      constant-T closed forms (A0=Tz, A2=Tz^3/6,
                                 B=Tz^3/3); published sigma_x table at 200 MeV;
-                                reference fixtures
+                                reference fixtures, incl. single depths
 idd_pristine                    FFT == direct Gaussian convolution; energy
                                 conservation of the CSDA binning; published
                                 peak IDD values; reference fixtures
@@ -37,7 +38,14 @@ idd_depth_straggling            with no straggling it equals idd_pristine;
 separable_dose                  brute-force triple loop (tiny problems),
                                 single-spot analytic Gaussian, linearity,
                                 mirror symmetry, published peak dose 635.2707
-relative_error / gamma_index    hand-computed cases; gamma(ref, ref) == 0
+                                of the reference's benchmark problem, and
+                                reference fixtures of that problem's dose
+separable_dose_averaged,        reference fixtures (dose, sigma_bar) of the
+sigma_bar                       per_layer=False path; equals separable_dose
+                                when every layer has the same sigma
+gaussian_field_weights          reference fixtures (bit-equal weights)
+relative_error, mean_rel_error  hand-computed cases
+gamma_index                     hand-computed cases; gamma(ref, ref) == 0
 objective_grad_w                dot-product test and central differences
 ==============================  ==============================================
 """
@@ -315,14 +323,20 @@ def fine_count(z, n_sub=4, max_step=0.02):
 
 def fe_moments(z, E0, sigma_x0=0.0, sigma_theta0=0.0, rho0=0.0, n_sub=4,
                max_step=0.02, T_fn=None):
-    """Fermi-Eyges moments and lateral sigma on z (len(z) >= 2). physics.py:221-286
+    """Fermi-Eyges moments and lateral sigma on z. physics.py:221-286
 
     B_i = sum_{j : zf_i > zf_j} (zf_i - zf_j)^2 T_j h_j is the explicit double
     sum (not 2*A2). ``T_fn`` replaces the scattering power (closed-form tests).
+    A single depth z0 is evaluated as the last point of the grid [0, z0], or
+    [0, 1e-6] when z0 <= 0 (physics.py:247-249, :284-285).
     """
-    z = np.asarray(z, dtype=np.float64)
-    if z.size < 2:
-        raise ValueError("fe_moments needs at least two depths")
+    z = np.asarray(z, dtype=np.float64).reshape(-1)
+    if z.size == 0:
+        raise ValueError("fe_moments needs at least one depth")
+    if z.size == 1:
+        z2 = np.array([0.0, z[0]]) if z[0] > 0 else np.array([0.0, 1e-6])
+        res = fe_moments(z2, E0, sigma_x0, sigma_theta0, rho0, n_sub, max_step, T_fn)
+        return {k: (np.asarray(v)[-1:] if k != "n_fine" else v) for k, v in res.items()}
     nf = fine_count(z, n_sub, max_step)
     zf = np.linspace(z[0], z[-1], nf)
     hf = np.gradient(zf)
@@ -426,6 +440,43 @@ def separable_dose(x, y, sx, sy, w, idd, sigma, z_idx=None):
     return D
 
 
+def sigma_bar(w, sigma):
+    """Layer-weight-averaged sigma(z) of the single-sigma approximation:
+    sum_l (W_l / sum W) sigma_l(z), W_l = sum of layer l's weights. engines.py:75-80"""
+    w = np.asarray(w, dtype=np.float64)
+    wl = w.reshape(w.shape[0], -1).sum(axis=1)
+    wl = wl / max(wl.sum(), 1e-30)
+    return (np.asarray(sigma, dtype=np.float64) * wl[:, None]).sum(axis=0)
+
+
+def separable_dose_averaged(x, y, sx, sy, w, idd, sigma):
+    """FP64 single-sigma approximation (per_layer=False). engines.py:184-191
+
+    A(z) = sum_l IDD_l(z) W_l and one Gaussian width sigma_bar(z) per slice:
+    D[z] = Gy(z) A(z) Gx(z)^T."""
+    w = np.asarray(w, dtype=np.float64)
+    idd = np.asarray(idd, dtype=np.float64)
+    L, nz = idd.shape
+    A = (idd.T @ w.reshape(L, -1)).reshape(nz, w.shape[1], w.shape[2])
+    sb = sigma_bar(w, sigma)
+    Gx = lateral(x, sx, sb)                                  # (nz, nx, nsx)
+    Gy = lateral(y, sy, sb)                                  # (nz, ny, nsy)
+    T = np.einsum("zij,zvj->ziv", A, Gy)
+    return np.einsum("zai,ziy->zya", Gx, T).transpose(0, 2, 1)
+
+
+def gaussian_field_weights(n_layers, n_spot=41, spot_spacing=0.5, spot_sigma=0.28, seed=0):
+    """Weights of gaussian_impt_field: Gaussian envelope times (1 + 0.25 N(0,1))
+    from default_rng(seed), clipped at 0. model.py:207-220"""
+    rng = np.random.default_rng(seed)
+    s = (np.arange(n_spot) - (n_spot - 1) / 2.0) * spot_spacing
+    X, Y = np.meshgrid(s, s)
+    env = np.exp(-(X**2 + Y**2) / (2.0 * spot_sigma**2))
+    env = env / env.max()
+    jit = 1.0 + 0.25 * rng.standard_normal((n_layers, n_spot, n_spot))
+    return np.clip(env[None, :, :] * jit, 0.0, None)
+
+
 def brute_force_dose(x, y, z_count, sx, sy, w, idd, sigma):
     """Textbook voxel x spot loop (tiny problems only). engines.py:102-136"""
     w = np.asarray(w, dtype=np.float64)
@@ -463,7 +514,8 @@ def relative_error(a, b, mask=None):
 
 
 def mean_rel_error(a, b, thr=0.10):
-    """Mean and max pointwise relative error on voxels >= thr * max|b|."""
+    """Mean and max pointwise relative error |a - b| / |b| on the voxels with
+    |b| >= thr * max|b| (the dose-weighted region a 10 % gamma threshold keeps)."""
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     m = np.abs(b) >= thr * np.abs(b).max()

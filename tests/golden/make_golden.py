@@ -76,6 +76,53 @@ def main():
     p = md.gaussian_impt_field(np.array([100.0, 120.0, 140.0]), md.DoseGrid.centered(16, 0.4),
                                n_spot=9, spot_spacing=0.5, seed=3)
     out["gauss_w"], out["gauss_idd"], out["gauss_sigma"] = p.lattice.weights, p.idd, p.sigma
+    # the other field builder, a divergent beam and peak-normalised IDDs
+    p = md.flat_impt_field(np.array([95.0, 125.0, 155.0]), md.DoseGrid.centered(16, 0.4),
+                           n_spot=9, spot_spacing=0.5, field_half_width=1.5, seed=5)
+    out["flatf_w"], out["flatf_idd"], out["flatf_sigma"] = p.lattice.weights, p.idd, p.sigma
+    p = md.gaussian_impt_field(np.array([110.0, 170.0]), md.DoseGrid.centered(16, 0.4),
+                               n_spot=7, spot_spacing=0.5, seed=4, beam_sigma_x0=0.25,
+                               beam_sigma_theta0=0.003, fluence_normalized=False)
+    out["gaussd_w"], out["gaussd_idd"], out["gaussd_sigma"] = p.lattice.weights, p.idd, p.sigma
+
+    # the two other range-straggling models of IDDModel (physics.py:310-322)
+    e_s = np.array([70.0, 100.0, 150.0, 200.0, 230.0])
+    for name in ("icru49", "none"):
+        mdl = ph.IDDModel(straggling_model=name)
+        sr = np.array([mdl.sigma_range(e) for e in e_s])
+        out[f"{name}_E"], out[f"{name}_sigR"], out[f"{name}_strag"] = e_s, sr[:, 0], sr[:, 1]
+        z, en = cases["small"]
+        out[f"{name}_idd"] = np.stack([ph.idd_pristine(z, e, mdl, normalize=False)[0] for e in en])
+
+    # single-depth grids: fermi_eyges_moments integrates over [0, z0] (physics.py:247-249)
+    z1 = np.array([0.0, 0.2, 4.0, 12.3, 30.0])
+    e1 = np.array([90.0, 150.0, 210.0])
+    out["nz1_z"], out["nz1_E"] = z1, e1
+    out["nz1_sigma"] = np.array([[ph.fermi_eyges_moments(np.array([zz]), e, sigma_x0=0.30)["sigma_x"][0]
+                                  for zz in z1] for e in e1])
+    out["nz1_idd"] = np.array([[ph.idd_pristine(np.array([zz]), e, ph.DEFAULT_IDD, normalize=False)[0][0]
+                                for zz in z1] for e in e1])
+
+    # the reference's own benchmark problem, run_benchmark.py:33-38: 128^3 at 2 mm, 41 x 41
+    # spots at 5 mm, clinical_layer_energies(40, 3, 25.5), gaussian_impt_field seed 0;
+    # its FP64 separable dose (engines.py:172-202) as the peak, per-slice sums and samples
+    eng = importlib.import_module("pbdose_ref.engines")
+    bp = md.gaussian_impt_field(md.clinical_layer_energies(40, 3.0, 25.5), md.DoseGrid.centered(128, 0.2),
+                                n_spot=41, spot_spacing=0.5, seed=0)
+    D = eng.separable_numpy(bp)
+    rng = np.random.default_rng(11)
+    flat = rng.choice(D.size, 4096, replace=False)
+    hot = np.argsort(D.ravel())[-256:]
+    idx = np.unique(np.concatenate([flat, hot]))
+    out["bp_idd"], out["bp_sigma"] = bp.idd, bp.sigma
+    out["bp_max"] = np.array([D.max()])
+    out["bp_slice_sum"] = D.sum(axis=(1, 2))
+    out["bp_idx"], out["bp_val"] = idx.astype(np.int64), D.ravel()[idx]
+    # the averaged-sigma approximation (per_layer=False) on a small field
+    sp = md.gaussian_impt_field(np.array([100.0, 120.0, 140.0]), md.DoseGrid.centered(16, 0.4),
+                                n_spot=9, spot_spacing=0.5, seed=3)
+    out["avg_dose"] = eng.separable_numpy(sp, per_layer=False)
+    out["avg_sigma_bar"] = eng.layerwise_sigma_bar(sp)
     np.savez_compressed(OUT, **out)
     print(f"wrote {OUT}: {os.path.getsize(OUT)} bytes, {len(out)} arrays")
 

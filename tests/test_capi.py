@@ -78,22 +78,35 @@ def test_forward_workspace_query():
 
 def test_lut_workspace_query():
     from pbdose import physics
+    L = _lib.lib()
     m = physics.beam_model()
-    n = _lib.lib().pbh_lut_workspace_bytes(m, 128, 40)
+    n = L.pbh_lut_workspace_bytes(m, 128, 0.0, 0.2, 40)
     assert n > 40 * 6000 * 16   # binning + Gaussian kernel per layer
+    # the Fermi-Eyges fine grid follows the depth span: a coarse grid needs more
+    assert L.pbh_lut_workspace_bytes(m, 128, 0.0, 3.0, 40) > n
+    # one depth: the moment grid is [0, z0] (physics.py:247-249)
+    assert L.pbh_lut_workspace_bytes(m, 1, 12.3, 1.0, 40) > 0
+    assert L.pbh_lut_workspace_bytes(m, 0, 0.0, 0.2, 40) == 0
+    assert L.pbh_lut_workspace_bytes(m, 128, 0.0, 0.0, 40) == 0
+    # the tangent build holds a derivative beside every table entry
+    assert L.pbh_lut_tangent_workspace_bytes(m, 128, 0.0, 0.2, 40) > n
     bad = physics.beam_model()
     bad.n_fine_idd = 1
-    assert _lib.lib().pbh_lut_workspace_bytes(bad, 128, 40) == 0
-    assert _lib.lib().pbh_build_lut(m, 128, 0.0, 0.2, 40, None, None, None, None, None, 0, None) == -1
+    assert L.pbh_lut_workspace_bytes(bad, 128, 0.0, 0.2, 40) == 0
+    assert L.pbh_build_lut(m, 128, 0.0, 0.2, 40, None, None, None, None, None, None, None, 0,
+                           None) == -1
+    assert L.pbh_build_lut_tangent(m, 128, 0.0, 0.2, 40, None, None, None, None, None, None, None,
+                                   None, None, None, 0, None) == -1
+    assert L.pbh_interp_table_slope(None, None, 10, None, None, None, 1, None) == -1
     # depth-dependent straggling: extra cumulative-integral and amplitude tables
     d = physics.beam_model(physics.IDDModel(depth_straggling=True))
     assert d.straggling_depth == 1
-    assert _lib.lib().pbh_lut_workspace_bytes(d, 128, 40) >= n + 40 * (4000 + 6000) * 8
+    assert L.pbh_lut_workspace_bytes(d, 128, 0.0, 0.2, 40) >= n + 40 * (4000 + 6000) * 8
     icru = physics.beam_model(physics.IDDModel(straggling_model="icru49"))
     icru.straggling_depth = 1                  # defined for the Bohr integral only
-    assert _lib.lib().pbh_lut_workspace_bytes(icru, 128, 40) == 0
+    assert L.pbh_lut_workspace_bytes(icru, 128, 0.0, 0.2, 40) == 0
     d.straggling_depth = 2
-    assert _lib.lib().pbh_lut_workspace_bytes(d, 128, 40) == 0
+    assert L.pbh_lut_workspace_bytes(d, 128, 0.0, 0.2, 40) == 0
 
 
 def test_product_refuses_cpu_device():

@@ -161,32 +161,36 @@ def test_clinical_objective_gradients_vs_finite_differences():
         fd_w.append(fd)
         g_w.append(gw[l, j, i])
     ew = rel(g_w, fd_w)
-    # range shifts: FP64 central differences of J, and the FP64 chain rule through
-    # the same h = 1e-3 secants of the look-up tables
+    # range shifts: the FP64 chain rule through the device's tangent tables (checks
+    # the adjoint's table gradients and the depth contraction), and central
+    # differences of the FP64 objective
     R = 2.0 * wv * (Dref - Dp)
-    fd_d, ch_d, g_d = [], [], []
+    didd_t, dsig_t = (t.cpu().numpy() for t in fm.lut_derivatives(None))
+    ch_d, g_d, fd_d = [], [], {h: [] for h in (1e-3, 2e-6, 1e-6)}
     for l in (0, 9, 17, 25, 33, 39):
         Dl_old = O.separable_dose(x, y, sx, sx, w32[l:l + 1], idd[l:l + 1], sig[l:l + 1])
-        parts, luts = [], []
-        for sgn in (+1, -1):
-            i_s, s_s = O.layer_luts(z, O.shifted_energies(E[l:l + 1], [sgn * h]))
-            luts.append((i_s, s_s))
-            Dl_new = O.separable_dose(x, y, sx, sx, w32[l:l + 1], i_s, s_s)
-            parts.append(O.objective(Dref - Dl_old + Dl_new, Dp, wv))
-        fd_d.append((parts[0] - parts[1]) / (2 * h))
+        for hh in fd_d:
+            parts = []
+            for sgn in (+1, -1):
+                i_s, s_s = O.layer_luts(z, O.shifted_energies(E[l:l + 1], [sgn * hh]))
+                Dl_new = O.separable_dose(x, y, sx, sx, w32[l:l + 1], i_s, s_s)
+                parts.append(O.objective(Dref - Dl_old + Dl_new, Dp, wv))
+            fd_d[hh].append((parts[0] - parts[1]) / (2 * hh))
         gi, gs = O.adjoint_lut(x, y, sx, sx, w32[l:l + 1], idd[l:l + 1], sig[l:l + 1], R)
-        ch_d.append(float(np.sum(gi * (luts[0][0] - luts[1][0]) + gs * (luts[0][1] - luts[1][1]))) / (2 * h))
+        ch_d.append(float(np.sum(gi * didd_t[l:l + 1] + gs * dsig_t[l:l + 1])))
         g_d.append(gd[l])
-    ec, ed = rel(g_d, ch_d), rel(g_d, fd_d)
+    rich = [(4.0 * a - b) / 3.0 for a, b in zip(fd_d[1e-6], fd_d[2e-6])]
+    ec, er, e3 = rel(g_d, ch_d), rel(g_d, rich), rel(g_d, fd_d[1e-3])
     print(f"  dJ/dw vs FD (64 spots) {ew:.2e}; dJ/ddelta vs FP64 chain rule {ec:.2e}, "
-          f"vs FD {ed:.2e}", flush=True)
+          f"vs FD h->0 (Richardson of 2e-6, 1e-6) {er:.2e}, vs FD h = 1e-3 {e3:.2e}", flush=True)
     assert ew <= 2e-5
     assert ec <= 5e-5
-    # J(delta) is not smooth on the 1e-3 cm scale: the reference's CSDA binning
-    # (physics.py:366-370) moves 0.5 MeV between adjacent bins whenever a bin edge
-    # crosses R0, so a secant over +-1e-3 cm and the global difference can differ by
-    # ~1e-3 relative for a layer whose edge crossing falls inside the interval.
-    assert ed <= 2e-3
+    # J(delta) is piecewise smooth with dense breaks (CSDA bin edges, fine-grid
+    # points and interpolation brackets crossing R0 and the depths,
+    # physics.py:366-370), so differences over +-1e-3 cm average the slope over
+    # many pieces, and the analytic (local) slope departs from that average
+    assert er <= 1e-3
+    assert e3 <= 1e-2
 
 
 def test_autograd_matches_torch_restatement_of_reference_engine():

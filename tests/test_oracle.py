@@ -357,3 +357,116 @@ def test_spot_dose_is_unit_weight_column_of_separable():
     D = O.separable_dose(x, y, sx, sy, e, idd, sig)
     np.testing.assert_allclose(O.spot_dose(x, y, sx[2], sy[1], idd[1], sig[1]), D,
                                rtol=1e-13, atol=1e-16)
+
+
+# ------------------------------------------- straggling models, single depths
+@pytest.mark.parametrize("model", ["icru49", "none"])
+def test_other_straggling_models_match_reference(ref_physics, model):
+    """IDDModel('icru49') and IDDModel('none') (physics.py:310-322): sigma_R and
+    its straggling part, and the pristine IDD built with them."""
+    for e, sr, st in zip(ref_physics[f"{model}_E"], ref_physics[f"{model}_sigR"],
+                         ref_physics[f"{model}_strag"]):
+        got = O.sigma_range(e, straggling=model)
+        assert got[0] == sr and got[1] == st
+    z, E = ref_physics["small_z"], ref_physics["small_E"]
+    idd = np.stack([O.idd_pristine(z, e, straggling=model) for e in E])
+    np.testing.assert_allclose(idd, ref_physics[f"{model}_idd"], rtol=0,
+                               atol=1e-13 * np.abs(ref_physics[f"{model}_idd"]).max())
+
+
+def test_single_depth_matches_reference(ref_physics):
+    """One depth z0: moments over [0, z0], [0, 1e-6] for z0 <= 0 (physics.py:247-249)."""
+    for i, e in enumerate(ref_physics["nz1_E"]):
+        for k, z in enumerate(ref_physics["nz1_z"]):
+            assert O.fe_moments([z], e, 0.30)["sigma_x"][0] == ref_physics["nz1_sigma"][i, k]
+            assert O.idd_pristine([z], e)[0] == ref_physics["nz1_idd"][i, k]
+
+
+# -------------------------------------------------------- field weight builders
+def test_field_weight_builders_match_reference(ref_physics):
+    """The product's host-side weight builders draw the seeded RNG in the
+    reference's order (model.py:207-220, :258-287): weights bit-equal."""
+    from pbdose import model
+    np.testing.assert_array_equal(model.flat_field_weights(21, 21, 10, seed=0), ref_physics["flat_w"])
+    np.testing.assert_array_equal(
+        model.flat_field_weights(9, 9, 3, field_half_width=1.5, spot_spacing=0.5, seed=5),
+        ref_physics["flatf_w"])
+    np.testing.assert_array_equal(model.gaussian_field_weights(3, 9, 0.5, seed=3), ref_physics["gauss_w"])
+    np.testing.assert_array_equal(model.gaussian_field_weights(2, 7, 0.5, seed=4), ref_physics["gaussd_w"])
+    np.testing.assert_array_equal(O.gaussian_field_weights(3, 9, 0.5, seed=3), ref_physics["gauss_w"])
+    with pytest.raises(ValueError):
+        model.flat_field_weights(9, 7, 3)
+
+
+# -------------------------------------------- the reference's benchmark problem
+def test_reference_benchmark_problem_peak_dose(ref_physics):
+    """run_benchmark.py:33-38: 128^3 at 2 mm, 41 x 41 spots at 5 mm, 40 clinical
+    layers, gaussian_impt_field seed 0. The oracle's FP64 dose peaks at the
+    published 635.2707 (02_performance_report.pdf section 1.2) and equals the
+    reference's own separable dose at sampled voxels and in every slice sum."""
+    g = O.centered_grid(128, 0.2)
+    x, y, z = O.grid_axes(g)
+    E = O.clinical_layer_energies(40, 3.0, 25.5)
+    idd, sig = O.layer_luts(z, E)
+    np.testing.assert_array_equal(idd, ref_physics["bp_idd"])
+    np.testing.assert_array_equal(sig, ref_physics["bp_sigma"])
+    w = O.gaussian_field_weights(40, 41, 0.5, seed=0)
+    s = O.spot_axis(41, 0.5)
+    D = O.separable_dose(x, y, s, s, w, idd, sig)
+    assert round(float(D.max()), 4) == 635.2707
+    assert abs(D.max() / ref_physics["bp_max"][0] - 1.0) <= 1e-13
+    np.testing.assert_allclose(D.ravel()[ref_physics["bp_idx"]], ref_physics["bp_val"], rtol=0,
+                               atol=1e-13 * D.max())
+    np.testing.assert_allclose(D.sum(axis=(1, 2)), ref_physics["bp_slice_sum"], rtol=1e-12)
+
+
+def test_averaged_sigma_dose_matches_reference(ref_physics):
+    """per_layer=False (engines.py:184-191) on the small Gaussian field."""
+    g = O.centered_grid(16, 0.4)
+    x, y, _ = O.grid_axes(g)
+    s = O.spot_axis(9, 0.5)
+    w = ref_physics["gauss_w"]
+    np.testing.assert_allclose(O.sigma_bar(w, ref_physics["gauss_sigma"]), ref_physics["avg_sigma_bar"],
+                               rtol=1e-15)
+    D = O.separable_dose_averaged(x, y, s, s, w, ref_physics["gauss_idd"], ref_physics["gauss_sigma"])
+    ref = ref_physics["avg_dose"]
+    assert np.abs(D - ref).max() <= 1e-13 * np.abs(ref).max()
+    # with one sigma for all layers the approximation is exact
+    same = np.broadcast_to(ref_physics["gauss_sigma"][1], ref_physics["gauss_sigma"].shape)
+    De = O.separable_dose(x, y, s, s, w, ref_physics["gauss_idd"], same)
+    Da = O.separable_dose_averaged(x, y, s, s, w, ref_physics["gauss_idd"], same)
+    assert np.abs(De - Da).max() <= 1e-13 * De.max()
+
+
+# ------------------------------------------------------------- error metrics
+def test_mean_rel_error_hand_case():
+    b = np.array([10.0, 5.0, 0.5, -2.0, 0.0])
+    a = np.array([11.0, 5.0, 1.0, -2.5, 3.0])
+    # voxels with |b| >= 1: 10, 5, -2 -> relative errors 0.1, 0, 0.25
+    mean, mx = O.mean_rel_error(a, b, thr=0.10)
+    assert abs(mean - 0.35 / 3) < 1e-15 and mx == 0.25
+    mean2, mx2 = O.mean_rel_error(a, b, thr=0.45)     # |b| >= 4.5: 0.1 and 0
+    assert mean2 == 0.05 and mx2 == 0.1
+
+
+def test_product_relative_error_is_the_reference_metric():
+    """pbdose.engines.relative_error (engines.py:364-378): same keys and values
+    as the hand case, with a mask, with tensors, and on an empty mask."""
+    import torch
+    from pbdose import engines
+    a = np.array([1.0, 2.0, 3.5])
+    b = np.array([1.0, 2.5, 4.0])
+    r = engines.relative_error(a, b)
+    assert set(r) == {"max_abs_err", "max_rel_err_global", "rms_err", "rms_rel_global", "n_vox"}
+    assert r["max_abs_err"] == 0.5 and r["max_rel_err_global"] == 0.125 and r["n_vox"] == 3
+    assert abs(r["rms_err"] - math.sqrt(0.5 / 3)) < 1e-15
+    assert abs(r["rms_rel_global"] - math.sqrt(0.5 / 3) / 4.0) < 1e-15
+    m = np.array([True, False, True])
+    rm = engines.relative_error(torch.tensor(a), torch.tensor(b), m)
+    assert rm["max_abs_err"] == 0.5 and rm["n_vox"] == 2 and rm["max_rel_err_global"] == 0.125
+    assert abs(rm["rms_err"] - math.sqrt(0.25 / 2)) < 1e-15
+    re = engines.relative_error(a, b, np.zeros(3, dtype=bool))
+    assert re["max_abs_err"] == 0.0 and re["n_vox"] == 0
+    rng = np.random.default_rng(2)
+    x, y = rng.random((4, 5)), rng.random((4, 5))
+    assert engines.relative_error(x, y) == O.relative_error(x, y)
