@@ -185,12 +185,13 @@ def test_clinical_objective_gradients_vs_finite_differences():
           f"vs FD h->0 (Richardson of 2e-6, 1e-6) {er:.2e}, vs FD h = 1e-3 {e3:.2e}", flush=True)
     assert ew <= 2e-5
     assert ec <= 5e-5
-    # J(delta) is piecewise smooth with dense breaks (CSDA bin edges, fine-grid
-    # points and interpolation brackets crossing R0 and the depths,
-    # physics.py:366-370), so differences over +-1e-3 cm average the slope over
-    # many pieces, and the analytic (local) slope departs from that average
-    assert er <= 1e-3
-    assert e3 <= 1e-2
+    # J(delta) is piecewise smooth with dense breaks (range-table knots crossing
+    # CSDA bin edges, fine-grid points crossing R0, physics.py:255-264, :359-370),
+    # some 1e-5 cm apart, so differences over +-1e-3 cm average the slope over
+    # many pieces: the analytic (local) slope departs from that average by a few
+    # per cent (measured 2.6e-2) while it equals the limit h -> 0 (measured 2.9e-7)
+    assert er <= 1e-5
+    assert e3 <= 5e-2
 
 
 def test_autograd_matches_torch_restatement_of_reference_engine():

@@ -95,32 +95,33 @@ def test_depth_straggling_small_spreads_and_no_straggling():
         physics.beam_model(physics.IDDModel(depth_straggling=True, straggling_model="icru49"))
 
 
-def _oracle_fd(z, E, h, model="bohr", depth=False, sigma_x0=0.30):
-    """Central difference of the oracle's tables in a range shift of every layer."""
-    out = []
-    for s in (+h, -h):
-        Es = O.shifted_energies(E, np.full(len(E), s))
-        if depth:
-            idd = np.stack([O.idd_depth_straggling(z, e) for e in Es])
-        else:
-            idd = np.stack([O.idd_pristine(z, e, straggling=model) for e in Es])
-        sig = np.stack([O.fe_moments(z, e, sigma_x0)["sigma_x"] for e in Es])
-        out.append((idd, sig))
-    return (out[0][0] - out[1][0]) / (2 * h), (out[0][1] - out[1][1]) / (2 * h)
+def _oracle_tables(z, E, shift, model="bohr", depth=False, sigma_x0=0.30):
+    """The oracle's tables with every layer's range shifted by `shift` cm."""
+    Es = O.shifted_energies(E, np.full(len(E), shift))
+    if depth:
+        idd = np.stack([O.idd_depth_straggling(z, e) for e in Es])
+    else:
+        idd = np.stack([O.idd_pristine(z, e, straggling=model) for e in Es])
+    sig = np.stack([O.fe_moments(z, e, sigma_x0)["sigma_x"] for e in Es])
+    return idd, sig
 
 
 @pytest.mark.parametrize("case", ["c2", "icru49", "depth"])
 def test_tangent_tables_vs_converged_differences(ref_physics, case):
     """pbh_build_lut_tangent: its tables equal pbh_build_lut's bit for bit, and
     its derivatives in a range shift (direction dE/dR from pbh_interp_table_slope)
-    equal the limit of the oracle's central differences.
+    equal the limit of the oracle's finite differences.
 
     The tables are piecewise smooth in the shift with dense breaks: every
-    interpolation bracket, CSDA bin edge and fine-grid point that crosses a
-    depth or R0 adds a kink or a step. So the check is made where the
-    differences have converged: entries whose central differences at h = 2e-6
-    and 1e-6 cm agree to 1e-8 of the layer's largest derivative, compared with
-    their Richardson extrapolation. Most entries qualify (asserted)."""
+    range-table knot that crosses a CSDA bin edge, every fine-grid point that
+    crosses R0 (the scattering power drops to 0 there) and every interpolation
+    bracket that crosses a depth adds a kink or a step, some 1e-5 cm apart. So
+    the derivative is checked where the oracle's differences at h = 1e-6 cm show
+    no break within 2h: the central differences at h and 2h agree, and so do the
+    one-sided second differences (a kink at the point itself shows only there),
+    each to 1e-8 of the layer's largest derivative. There the derivative must
+    equal the Richardson extrapolation of the central differences. Most entries
+    qualify (asserted)."""
     from pbdose import physics
     z = ref_physics["c2_z"]
     if case == "c2":
@@ -143,17 +144,21 @@ def test_tangent_tables_vs_converged_differences(ref_physics, case):
         assert torch.equal(u, v)
     ad = [t[3].cpu().numpy(), t[4].cpu().numpy()]
     mname = "icru49" if case == "icru49" else "bohr"
-    f1 = _oracle_fd(z, E, 1e-6, mname, depth)
-    f2 = _oracle_fd(z, E, 2e-6, mname, depth)
+    h = 1e-6
+    v = {s: _oracle_tables(z, E, s * h, mname, depth) for s in (-2, -1, 0, 1, 2)}
     for q, name in ((0, "idd"), (1, "sigma")):
-        sc = np.abs(f1[q]).max(axis=1, keepdims=True)
+        c1 = (v[1][q] - v[-1][q]) / (2 * h)
+        c2 = (v[2][q] - v[-2][q]) / (4 * h)
+        k1 = (v[1][q] - 2 * v[0][q] + v[-1][q]) / h          # h f'' when smooth
+        k2 = (v[2][q] - 2 * v[0][q] + v[-2][q]) / (2 * h)    # 2 h f''
+        sc = np.abs(c1).max(axis=1, keepdims=True)
         sc = np.where(sc > 0, sc, 1.0)
-        ok = np.abs(f2[q] - f1[q]) <= 1e-8 * sc
-        rich = (4.0 * f1[q] - f2[q]) / 3.0
+        ok = (np.abs(c2 - c1) <= 1e-8 * sc) & (np.abs(k2 - 2 * k1) <= 1e-8 * sc)
+        rich = (4.0 * c1 - c2) / 3.0
         err = np.abs(ad[q] - rich) / sc
         frac = float(ok.mean())
         e_ok = float(err[ok].max())
-        print(f"  d{name}/ddelta: {frac:.3f} of entries converged; max error there {e_ok:.2e}; "
-              f"median error everywhere {float(np.median(err)):.2e}", flush=True)
-        assert frac >= (0.9 if q == 0 else 0.6)
+        print(f"  d{name}/ddelta: {frac:.3f} of entries without a break within 2h; max error there "
+              f"{e_ok:.2e}; median error everywhere {float(np.median(err)):.2e}", flush=True)
+        assert frac >= (0.85 if q == 0 else 0.6)
         assert e_ok <= 1e-7
