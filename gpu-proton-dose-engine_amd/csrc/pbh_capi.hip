@@ -12,6 +12,8 @@ extern "C" const char* pbh_strerror(int code)
     case PBH_ERR_UNSUPPORTED: return "unsupported configuration";
     case PBH_ERR_NO_SMEM: return "insufficient LDS per workgroup";
     case PBH_ERR_WORKSPACE: return "workspace too small (query the matching *_workspace_bytes)";
+    case PBH_ERR_SCRATCH: return "problem needs more than the fixed scratch of pbh_forward_noalloc "
+                                 "(PBH_NOALLOC_SCRATCH_BYTES)";
     default: return "unknown error";
     }
 }

@@ -14,6 +14,7 @@ _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libpbdose_hip.so")
 
 PBH_OK = 0
+PBH_NOALLOC_SCRATCH_BYTES = 32 << 20      # include/pbdose_hip.h
 PREC = {"fp32": 0, "bf16": 1, "fp16": 2, "bf16x3": 3, "fp16x3": 4,
         "bf16x3_peak": 5, "fp16x3_peak": 6}
 
@@ -75,6 +76,7 @@ SIGNATURES = {
     "pbh_forward": (C.c_int, [_G, _L, _P, _P, _P, _P, C.c_int, _P, C.c_size_t, _P]),
     "pbh_forward_tuned": (C.c_int, [_G, _L, _P, _P, _P, _P, C.c_int, C.c_int, _P, C.c_size_t, _P]),
     "pbh_forward_choose_chunk": (C.c_int, [_G, _L, C.c_int, _P, C.c_int]),
+    "pbh_forward_noalloc": (C.c_int, [_G, _L, _P, _P, _P, _P, C.c_int, C.c_int, _P]),
     "pbh_adjoint_workspace_bytes": (C.c_size_t, [_G, _L, C.c_int]),
     "pbh_adjoint": (C.c_int, [_G, _L, _P, _P, _P, _P, _P, _P, _P, C.c_int, _P, C.c_size_t, _P]),
     "pbh_adjoint_tuned": (C.c_int, [_G, _L, _P, _P, _P, _P, _P, _P, _P, C.c_int, C.c_int, _P, C.c_size_t, _P]),

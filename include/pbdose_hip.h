@@ -55,6 +55,7 @@ extern "C" {
 #define PBH_ERR_UNSUPPORTED   -5
 #define PBH_ERR_NO_SMEM       -6
 #define PBH_ERR_WORKSPACE     -7
+#define PBH_ERR_SCRATCH       -8   /* problem too large for pbh_forward_noalloc's fixed scratch */
 
 /* ---- arithmetic forms of the dose contraction ---- */
 #define PBH_PREC_FP32        0  /* FP32 factors, FP32 MFMA, FP32 accumulation */
@@ -215,6 +216,23 @@ int pbh_forward_tuned(const PbhGrid* g, const PbhLattice* lat,
  * layer_chunk, engines.py:208, bounds memory, not load balance). */
 int pbh_forward_choose_chunk(const PbhGrid* g, const PbhLattice* lat, int precision,
                              const int32_t* active_per_z, int compute_units);
+
+/* pbh_forward_tuned without a workspace argument and without any allocation.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). The workspace is a fixed device array of
+ * PBH_NOALLOC_SCRATCH_BYTES inside the library, one per GPU, held until the
+ * process ends; a problem whose forward workspace
+ * (pbh_forward_workspace_bytes_tuned with the same layer_chunk) is larger
+ * returns PBH_ERR_SCRATCH (the 128^3, 41 x 41 x 40-layer field needs about
+ * 17 MB at its automatic chunk). Calls share their GPU's array: calls that
+ * overlap on one GPU race, so serialise them (one stream, or events). The first
+ * call on a device resolves the array's address on that device (no allocation).
+ * Results equal pbh_forward_tuned's bit for bit. */
+#define PBH_NOALLOC_SCRATCH_BYTES (32u << 20)
+int pbh_forward_noalloc(const PbhGrid* g, const PbhLattice* lat,
+                        const float* w, const double* idd, const double* sigma,
+                        float* dose, int precision, int layer_chunk, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Adjoint of the forward map (inverse planning).
