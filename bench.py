@@ -49,6 +49,12 @@ import torch  # noqa: E402
 # profiles/counters_forward_adjoint_128cube_67240spots.md
 TRAFFIC_KIB = {"fp32": (5185.5, 15232.0), "bf16x3": (8600.5, 20943.6), "bf16": (8121.8, 27087.6)}
 CALL_KERNELS = {"fp32": "fwd_fp32_kernel + combine_kernel"}
+KERNEL_NAME = {"fp32": "fwd_fp32_kernel"}
+# MFMA FLOP the contraction kernel issues per launch on this workload (not the
+# algorithmic count: windows skip K steps, the x3 forms run three products), from
+# the rocprofv3 counter SQ_INSTS_VALU_MFMA_MOPS_* (units of 512 FLOP) in the
+# counter profile under profiles/; None where not profiled
+MFMA_FLOP_ISSUED: dict = {}
 FETCH_SCALE, WRITE_SCALE = 2.000, 1.000
 
 # published dense MFMA peaks of MI355X (TFLOP/s) for the arithmetic each form runs on
@@ -100,9 +106,23 @@ def call_events(fn, steps):
     return float(np.mean([s.elapsed_time(e) for s, e in ev]))
 
 
+def kernel_events(dp, precision, steps, out):
+    """Mean duration of the fused contraction kernel alone: pbh_forward_profiled
+    records a pair of events on the launch stream right around it, per call."""
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    for s, e in ev:                       # create the events before handing them over
+        s.record()
+        e.record()
+    for s, e in ev:
+        dp.forward(precision, out=out, kernel_events=(s, e))
+    torch.cuda.synchronize()
+    return float(np.mean([s.elapsed_time(e) for s, e in ev]))
+
+
 def time_forward(dp, precision, steps, warmup):
     """Returns (dose, wall ms per step over `steps` back-to-back steps, ms per call
-    from per-call events)."""
+    from per-call events, ms of the contraction kernel alone)."""
     out = torch.empty(dp.shape, dtype=torch.float32, device=dp.dev)
     for _ in range(warmup):
         dp.forward(precision, out=out)
@@ -113,7 +133,8 @@ def time_forward(dp, precision, steps, warmup):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     launch_ms = call_events(lambda: dp.forward(precision, out=out), steps)
-    return out, wall * 1e3 / steps, launch_ms
+    kernel_ms = kernel_events(dp, precision, steps, out)
+    return out, wall * 1e3 / steps, launch_ms, kernel_ms
 
 
 def accuracy(ref64, D):
@@ -200,31 +221,58 @@ def time_robust(precision, steps, objective):
     return out
 
 
-def cpu_leg(prob, budget_s=20.0, max_reps=15):
+def host_cores():
+    """(logical CPUs, physical cores, CPUs this process may run on)."""
+    logical = os.cpu_count() or 1
+    try:
+        import psutil
+        physical = psutil.cpu_count(logical=False) or logical
+    except Exception:
+        physical = logical
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except Exception:
+        allowed = logical
+    return logical, physical, allowed
+
+
+def cpu_leg(prob, budget_s=12.0, max_reps=15):
     """CPU baseline: the host restatement of the reference's batched-matmul engine
-    (FP64, all host threads) on full forwards of the field. Its output is the FP64
-    reference dose the GPU forms are checked against."""
+    (oracle/torch_cpu.py: broadcast exp factors, two torch.bmm, layer sum) on full
+    forwards of the field, in FP64 (the reference's CPU default) and FP32, with the
+    reference's protocol (2 warm-up runs, median of up to 15, time.perf_counter).
+    Threads: the host share of one GPU on this machine, min(16, OMP_NUM_THREADS,
+    CPUs), whatever os.cpu_count() reports; both counts are stated. Its FP64 output
+    is the reference dose the GPU forms are checked against."""
     from oracle.torch_cpu import torch_separable_cpu
-    # the host share of one GPU: at most 16 threads, whatever os.cpu_count() reports
-    nthreads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS") or 16), os.cpu_count() or 1))
+    logical, physical, allowed = host_cores()
+    nthreads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS") or 16), allowed))
     torch.set_num_threads(nthreads)
     g, lat = prob.grid, prob.lattice
     w32 = lat.weights.astype(np.float32).astype(np.float64)     # the weights the GPU sees
-    args = (g.x, g.y, lat.spot_x, lat.spot_y, w32, prob.idd, prob.sigma)
-    ref = None
-    for _ in range(2):
-        ref = torch_separable_cpu(*args)
-    ts = []
-    t_start = time.perf_counter()
-    while len(ts) < max_reps and (time.perf_counter() - t_start) < budget_s:
-        t0 = time.perf_counter()
-        torch_separable_cpu(*args)
-        ts.append(time.perf_counter() - t0)
-    med = float(np.median(ts))
+    rows, ref = {}, None
+    for name, dt in (("fp64", torch.float64), ("fp32", torch.float32)):
+        args = (g.x, g.y, lat.spot_x, lat.spot_y, w32, prob.idd, prob.sigma)
+        for _ in range(2):
+            out = torch_separable_cpu(*args, dtype=dt)
+        if dt == torch.float64:
+            ref = out
+        ts = []
+        t_start = time.perf_counter()
+        while len(ts) < max_reps and (time.perf_counter() - t_start) < budget_s:
+            t0 = time.perf_counter()
+            torch_separable_cpu(*args, dtype=dt)
+            ts.append(time.perf_counter() - t0)
+        rows[name] = {"median_ms": float(np.median(ts)) * 1e3, "min_ms": min(ts) * 1e3, "reps": len(ts)}
+    med = rows["fp64"]["median_ms"] * 1e-3
     base = {"value": 1.0 / med, "unit": "fields/s", "cores": nthreads, "kind": "port",
-            "sample": f"{len(ts)} full forwards of the same 128^3 / 67,240-spot field, FP64 "
-                      f"torch bmm on the host (median {med * 1e3:.1f} ms, min "
-                      f"{min(ts) * 1e3:.1f} ms; 2 warm-up runs)"}
+            "sample": f"{rows['fp64']['reps']} full forwards of the same 128^3 / 67,240-spot field, FP64 "
+                      f"torch bmm on the host (median {rows['fp64']['median_ms']:.1f} ms, min "
+                      f"{rows['fp64']['min_ms']:.1f} ms; 2 warm-up runs) on {nthreads} threads",
+            "host_logical_cpus": logical, "host_physical_cores": physical, "cpus_allowed": allowed,
+            "threads_note": "threads = the host share of one GPU on this machine (16), not all its CPUs",
+            "fp32": {"value": 1e3 / rows["fp32"]["median_ms"], "unit": "fields/s", **rows["fp32"]},
+            "fp64": rows["fp64"]}
     return base, ref.numpy()
 
 
@@ -282,11 +330,25 @@ def main():
     g, lat = prob.grid, prob.lattice
     flops = field_flops(g.nx, g.ny, g.nz, lat.n_sx, lat.n_sy, lat.n_layers)
 
-    D, ms_step, launch_ms = time_forward(dp, args.precision, args.steps, args.warmup)
-    if args.precision != "fp32":
-        D32, ms32, launch32 = time_forward(dp, "fp32", args.steps, args.warmup)
-    else:
-        D32, ms32, launch32 = D, ms_step, launch_ms
+    # every arithmetic form: back-to-back steps, per-call events, the contraction
+    # kernel alone; the selected form is the headline
+    forms = ["fp32", "bf16", "fp16", "bf16x3", "fp16x3", "bf16x3_peak", "fp16x3_peak"]
+    if args.precision not in forms:
+        raise SystemExit(f"unknown precision {args.precision}")
+    per_form, doses = {}, {}
+    for f in forms:
+        steps = args.steps if f in (args.precision, "fp32") else max(20, args.steps // 4)
+        D, ms, call_ms, kern_ms = time_forward(dp, f, steps, args.warmup)
+        doses[f] = D
+        per_form[f] = {"ms_per_step": ms, "fields_per_s": 1e3 / ms, "call_us": call_ms * 1e3,
+                       "kernel_us": kern_ms * 1e3, "steps": steps,
+                       "kernel_achieved_tflop_s": flops / (kern_ms * 1e-3) / 1e12,
+                       "kernel_frac_of_peak": flops / (kern_ms * 1e-3) / 1e12 / PEAK[base_type(f)],
+                       "accuracy_vs_fp64": None}
+    ms_step, launch_ms = per_form[args.precision]["ms_per_step"], per_form[args.precision]["call_us"] * 1e-3
+    kernel_ms = per_form[args.precision]["kernel_us"] * 1e-3
+    ms32 = per_form["fp32"]["ms_per_step"]
+    D, D32 = doses[args.precision], doses["fp32"]
     adj32_ms, adj32_ev, g32 = time_adjoint(dp, D32, max(5, args.steps // 4), 3)
     aform = adjoint_form(args.precision)
     if aform != "fp32":
@@ -297,9 +359,11 @@ def main():
     pcie_ms = time_torch_separable(prob, args.precision, max(5, args.steps // 10))
     bt = base_type(args.precision)
     peak = PEAK[bt]
-    achieved = flops / (launch_ms * 1e-3) / 1e12
+    achieved_k = flops / (kernel_ms * 1e-3) / 1e12
+    achieved_c = flops / (launch_ms * 1e-3) / 1e12
+    issued = MFMA_FLOP_ISSUED.get(args.precision)
     line = {
-        "metric": "dose fields/s at 128^3 x 67,240 spots (forward); fwd+adjoint ms; mixed vs FP32",
+        "metric": "dose fields/s at 128\u00b3\u00d767,240 spots; fwd+adjoint ms; mixed vs FP32 @\u22641e-5 rel",
         "value": 1e3 / ms_step,
         "unit": "fields/s",
         "n_gpus": 1,
@@ -315,9 +379,10 @@ def main():
                                "70-180 MeV (67,240 spots), forward dose",
                    "precision": args.precision},
         "accuracy_vs_fp64": None,
-        "fp32": {"fields_per_s": 1e3 / ms32, "ms": ms32, "launch_ms": launch32,
+        "fp32": {"fields_per_s": 1e3 / ms32, "ms": ms32, "launch_ms": per_form["fp32"]["call_us"] * 1e-3,
                  "accuracy_vs_fp64": None},
         "speedup_vs_fp32": ms32 / ms_step,
+        "forms": per_form,
         "adjoint_form": aform,
         "adjoint_ms": adj_ms,
         "adjoint_ms_fp32": adj32_ms,
@@ -329,11 +394,21 @@ def main():
         "fwd_adj_speedup_vs_fp32": (ms32 + adj32_ms) / (ms_step + adj_ms),
         "torch_separable_ms_incl_host_copies": pcie_ms,
         "roofline": {"bound": "mfma",
-                     "kernel": "forward call: " + CALL_KERNELS.get(
-                         args.precision, "wsplit_kernel + fwd_mixed_kernel + combine_kernel"),
-                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": traffic_bytes(args.precision),
-                     "algorithmic_flop_per_launch": flops, "launch_ms": launch_ms},
+                     "kernel": KERNEL_NAME.get(args.precision, "fwd_mixed_kernel"),
+                     "achieved": achieved_k, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved_k / peak, "traffic": traffic_bytes(args.precision),
+                     "algorithmic_flop_per_launch": flops, "kernel_ms": kernel_ms,
+                     "mfma_flop_issued_per_launch": issued,
+                     "mfma_issued_tflop_s": issued / (kernel_ms * 1e-3) / 1e12 if issued else None,
+                     "note": "achieved = algorithmic FLOP / the contraction kernel's mean duration "
+                             "(events right around it on its stream); mfma_flop_issued = MFMA "
+                             "instructions the kernel issues (windows skip some, split forms run "
+                             "three products) from the rocprofv3 counter profile; traffic = the "
+                             "forward call's HBM bytes from the same profile"},
+        "roofline_call": {"kernels": CALL_KERNELS.get(args.precision,
+                                                      "wsplit_kernel + fwd_mixed_kernel + combine_kernel"),
+                          "achieved": achieved_c, "peak": peak, "unit": "TFLOP/s", "frac": achieved_c / peak,
+                          "launch_ms": launch_ms},
     }
     if not args.no_robust:
         rs = max(5, args.steps // 10)
@@ -345,8 +420,10 @@ def main():
             "forward_adjoint": time_robust(args.precision, rs, objective=True)}
     if not args.no_cpu:
         line["cpu_baseline"], ref64 = cpu_leg(prob)
-        line["accuracy_vs_fp64"] = accuracy(ref64, D)
-        line["fp32"]["accuracy_vs_fp64"] = accuracy(ref64, D32)
+        for f in forms:
+            per_form[f]["accuracy_vs_fp64"] = accuracy(ref64, doses[f])
+        line["accuracy_vs_fp64"] = per_form[args.precision]["accuracy_vs_fp64"]
+        line["fp32"]["accuracy_vs_fp64"] = per_form["fp32"]["accuracy_vs_fp64"]
     print(json.dumps(line), flush=True)
 
 

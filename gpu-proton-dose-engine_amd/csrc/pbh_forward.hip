@@ -425,7 +425,8 @@ static int nchunks(const FwdArgs& a, int chunk) { return (a.L + chunk - 1) / chu
 static size_t ws_peaks(const FwdArgs& a) { return align256((size_t)a.L * sizeof(float)); }
 static size_t ws_wsplit(const FwdArgs& a)
 {
-    return align256((size_t)a.L * chunk_count(a.nsy) * chunk_count(a.nsx) * FT_WSPLIT * sizeof(unsigned short));
+    const size_t chunks = (size_t)a.L * chunk_count(a.nsy) * chunk_count(a.nsx);
+    return align256(chunks * FT_WSPLIT * sizeof(unsigned short)) + align256(chunks * sizeof(float2));
 }
 
 static size_t ws_layers(const FwdArgs& a)
@@ -462,12 +463,16 @@ int pbh_plan_forward(const FwdArgs& a, int kx_pad, int ky_pad, void* ws, size_t 
     }
     plan->part = nullptr;
     plan->wsplit = nullptr;
+    plan->wpar = nullptr;
     plan->layers = nullptr;
     plan->nlayers = nullptr;
     const size_t need = pbh_forward_ws_bytes(a, chunk);
     if (ws && ws_bytes >= need) {
         char* p = (char*)ws;
         plan->wsplit = (unsigned short*)(p + ws_peaks(a));
+        plan->wpar = (float2*)(p + ws_peaks(a) +
+                               align256((size_t)a.L * chunk_count(a.nsy) * chunk_count(a.nsx) * FT_WSPLIT *
+                                        sizeof(unsigned short)));
         char* lp = p + ws_peaks(a) + ws_wsplit(a);
         plan->nlayers = (int*)lp;
         plan->layers = (LayerEntry*)(lp + align256((size_t)a.nz * sizeof(int)));
@@ -494,7 +499,7 @@ int pbh_launch_combine_batch(const BatchFwdItem* items, const int* wg0, int n, i
 }
 
 int pbh_forward_mixed(const FwdArgs& a, double oz, double dz, int precision, int layer_chunk,
-                      void* ws, size_t ws_bytes, hipStream_t s);
+                      void* ws, size_t ws_bytes, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1);
 
 extern "C" size_t pbh_forward_workspace_bytes_tuned(const PbhGrid* g, const PbhLattice* lat, int precision,
                                                    int layer_chunk)
@@ -541,9 +546,11 @@ extern "C" size_t pbh_forward_workspace_bytes(const PbhGrid* g, const PbhLattice
     return pbh_forward_workspace_bytes_tuned(g, lat, precision, 0);
 }
 
-extern "C" int pbh_forward_tuned(const PbhGrid* g, const PbhLattice* lat, const float* w,
-                                 const double* idd, const double* sigma, float* dose, int precision,
-                                 int layer_chunk, void* workspace, size_t workspace_bytes, void* stream)
+// pbh_forward_tuned, recording ev0 / ev1 (when not null) on the stream right before
+// and right after the contraction kernel.
+static int forward_impl(const PbhGrid* g, const PbhLattice* lat, const float* w, const double* idd,
+                        const double* sigma, float* dose, int precision, int layer_chunk, void* workspace,
+                        size_t workspace_bytes, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
     int rc = validate_problem(g, lat);
     if (rc != PBH_OK) return rc;
@@ -551,9 +558,8 @@ extern "C" int pbh_forward_tuned(const PbhGrid* g, const PbhLattice* lat, const 
     if (precision < PBH_PREC_FP32 || precision > PBH_PREC_FP16X3_PEAK) return PBH_ERR_UNSUPPORTED;
     if (layer_chunk < 0) return PBH_ERR_BAD_DIM;
     const FwdArgs a = make_fwd_args(g, lat, w, idd, sigma, dose);
-    hipStream_t s = (hipStream_t)stream;
     if (precision != PBH_PREC_FP32)
-        return pbh_forward_mixed(a, g->oz, g->dz, precision, layer_chunk, workspace, workspace_bytes, s);
+        return pbh_forward_mixed(a, g->oz, g->dz, precision, layer_chunk, workspace, workspace_bytes, s, ev0, ev1);
     FwdPlan pl;
     rc = pbh_plan_forward(a, chunk_pad(a.nsx, a.nsy), chunk_pad(a.nsx, a.nsy), workspace, workspace_bytes,
                           layer_chunk, &pl);
@@ -562,6 +568,7 @@ extern "C" int pbh_forward_tuned(const PbhGrid* g, const PbhLattice* lat, const 
     pl.nlayers = nullptr;
     const dim3 grid((unsigned)(tile_count(a) * pl.nchunk));
     const bool q5 = fp32_q5(pl);
+    if (ev0 && (rc = hip_rc(hipEventRecord(ev0, s), "hipEventRecord")) != PBH_OK) return rc;
     switch (chunk_pad(a.nsx, a.nsy)) {
 #define PBH_FP32_CASE(N) \
     case N: \
@@ -576,7 +583,26 @@ extern "C" int pbh_forward_tuned(const PbhGrid* g, const PbhLattice* lat, const 
     default: return PBH_ERR_UNSUPPORTED;
     }
     if ((rc = launch_rc("fwd_fp32_kernel")) != PBH_OK) return rc;
+    if (ev1 && (rc = hip_rc(hipEventRecord(ev1, s), "hipEventRecord")) != PBH_OK) return rc;
     return pbh_launch_combine(a, pl, s);
+}
+
+extern "C" int pbh_forward_tuned(const PbhGrid* g, const PbhLattice* lat, const float* w,
+                                 const double* idd, const double* sigma, float* dose, int precision,
+                                 int layer_chunk, void* workspace, size_t workspace_bytes, void* stream)
+{
+    return forward_impl(g, lat, w, idd, sigma, dose, precision, layer_chunk, workspace, workspace_bytes,
+                        (hipStream_t)stream, nullptr, nullptr);
+}
+
+extern "C" int pbh_forward_profiled(const PbhGrid* g, const PbhLattice* lat, const float* w,
+                                    const double* idd, const double* sigma, float* dose, int precision,
+                                    int layer_chunk, void* workspace, size_t workspace_bytes,
+                                    void* ev_start, void* ev_end, void* stream)
+{
+    if (!ev_start || !ev_end) return PBH_ERR_NULL_ARG;
+    return forward_impl(g, lat, w, idd, sigma, dose, precision, layer_chunk, workspace, workspace_bytes,
+                        (hipStream_t)stream, (hipEvent_t)ev_start, (hipEvent_t)ev_end);
 }
 
 extern "C" int pbh_forward(const PbhGrid* g, const PbhLattice* lat, const float* w,

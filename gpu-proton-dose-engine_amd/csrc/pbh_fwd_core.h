@@ -71,6 +71,10 @@ struct FwdPlan {
     AxisTable tx, ty;
     float* part;        // [nchunk - 1][tiles][FT_TY * FT_TX] partial tiles (nchunk > 1)
     unsigned short* wsplit;   // mixed forms: [L][chunk y][chunk x][FT_WSPLIT] split weights
+    // FP16 forms: per (layer, chunk y, chunk x) {2^e, b}: the split weights are
+    // w 2^-e with max |w 2^-e| in [1, 2), and b bounds their column sums
+    // max_i sum_j |w_ji 2^-e|, so |c Gy W| <= c 2^e b for Gaussian factors <= 1
+    float2* wpar;
     // mixed forms: the active layers of every slice in layer order, [nz][L] (first
     // nlayers[z] of row z), written by the weight-split launch
     LayerEntry* layers;

@@ -23,10 +23,13 @@
 // geometries compute the slots at run time (QC = 0).
 //
 // BF16 shares FP32's exponent range: stage B accumulates every layer's products
-// straight into the running dose accumulators. FP16 has a narrow exponent range,
-// so U is scaled per row block by a power of two (largest magnitude into [1, 2));
-// each layer's contribution is then summed in fresh FP32 accumulators and joins
-// the running dose with the inverse scale applied.
+// straight into the running dose accumulators. FP16 has a narrow exponent range:
+// the weight-split launch scales the weights of every (layer, spot chunk) by a
+// power of two (largest magnitude into [1, 2)) and records a bound on their column
+// sums. Each workgroup takes the largest bound of its layers, 2^eacc, and keeps the
+// dose scaled by 2^-eacc: every layer's U is stored below 2^14 with no reduction
+// in the kernel, and its products accumulate straight into the accumulators, as
+// for BF16; the tile is scaled back by 2^eacc when it is stored.
 #include "pbh_mixed_core.h"
 
 namespace pbh {
@@ -66,20 +69,59 @@ constexpr int WS_PAIRS = SCH * KYP / 2;                  // 1536 (i, j pair) ent
 constexpr int WS_PARTS = WS_PAIRS / 256;                 // workgroups of 256 threads per chunk
 
 template <typename T>
-__device__ __forceinline__ void wsplit_body(const FwdArgs& a, unsigned short* dst, int blk, int part)
+__device__ __forceinline__ void wsplit_body(const FwdArgs& a, unsigned short* dst, float2* wpar, int blk, int part)
 {
+    constexpr bool F16 = !__is_same(T, __bf16);
     const int ncx = chunk_count(a.nsx), ncy = chunk_count(a.nsy);
     const int cx = blk % ncx, cy = (blk / ncx) % ncy, l = blk / (ncx * ncy);
     if (l >= a.L) return;
     const int i0 = chunk_begin(a.nsx, ncx, cx), ni = chunk_begin(a.nsx, ncx, cx + 1) - i0;
     const int j0 = chunk_begin(a.nsy, ncy, cy), nj = chunk_begin(a.nsy, ncy, cy + 1) - j0;
     const float* wl = a.w + (size_t)l * a.nsy * a.nsx;
+    float sc = 1.0f;
+    if (F16) {
+        // FP16's exponent range: the chunk's weights are scaled by 2^-e so that their
+        // largest magnitude lies in [1, 2); every part of the chunk forms the same
+        // maxima (same order), and part 0 records 2^e and the column-sum bound
+        // wave g sums rows j = g, g + 4, ... of column i = lane; the four partial
+        // column sums are added in wave order, then maximised over the columns
+        __shared__ float part_sum[FT_THREADS / 64][64];
+        __shared__ float part_max[FT_THREADS / 64];
+        const int col = threadIdx.x & 63, g = threadIdx.x >> 6;
+        float csum = 0.0f, mx = 0.0f;
+        if (col < ni) {
+            for (int j = g; j < nj; j += FT_THREADS / 64) {
+                const float v = fabsf(wl[(size_t)(j0 + j) * a.nsx + i0 + col]);
+                csum += v;
+                mx = fmaxf(mx, v);
+            }
+        }
+        part_sum[g][col] = csum;
+        for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+        if (col == 0) part_max[g] = mx;
+        __syncthreads();
+        float cmax = 0.0f, wmax = 0.0f;
+        if (g == 0) {
+            float t = 0.0f;
+#pragma unroll
+            for (int q = 0; q < FT_THREADS / 64; ++q) t += part_sum[q][col];
+            for (int o = 32; o > 0; o >>= 1) t = fmaxf(t, __shfl_xor(t, o, 64));
+            if (col == 0) part_sum[0][0] = t;
+        }
+        __syncthreads();
+        cmax = part_sum[0][0];
+#pragma unroll
+        for (int q = 0; q < FT_THREADS / 64; ++q) wmax = fmaxf(wmax, part_max[q]);
+        const int e = (wmax > 0.0f && wmax <= 3.0e38f) ? ilogbf(wmax) : 0;
+        sc = ldexpf(1.0f, -e);
+        if (part == 0 && threadIdx.x == 0) wpar[blk] = make_float2(ldexpf(1.0f, e), cmax * sc);
+    }
     unsigned* hi = (unsigned*)(dst + (size_t)blk * FT_WSPLIT);
     unsigned* lo = hi + WS_PAIRS;
     const int p = part * 256 + threadIdx.x;               // pair index: i = p / 32, j = 2 (p % 32) + {0, 1}
     const int i = p / (KYP / 2), j = 2 * (p % (KYP / 2));
-    const float v0 = (i < ni && j < nj) ? wl[(size_t)(j0 + j) * a.nsx + i0 + i] : 0.0f;
-    const float v1 = (i < ni && j + 1 < nj) ? wl[(size_t)(j0 + j + 1) * a.nsx + i0 + i] : 0.0f;
+    const float v0 = (i < ni && j < nj) ? wl[(size_t)(j0 + j) * a.nsx + i0 + i] * sc : 0.0f;
+    const float v1 = (i < ni && j + 1 < nj) ? wl[(size_t)(j0 + j + 1) * a.nsx + i0 + i] * sc : 0.0f;
     T h0, r0, h1, r1;
     split<T>(v0, h0, r0);
     split<T>(v1, h1, r1);
@@ -123,7 +165,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void wsplit_kernel(FwdArgs a, FwdPlan pl, double oz, double dz, float* r_peak)
 {
     if (blockIdx.y != WS_PARTS) {
-        wsplit_body<T>(a, pl.wsplit, blockIdx.x, blockIdx.y);
+        wsplit_body<T>(a, pl.wsplit, pl.wpar, blockIdx.x, blockIdx.y);
         return;
     }
     layer_lists_body(a, pl, blockIdx.x, gridDim.x);
@@ -139,7 +181,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void wsplit_batch_kernel(const BatchFwdItem* __restrict__ items)
 {
     const FwdArgs a = items[blockIdx.z].a;
-    wsplit_body<T>(a, items[blockIdx.z].pl.wsplit, blockIdx.x, blockIdx.y);
+    wsplit_body<T>(a, items[blockIdx.z].pl.wsplit, items[blockIdx.z].pl.wpar, blockIdx.x, blockIdx.y);
 }
 
 // 4 waves (two workgroups per CU), one-barrier software pipeline over the
@@ -169,8 +211,6 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
     constexpr int TYN = !TAB ? 1 : QC ? table_slots(2 * (FT_TY - 1) + QC * (KYP - 1) + 1) : FT_TAB;
     __shared__ unsigned Tx[TAB ? 2 : 1][TXN];
     __shared__ unsigned Ty[TAB ? 2 : 1][TYN];
-    __shared__ float uscale[2][2];
-    __shared__ float bmax[2][4];
     __shared__ LayerList ll;
     // contraction windows of the current spot chunk, per active layer: [0, 1] stage A of
     // row block 0 / 1, [4 + w] stage B of wave w; k0 | nstep << 6
@@ -207,6 +247,7 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
     const double inv_dsx = 1.0 / a.dsx, inv_dsy = 1.0 / a.dsy;
 
     v16f acc = {};
+    int eacc = -1000;     // FP16 forms: the accumulators hold dose 2^-eacc (-1000: none yet)
     u4v wreg[NV];
 
     const int lo = ch * pl.chunk, hi = lo + pl.chunk;   // this workgroup's active layers
@@ -220,6 +261,26 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
         const int m0 = max(0, lo - base), m1 = min(nact, hi - base);
         base += nact;
         if (m1 <= m0) continue;
+        if (F16) {
+            // the accumulators hold dose 2^-eacc: eacc covers the largest bound of this
+            // pass's layers on every spot chunk (the accumulated part is rescaled exactly
+            // when a later pass needs a larger one)
+            int e = -1000;
+            for (int m = m0; m < m1; ++m)
+                for (int q = 0; q < ncy * ncx; ++q) {
+                    const float2 wp = pl.wpar[(size_t)ll.lay[m] * ncy * ncx + q];
+                    const float bound = ll.c[m] * wp.x * wp.y;
+                    if (bound > 0.0f && bound <= 3.0e38f) e = max(e, ilogbf(bound) - 13);
+                }
+            if (e > eacc) {
+                if (eacc > -1000) {
+                    const float r = ldexpf(1.0f, eacc - e);
+#pragma unroll
+                    for (int r16 = 0; r16 < 16; ++r16) acc[r16] *= r;
+                }
+                eacc = e;
+            }
+        }
         for (int cy = 0; cy < ncy; ++cy) {
             const int j0 = chunk_begin(a.nsy, ncy, cy), nj = chunk_begin(a.nsy, ncy, cy + 1) - j0;
             const double Cy = (a.oy + (double)y0 * a.dy) - (a.sy0 + (double)j0 * a.dsy);
@@ -260,6 +321,13 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                 const unsigned short* wchunk = pl.wsplit + ((size_t)cy * ncx + cx) * FT_WSPLIT;
                 const size_t wlayer = (size_t)ncy * ncx * FT_WSPLIT;
 
+                // FP16 forms: U of layer m on this chunk is stored as c 2^e 2^-eacc U_s, below
+                // 2^14 by the weight-split bound c 2^e b of |c Gy W| (U_s = Gy W 2^-e)
+                auto u_factor = [&](int m) {
+                    const float2 wp = pl.wpar[((size_t)ll.lay[m] * ncy + cy) * ncx + cx];
+                    // eacc still unset: every bound of the pass is 0 (all weights 0)
+                    return eacc > -1000 ? ldexpf(ll.c[m] * wp.x, -eacc) : 0.0f;
+                };
                 auto is_x3 = [&](int m) {
                     if (MODE == MODE_X3) return true;
                     if (MODE == MODE_PLAIN) return false;
@@ -358,27 +426,9 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                             }
                         }
                     }
-                    float inv_scale = 1.0f;
-                    if (F16) {
-                        // row block rbA is shared by waves rbA and rbA + 2: combine maxima in LDS
-                        float mx = 0.0f;
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            mx = fmaxf(mx, fabsf(c * u0[r]));
-                            if (twoA) mx = fmaxf(mx, fabsf(c * u1[r]));
-                        }
-                        for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-                        if (lane == 0) bmax[b][wave] = mx;
-                        __syncthreads();
-                        mx = fmaxf(bmax[b][rbA], bmax[b][rbA + 2]);
-                        float scale = 1.0f;
-                        if (mx > 0.0f && mx < 3.0e38f) {
-                            const int ex = ilogbf(mx);            // 2^ex <= mx < 2^(ex+1)
-                            scale = ldexpf(1.0f, ex);
-                            inv_scale = ldexpf(1.0f, -ex);
-                        }
-                        if (lane == 0 && wave < 2) uscale[b][wave] = scale;
-                    }
+                    // FP16: U is stored as (c 2^e 2^-eU) U_s with the layer's power of two 2^eU,
+                    // from the weight-split bound, putting |U| below 2^14 (no reduction needed)
+                    const float cs = F16 ? u_factor(m) : c;
                     // C layout of U^T (16x16): spots 16 cb + 4 gA + r (r < 4), row yA:
                     // four consecutive spots of U's row yA, one 8-byte store per half
                     auto put = [&](v4f u, int cb) {
@@ -386,8 +436,8 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
 #pragma unroll
                         for (int q = 0; q < 2; ++q) {
                             T h0, l0, h1, l1;
-                            split<T>(c * u[2 * q] * inv_scale, h0, l0);
-                            split<T>(c * u[2 * q + 1] * inv_scale, h1, l1);
+                            split<T>(cs * u[2 * q], h0, l0);
+                            split<T>(cs * u[2 * q + 1], h1, l1);
                             hv[q] = (unsigned)bits16(h0) | ((unsigned)bits16(h1) << 16);
                             lv[q] = (unsigned)bits16(l0) | ((unsigned)bits16(l1) << 16);
                         }
@@ -429,10 +479,8 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                             }
                         }
                     }
-                    // without a per-layer scale the products accumulate straight into acc
-                    v16f p;
-                    if (F16) p = v16f{};
-                    else p = acc;
+                    // the products accumulate straight into acc (FP16: scaled by 2^-eacc)
+                    v16f p = acc;
 #pragma unroll
                     for (int ks = 0; ks < KB; ++ks) {
                         if (ks >= nkB) break;
@@ -444,14 +492,7 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                             p = mfma32(*(const V8*)&Ul[b][off], bh[ks], p);
                         }
                     }
-                    if (F16) {
-                        // rows 0-15 / 16-31 came from stage-A row blocks 0 / 1
-                        const float s0 = uscale[b][0], s1 = uscale[b][1];
-#pragma unroll
-                        for (int r = 0; r < 16; ++r) acc[r] += (r < 8 ? s0 : s1) * p[r];
-                    } else {
-                        acc = p;
-                    }
+                    acc = p;
                 };
 
                 load_w(m0);
@@ -483,6 +524,11 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
         }
     }
     if (ch > 0 && base <= lo) return;                   // no layers in this chunk: nothing to add
+    if (F16 && eacc > -1000) {
+        const float r = ldexpf(1.0f, eacc);
+#pragma unroll
+        for (int r16 = 0; r16 < 16; ++r16) acc[r16] *= r;
+    }
     store_block(a, pl, tile, ch, z, y0, x0, 0, cnB, acc, (size_t)tiles);
 }
 
@@ -615,7 +661,7 @@ using namespace pbh;
 float* pbh_forward_ws_peaks(const FwdArgs& a, void* ws);
 
 int pbh_forward_mixed(const FwdArgs& a, double oz, double dz, int precision, int layer_chunk, void* ws,
-                      size_t ws_bytes, hipStream_t s)
+                      size_t ws_bytes, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
     const int kb = (chunk_pad(a.nsx, 1) + 15) / 16;   // stage B K steps of 16 spots
     if (!ws) return PBH_ERR_NULL_ARG;
@@ -634,8 +680,10 @@ int pbh_forward_mixed(const FwdArgs& a, double oz, double dz, int precision, int
     else
         hipLaunchKernelGGL(wsplit_kernel<__bf16>, wgrid, dim3(256), 0, s, a, pl, oz, dz, r_peak);
     if ((rc = launch_rc("wsplit_kernel")) != PBH_OK) return rc;
+    if (ev0 && (rc = hip_rc(hipEventRecord(ev0, s), "hipEventRecord")) != PBH_OK) return rc;
     if ((rc = dispatch_precision(precision, mixed_key(pl, kb), SingleLaunch{a, pl, oz, dz, r_peak, s})) != PBH_OK)
         return rc;
+    if (ev1 && (rc = hip_rc(hipEventRecord(ev1, s), "hipEventRecord")) != PBH_OK) return rc;
     return pbh_launch_combine(a, pl, s);
 }
 

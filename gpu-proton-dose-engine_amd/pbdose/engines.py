@@ -115,11 +115,14 @@ class DeviceProblem:
         return self._ws[key]
 
     def forward(self, precision: str = "fp32", out: Optional[torch.Tensor] = None,
-                w: Optional[torch.Tensor] = None, layer_chunk: int = 0) -> torch.Tensor:
+                w: Optional[torch.Tensor] = None, layer_chunk: int = 0, kernel_events=None) -> torch.Tensor:
         """Dose (nz, ny, nx) float32, asynchronous on the current stream.
         ``layer_chunk``: at most this many active layers of a depth slice per
         workgroup (0 = chosen from this problem's active layers per slice by
-        pbh_forward_choose_chunk, >= L = one workgroup per dose tile)."""
+        pbh_forward_choose_chunk, >= L = one workgroup per dose tile).
+        ``kernel_events``: a pair of recorded torch.cuda.Event(enable_timing=True)
+        that pbh_forward_profiled re-records right before and after the fused
+        contraction kernel (for timing that kernel alone)."""
         p = _lib.PREC[precision]
         if out is None:
             out = torch.empty(self.shape, dtype=torch.float32, device=self.dev)
@@ -131,6 +134,14 @@ class DeviceProblem:
         if int(layer_chunk) == 0:
             layer_chunk = self.auto_chunk(p)
         ws = self.workspace("fwd", p, int(layer_chunk))
+        if kernel_events is not None:
+            e0, e1 = kernel_events
+            _lib.check(_lib.lib().pbh_forward_profiled(
+                self.grid_s, self.lat_s, _lib.ptr(wt), _lib.ptr(self.idd), _lib.ptr(self.sigma),
+                _lib.ptr(out), p, int(layer_chunk), _lib.ptr(ws), ws.numel(),
+                _lib.C.c_void_p(e0.cuda_event), _lib.C.c_void_p(e1.cuda_event),
+                _lib.stream_of(self.dev)), "pbh_forward_profiled")
+            return out
         _lib.check(_lib.lib().pbh_forward_tuned(self.grid_s, self.lat_s, _lib.ptr(wt),
                                                 _lib.ptr(self.idd), _lib.ptr(self.sigma),
                                                 _lib.ptr(out), p, int(layer_chunk), _lib.ptr(ws),

@@ -208,6 +208,18 @@ int pbh_forward_tuned(const PbhGrid* g, const PbhLattice* lat,
                       float* dose, int precision, int layer_chunk,
                       void* workspace, size_t workspace_bytes, void* stream);
 
+/* pbh_forward_tuned that also records two caller-created HIP events
+ * (hipEvent_t, passed as void*) on `stream`: ev_start right before the fused
+ * contraction kernel and ev_end right after it, so that the caller can time
+ * that kernel alone (the weight split before it and the layer-chunk combine
+ * after it fall outside). Results equal pbh_forward_tuned's. No counterpart in
+ * the reference (engines.py:208-283 times whole calls, benchmark.py:42-69). */
+int pbh_forward_profiled(const PbhGrid* g, const PbhLattice* lat,
+                         const float* w, const double* idd, const double* sigma,
+                         float* dose, int precision, int layer_chunk,
+                         void* workspace, size_t workspace_bytes,
+                         void* ev_start, void* ev_end, void* stream);
+
 /* Layer chunk for pbh_forward_tuned from the number of active layers (IDD != 0)
  * of every depth slice, active_per_z[nz] (host memory): simulates the dispatch
  * of the forward's workgroups onto the resident slots (compute_units CUs, <= 0:

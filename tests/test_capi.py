@@ -61,6 +61,15 @@ def test_forward_rejects_null_and_bad_precision_without_launching():
     assert L.pbh_forward(g, lat, nul, nul, nul, nul, 0, nul, 0, nul) == -1
     one = C.c_void_p(16)   # never dereferenced: the precision check fails first
     assert L.pbh_forward(g, lat, one, one, one, one, 99, nul, 0, nul) == -5
+    # the profiled forward needs both events; the no-workspace forward checks its
+    # arguments and its scratch limit before touching the device
+    assert L.pbh_forward_profiled(g, lat, one, one, one, one, 3, 0, nul, 0, nul, one, nul) == -1
+    assert L.pbh_forward_noalloc(g, lat, nul, nul, nul, nul, 3, 0, nul) == -1
+    assert L.pbh_forward_noalloc(g, lat, one, one, one, one, 99, 0, nul) == -5
+    big = _lib.PbhGrid(512, 512, 512, 0, 0.2, 0.2, 0.2, 0, 0, 0)
+    blat = _lib.PbhLattice(41, 41, 40, 0, 0.5, 0.5, 0.0, 0.0)
+    assert L.pbh_forward_workspace_bytes_tuned(big, blat, 3, 1) > _lib.PBH_NOALLOC_SCRATCH_BYTES
+    assert L.pbh_forward_noalloc(big, blat, one, one, one, one, 3, 1, nul) == -8
 
 
 def test_forward_workspace_query():

@@ -64,10 +64,13 @@ def test_config_small_64cube():
     check(p)
 
 
-def test_config_clinical_128cube_sampled_slices():
-    print("case 128^3 / 41x41 / 40 layers (sampled slices)", flush=True)
+def test_config_clinical_128cube_full_volume():
+    """The clinical workload (128^3 @ 2 mm, 41 x 41 @ 5 mm, 40 layers 70-180 MeV,
+    lognormal(0, 0.5) weights from seed 0): every voxel of the FP32 dose against
+    the FP64 oracle."""
+    print("case 128^3 / 41x41 / 40 layers (full volume vs the FP64 oracle)", flush=True)
     p = make_problem(128, 128, 128, 0.2, 41, 41, 0.5, np.linspace(70, 180, 40))
-    check(p, z_idx=np.arange(0, 128, 9))
+    check(p)
 
 
 @pytest.mark.parametrize("shape", [(37, 70, 5), (130, 65, 3), (1, 1, 2), (200, 33, 4)])
@@ -155,6 +158,50 @@ def test_mixed_precision_forms(precision):
     print(f"case mixed {precision} on 64^3 / 21x21 / 10 layers", flush=True)
     p = make_problem(64, 64, 64, 0.2, 21, 21, 0.5, np.linspace(100, 150, 10), seed=4)
     check(p, tol=MIXED_TOL[precision], precision=precision)
+
+
+@pytest.mark.parametrize("precision", ["fp16", "fp16x3", "fp16x3_peak"])
+@pytest.mark.parametrize("scale_exp", [-26, -12, 17, 30])
+def test_fp16_forms_at_any_weight_scale(precision, scale_exp):
+    """FP16 has a narrow exponent range: weights of 2^17 (proton counts) would
+    overflow it and weights of 2^-12 would lose the low half to subnormals. The
+    weight split scales every (layer, spot chunk) by a power of two and the layer's
+    U by its bound, so the relative error is that of O(1) weights."""
+    print(f"case {precision} with weights x 2^{scale_exp}", flush=True)
+    p = make_problem(64, 64, 24, 0.2, 21, 21, 0.5, np.linspace(100, 150, 6), seed=6)
+    p.lattice.weights = p.lattice.weights * 2.0 ** scale_exp
+    check(p, tol=MIXED_TOL[precision], precision=precision)
+
+
+@pytest.mark.parametrize("precision", ["bf16x3_peak", "fp16x3_peak"])
+def test_peak_forms_split_near_each_bragg_peak(precision):
+    """The "_peak" forms: the weight-split launch writes each layer's Bragg-peak
+    depth (np.argmax of its IDD on the grid) into the workspace, and one-layer
+    fields are within the split bound on the slices within 1 cm of that depth and
+    within the single-product bound elsewhere."""
+    from pbdose import _lib, engines
+    print(f"case peak depths {precision}", flush=True)
+    for E in (95.0, 140.0):
+        p = make_problem(64, 64, 96, 0.2, 15, 15, 0.5, [E], seed=8)
+        dp = engines.DeviceProblem(p)
+        D = dp.forward(precision).double().cpu().numpy()
+        chunk = dp.auto_chunk(_lib.PREC[precision])
+        ws = dp.workspace("fwd", _lib.PREC[precision], chunk)
+        r_peak = ws[:4].view(torch.float32).cpu().numpy()
+        z_pk = p.grid.oz + np.argmax(p.idd[0]) * p.grid.dz
+        assert abs(float(r_peak[0]) - z_pk) <= 1e-5
+        ref = oracle_dose(p)
+        # slices at exactly 1 cm are left out: the kernel compares in FP32
+        near = np.abs(p.grid.z - z_pk) <= 0.99
+        far = np.abs(p.grid.z - z_pk) >= 1.01
+        assert near.any() and far.any()
+        scale = np.abs(ref).max()
+        e_near = float(np.abs(D[near] - ref[near]).max() / scale)
+        e_far = float(np.abs(D[far] - ref[far]).max() / scale)
+        split_tol = MIXED_TOL[precision.replace("_peak", "")]
+        print(f"  E {E}: peak depth {float(r_peak[0]):.2f} cm; error within 1 cm {e_near:.2e} "
+              f"(bar {split_tol:.0e}), elsewhere {e_far:.2e}", flush=True)
+        assert e_near <= split_tol and e_far <= MIXED_TOL[precision]
 
 
 @pytest.mark.parametrize("precision", ["bf16x3", "fp16x3"])
