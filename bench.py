@@ -336,11 +336,16 @@ def main():
     if args.precision not in forms:
         raise SystemExit(f"unknown precision {args.precision}")
     per_form, doses = {}, {}
+    from pbdose import _lib
     for f in forms:
+        # measured tile height and layer chunk for this field (cached; outside the timing)
+        dp.autotune(f)
         steps = args.steps if f in (args.precision, "fp32") else max(20, args.steps // 4)
         D, ms, call_ms, kern_ms = time_forward(dp, f, steps, args.warmup)
         doses[f] = D
+        chunk, rows = dp._tuned[_lib.PREC[f]]
         per_form[f] = {"ms_per_step": ms, "fields_per_s": 1e3 / ms, "call_us": call_ms * 1e3,
+                       "tile": f"{rows} x 128 voxels, layer chunk {chunk} (autotuned)",
                        "kernel_us": kern_ms * 1e3, "steps": steps,
                        "kernel_achieved_tflop_s": flops / (kern_ms * 1e-3) / 1e12,
                        "kernel_frac_of_peak": flops / (kern_ms * 1e-3) / 1e12 / PEAK[base_type(f)],

@@ -52,7 +52,8 @@ constexpr int RED_ELEMS = 64;           // dJ/dw elements per depth-reduction wo
 
 // Host-side builders and launchers (pbh_forward.hip, pbh_mixed.hip, pbh_adjoint.hip).
 pbh::FwdArgs make_fwd_args(const PbhGrid* g, const PbhLattice* lat, const float* w,
-                           const double* idd, const double* sigma, float* dose);
+                           const double* idd, const double* sigma, float* dose,
+                           int tile_rows = pbh::FT_TY);
 int pbh_make_adj_args(const PbhGrid* g, const PbhLattice* lat, const float* w, const double* idd,
                       const double* sigma, const float* R, float* part, double* grad_idd,
                       double* grad_sigma, int split, pbh::AdjArgs* out);

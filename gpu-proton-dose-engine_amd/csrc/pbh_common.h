@@ -78,6 +78,7 @@ struct FwdArgs {
     const double* sigma;             // (L, nz)
     float* dose;                     // (nz, ny, nx)
     int tiles_x, tiles_y;
+    int tile_rows;                   // dose tile rows (y): 32, or 64 for the mixed forms
 };
 
 }  // namespace pbh

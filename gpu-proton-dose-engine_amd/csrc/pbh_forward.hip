@@ -303,7 +303,7 @@ extern "C" int pbh_validate(const PbhGrid* g, const PbhLattice* lat)
 }
 
 FwdArgs make_fwd_args(const PbhGrid* g, const PbhLattice* lat, const float* w,
-                      const double* idd, const double* sigma, float* dose)
+                      const double* idd, const double* sigma, float* dose, int tile_rows)
 {
     FwdArgs a;
     a.nx = g->nx; a.ny = g->ny; a.nz = g->nz;
@@ -314,7 +314,8 @@ FwdArgs make_fwd_args(const PbhGrid* g, const PbhLattice* lat, const float* w,
     a.sy0 = lat->y0 - (lat->n_sy - 1) / 2.0 * lat->dy_spot;
     a.w = w; a.idd = idd; a.sigma = sigma; a.dose = dose;
     a.tiles_x = cdiv(g->nx, FT_TX);
-    a.tiles_y = cdiv(g->ny, FT_TY);
+    a.tile_rows = tile_rows;
+    a.tiles_y = cdiv(g->ny, tile_rows);
     return a;
 }
 
@@ -385,7 +386,32 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 // of two chunks per tile.
 constexpr double FT_WG_COST = 2.0;
 
-static int model_chunk(const FwdArgs& a, const int32_t* active, long long slots)
+// Makespan of the simulated dispatch (see model_chunk) for one chunk: workgroups cost
+// `layer_cost` per layer plus `fixed` each. Returns the makespan in layers.
+static double model_makespan(const FwdArgs& a, const int32_t* active, long long slots, int c,
+                             double layer_cost, double fixed, std::vector<double>& heap)
+{
+    const long long per_z = (long long)a.tiles_x * a.tiles_y;
+    const int nch = (a.L + c - 1) / c;
+    heap.assign((size_t)slots, 0.0);                              // min-heap of slot finish times
+    for (int ch = 0; ch < nch; ++ch) {
+        for (int z = 0; z < a.nz; ++z) {
+            const int act = active[z] < 0 ? 0 : (active[z] > a.L ? a.L : active[z]);
+            const int n = std::min(c, std::max(0, act - ch * c));
+            const double w = (ch > 0 && n == 0) ? 0.05 : n * layer_cost + fixed;
+            for (long long t = 0; t < per_z; ++t) {
+                std::pop_heap(heap.begin(), heap.end(), std::greater<double>());
+                heap.back() += w;
+                std::push_heap(heap.begin(), heap.end(), std::greater<double>());
+            }
+        }
+    }
+    return *std::max_element(heap.begin(), heap.end());
+}
+
+// Best chunk and its makespan for one tile height.
+static int model_chunk_cost(const FwdArgs& a, const int32_t* active, long long slots, double layer_cost,
+                            double fixed, double* best_span)
 {
     const long long per_z = (long long)a.tiles_x * a.tiles_y;
     if (slots < 1) slots = 1;
@@ -396,26 +422,19 @@ static int model_chunk(const FwdArgs& a, const int32_t* active, long long slots)
     for (int c = a.L; c >= std::min(a.L, FT_MIN_CHUNK); --c) {
         const int nch = (a.L + c - 1) / c;
         if ((long long)nch * per_z * a.nz > (1LL << 24)) break;     // keep the model cheap
-        heap.assign((size_t)slots, 0.0);                              // min-heap of slot finish times
-        for (int ch = 0; ch < nch; ++ch) {
-            for (int z = 0; z < a.nz; ++z) {
-                const int act = active[z] < 0 ? 0 : (active[z] > a.L ? a.L : active[z]);
-                const int n = std::min(c, std::max(0, act - ch * c));
-                const double w = (ch > 0 && n == 0) ? 0.05 : n + FT_WG_COST;
-                for (long long t = 0; t < per_z; ++t) {
-                    std::pop_heap(heap.begin(), heap.end(), std::greater<double>());
-                    heap.back() += w;
-                    std::push_heap(heap.begin(), heap.end(), std::greater<double>());
-                }
-            }
-        }
-        const double span = *std::max_element(heap.begin(), heap.end());
+        const double span = model_makespan(a, active, slots, c, layer_cost, fixed, heap);
         if (c == a.L || span < best_t * 0.98) {
             best = c;
             best_t = span;
         }
     }
+    if (best_span) *best_span = best_t;
     return best;
+}
+
+static int model_chunk(const FwdArgs& a, const int32_t* active, long long slots)
+{
+    return model_chunk_cost(a, active, slots, 1.0, FT_WG_COST, nullptr);
 }
 
 static int nchunks(const FwdArgs& a, int chunk) { return (a.L + chunk - 1) / chunk; }
@@ -438,7 +457,7 @@ size_t pbh_forward_ws_bytes(const FwdArgs& a, int layer_chunk)
 {
     const int chunk = layer_chunk > 0 ? layer_chunk : pbh_auto_chunk(a);
     size_t n = ws_peaks(a) + ws_wsplit(a) + ws_layers(a);
-    n += (size_t)(nchunks(a, chunk) - 1) * tile_count(a) * FT_TY * FT_TX * sizeof(float);
+    n += (size_t)(nchunks(a, chunk) - 1) * tile_count(a) * a.tile_rows * FT_TX * sizeof(float);
     return n;
 }
 
@@ -456,7 +475,7 @@ int pbh_plan_forward(const FwdArgs& a, int kx_pad, int ky_pad, void* ws, size_t 
     plan->nchunk = nchunks(a, chunk);
     if (tile_count(a) * plan->nchunk > 0x7fffffffLL) return PBH_ERR_UNSUPPORTED;
     plan->table = axis_table(a.dx, a.dsx, FT_TX, kx_pad, &plan->tx) &&
-                  axis_table(a.dy, a.dsy, FT_TY, ky_pad, &plan->ty);
+                  axis_table(a.dy, a.dsy, a.tile_rows, ky_pad, &plan->ty);
     if (!plan->table) {
         plan->tx = AxisTable{0, 0, 0.0, 0};
         plan->ty = AxisTable{0, 0, 0.0, 0};
@@ -506,7 +525,7 @@ extern "C" size_t pbh_forward_workspace_bytes_tuned(const PbhGrid* g, const PbhL
 {
     if (validate_problem(g, lat) != PBH_OK || layer_chunk < 0) return 0;
     (void)precision;
-    const FwdArgs a = make_fwd_args(g, lat, nullptr, nullptr, nullptr, nullptr);
+    const FwdArgs a = make_fwd_args(g, lat, nullptr, nullptr, nullptr, nullptr, FT_TY);
     return pbh_forward_ws_bytes(a, layer_chunk > 0 && layer_chunk < a.L ? layer_chunk : (layer_chunk ? a.L : 0));
 }
 
@@ -524,7 +543,7 @@ extern "C" int pbh_forward_choose_chunk(const PbhGrid* g, const PbhLattice* lat,
     if (rc != PBH_OK) return rc;
     if (!active_per_z) return PBH_ERR_NULL_ARG;
     if (precision < PBH_PREC_FP32 || precision > PBH_PREC_FP16X3_PEAK) return PBH_ERR_UNSUPPORTED;
-    const FwdArgs a = make_fwd_args(g, lat, nullptr, nullptr, nullptr, nullptr);
+    const FwdArgs a = make_fwd_args(g, lat, nullptr, nullptr, nullptr, nullptr, FT_TY);
     // workgroup slots per CU: 3 for the mixed kernels on the P = 2, Q = 5 table
     // geometry, 2 for the others. (The single-product kernel holds 4, but those four
     // share the CU's issue bandwidth; a sweep measured its best chunk where the
@@ -541,23 +560,74 @@ extern "C" int pbh_forward_choose_chunk(const PbhGrid* g, const PbhLattice* lat,
     return model_chunk(a, active_per_z, (long long)per_cu * cus);
 }
 
+// Tile height and layer chunk from the active layers per slice: the dispatch model of
+// each tile height, in units of one 32-row tile-layer. A 64-row workgroup (16-bit
+// forms on the P = 2, Q = 5 geometry) costs about as much per layer as a 32-row one
+// although it covers twice the rows (operand gathers, weight copies and table builds
+// shared by two row blocks; 1.05 per layer and 2.25 fixed, fitted to a sweep of
+// chunks 8-40 at both heights on the 128^3 / 41 x 41 x 40-layer field), but only
+// two fit per CU for the split forms (LDS) and three for the single-product forms.
+constexpr double FT64_LAYER_COST = 1.05, FT64_WG_COST = 2.25;
+
+extern "C" int pbh_forward_choose_shape(const PbhGrid* g, const PbhLattice* lat, int precision,
+                                        const int32_t* active_per_z, int compute_units, int32_t* tile_rows)
+{
+    int rc = validate_problem(g, lat);
+    if (rc != PBH_OK) return rc;
+    if (!active_per_z || !tile_rows) return PBH_ERR_NULL_ARG;
+    if (precision < PBH_PREC_FP32 || precision > PBH_PREC_FP16X3_PEAK) return PBH_ERR_UNSUPPORTED;
+    const int cus = compute_units > 0 ? compute_units : pbh_device_cus();
+    const int c32 = pbh_forward_choose_chunk(g, lat, precision, active_per_z, cus);
+    *tile_rows = FT_TY;
+    if (c32 < 0 || precision == PBH_PREC_FP32) return c32;
+    const FwdArgs a32 = make_fwd_args(g, lat, nullptr, nullptr, nullptr, nullptr, FT_TY);
+    FwdPlan pl;
+    if (pbh_plan_forward(a32, chunk_pad(a32.nsx, a32.nsy), chunk_pad(a32.nsx, a32.nsy), nullptr, 0, a32.L, &pl) !=
+            PBH_OK ||
+        !(pl.table && pl.tx.P == 2 && pl.tx.Q == 5 && pl.ty.P == 2 && pl.ty.Q == 5))
+        return c32;
+    double t32 = 0.0, t64 = 0.0;
+    model_chunk_cost(a32, active_per_z, 3LL * cus, 1.0, FT_WG_COST, &t32);
+    const FwdArgs a64 = make_fwd_args(g, lat, nullptr, nullptr, nullptr, nullptr, 64);
+    const bool plain = precision == PBH_PREC_BF16 || precision == PBH_PREC_FP16;
+    const int c64 = model_chunk_cost(a64, active_per_z, (plain ? 3LL : 2LL) * cus, FT64_LAYER_COST,
+                                     FT64_WG_COST, &t64);
+    if (t64 < t32) {
+        *tile_rows = 64;
+        return c64;
+    }
+    return c32;
+}
+
 extern "C" size_t pbh_forward_workspace_bytes(const PbhGrid* g, const PbhLattice* lat, int precision)
 {
     return pbh_forward_workspace_bytes_tuned(g, lat, precision, 0);
 }
 
-// pbh_forward_tuned, recording ev0 / ev1 (when not null) on the stream right before
-// and right after the contraction kernel.
+// Dose tile rows of a call: 0 = the default (32); 32 for every form, 64 for the mixed
+// forms on the P = 2, Q = 5 table geometry (checked when planned).
+static int tile_rows_of(int precision, int tile_rows)
+{
+    if (tile_rows == 0) return FT_TY;
+    if (tile_rows != 32 && tile_rows != 64) return PBH_ERR_BAD_DIM;
+    if (tile_rows == 64 && precision == PBH_PREC_FP32) return PBH_ERR_UNSUPPORTED;
+    return tile_rows;
+}
+
+// pbh_forward_tuned with a tile height, recording ev0 / ev1 (when not null) on the
+// stream right before and right after the contraction kernel.
 static int forward_impl(const PbhGrid* g, const PbhLattice* lat, const float* w, const double* idd,
-                        const double* sigma, float* dose, int precision, int layer_chunk, void* workspace,
-                        size_t workspace_bytes, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
+                        const double* sigma, float* dose, int precision, int layer_chunk, int tile_rows,
+                        void* workspace, size_t workspace_bytes, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
     int rc = validate_problem(g, lat);
     if (rc != PBH_OK) return rc;
     if (!w || !idd || !sigma || !dose) return PBH_ERR_NULL_ARG;
     if (precision < PBH_PREC_FP32 || precision > PBH_PREC_FP16X3_PEAK) return PBH_ERR_UNSUPPORTED;
     if (layer_chunk < 0) return PBH_ERR_BAD_DIM;
-    const FwdArgs a = make_fwd_args(g, lat, w, idd, sigma, dose);
+    const int rows = tile_rows_of(precision, tile_rows);
+    if (rows < 0) return rows;
+    const FwdArgs a = make_fwd_args(g, lat, w, idd, sigma, dose, rows);
     if (precision != PBH_PREC_FP32)
         return pbh_forward_mixed(a, g->oz, g->dz, precision, layer_chunk, workspace, workspace_bytes, s, ev0, ev1);
     FwdPlan pl;
@@ -591,18 +661,37 @@ extern "C" int pbh_forward_tuned(const PbhGrid* g, const PbhLattice* lat, const 
                                  const double* idd, const double* sigma, float* dose, int precision,
                                  int layer_chunk, void* workspace, size_t workspace_bytes, void* stream)
 {
-    return forward_impl(g, lat, w, idd, sigma, dose, precision, layer_chunk, workspace, workspace_bytes,
+    return forward_impl(g, lat, w, idd, sigma, dose, precision, layer_chunk, 0, workspace, workspace_bytes,
                         (hipStream_t)stream, nullptr, nullptr);
+}
+
+extern "C" int pbh_forward_shaped(const PbhGrid* g, const PbhLattice* lat, const float* w,
+                                  const double* idd, const double* sigma, float* dose, int precision,
+                                  int layer_chunk, int tile_rows, void* workspace, size_t workspace_bytes,
+                                  void* stream)
+{
+    return forward_impl(g, lat, w, idd, sigma, dose, precision, layer_chunk, tile_rows, workspace,
+                        workspace_bytes, (hipStream_t)stream, nullptr, nullptr);
+}
+
+extern "C" size_t pbh_forward_workspace_bytes_shaped(const PbhGrid* g, const PbhLattice* lat, int precision,
+                                                     int layer_chunk, int tile_rows)
+{
+    if (validate_problem(g, lat) != PBH_OK || layer_chunk < 0) return 0;
+    const int rows = tile_rows_of(precision, tile_rows);
+    if (rows < 0) return 0;
+    const FwdArgs a = make_fwd_args(g, lat, nullptr, nullptr, nullptr, nullptr, rows);
+    return pbh_forward_ws_bytes(a, layer_chunk > 0 && layer_chunk < a.L ? layer_chunk : (layer_chunk ? a.L : 0));
 }
 
 extern "C" int pbh_forward_profiled(const PbhGrid* g, const PbhLattice* lat, const float* w,
                                     const double* idd, const double* sigma, float* dose, int precision,
-                                    int layer_chunk, void* workspace, size_t workspace_bytes,
+                                    int layer_chunk, int tile_rows, void* workspace, size_t workspace_bytes,
                                     void* ev_start, void* ev_end, void* stream)
 {
     if (!ev_start || !ev_end) return PBH_ERR_NULL_ARG;
-    return forward_impl(g, lat, w, idd, sigma, dose, precision, layer_chunk, workspace, workspace_bytes,
-                        (hipStream_t)stream, (hipEvent_t)ev_start, (hipEvent_t)ev_end);
+    return forward_impl(g, lat, w, idd, sigma, dose, precision, layer_chunk, tile_rows, workspace,
+                        workspace_bytes, (hipStream_t)stream, (hipEvent_t)ev_start, (hipEvent_t)ev_end);
 }
 
 extern "C" int pbh_forward(const PbhGrid* g, const PbhLattice* lat, const float* w,

@@ -7,7 +7,7 @@
 
 namespace pbh {
 
-constexpr int FT_TY = 32;        // dose tile rows (y) per workgroup
+constexpr int FT_TY = 32;        // dose tile rows (y) per workgroup (FP32; the mixed forms also 64)
 constexpr int FT_TX = 128;       // dose tile columns (x) per workgroup
 constexpr int FT_THREADS = 256;  // 4 waves; two workgroups share a CU
 constexpr int FT_TAB = 1024;     // entries per axis table
@@ -69,7 +69,7 @@ struct FwdPlan {
     int nchunk;         // workgroups per dose tile: ceil(L / chunk)
     int table;          // 1: factors gathered from per-(layer, depth) tables
     AxisTable tx, ty;
-    float* part;        // [nchunk - 1][tiles][FT_TY * FT_TX] partial tiles (nchunk > 1)
+    float* part;        // [nchunk - 1][tiles][tile_rows * FT_TX] partial tiles (nchunk > 1)
     unsigned short* wsplit;   // mixed forms: [L][chunk y][chunk x][FT_WSPLIT] split weights
     // FP16 forms: per (layer, chunk y, chunk x) {2^e, b}: the split weights are
     // w 2^-e with max |w 2^-e| in [1, 2), and b bounds their column sums
@@ -156,7 +156,7 @@ __device__ inline void store_block(const FwdArgs& a, const FwdPlan& pl, int tile
     const int lane = threadIdx.x & 63;
     const int xl = cn + (lane & 31);
     if (c > 0) {
-        float* mine = pl.part + ((size_t)(c - 1) * tiles + tile) * (FT_TY * FT_TX);
+        float* mine = pl.part + ((size_t)(c - 1) * tiles + tile) * ((size_t)a.tile_rows * FT_TX);
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int row = rm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
@@ -200,13 +200,14 @@ __device__ inline void combine_body(const FwdArgs& a, const FwdPlan& pl, int til
     const int nact = pl.nlayers ? pl.nlayers[z] : count_active(a, z, cnt);
     const int nch = nact > pl.chunk ? (nact + pl.chunk - 1) / pl.chunk : 1;
     if (nch <= 1) return;
-    const float* part0 = pl.part + (size_t)tile * (FT_TY * FT_TX);
-    const size_t cstride = tiles * (FT_TY * FT_TX);
+    const int TR = a.tile_rows;
+    const float* part0 = pl.part + (size_t)tile * ((size_t)TR * FT_TX);
+    const size_t cstride = tiles * ((size_t)TR * FT_TX);
     const bool vec = (a.nx & 3) == 0 && tx * FT_TX + FT_TX <= a.nx && ((uintptr_t)a.dose & 15) == 0;
     if (vec) {
         // whole 128-column rows inside the grid: four consecutive voxels per thread
-        for (int e = 4 * threadIdx.x; e < FT_TY * FT_TX; e += 4 * blockDim.x) {
-            const int y = ty * FT_TY + e / FT_TX, x = tx * FT_TX + e % FT_TX;
+        for (int e = 4 * threadIdx.x; e < TR * FT_TX; e += 4 * blockDim.x) {
+            const int y = ty * TR + e / FT_TX, x = tx * FT_TX + e % FT_TX;
             if (y >= a.ny) break;
             float4* d = (float4*)(a.dose + ((size_t)z * a.ny + y) * a.nx + x);
             float4 v = *d;
@@ -221,8 +222,8 @@ __device__ inline void combine_body(const FwdArgs& a, const FwdPlan& pl, int til
         }
         return;
     }
-    for (int e = threadIdx.x; e < FT_TY * FT_TX; e += blockDim.x) {
-        const int y = ty * FT_TY + e / FT_TX, x = tx * FT_TX + e % FT_TX;
+    for (int e = threadIdx.x; e < TR * FT_TX; e += blockDim.x) {
+        const int y = ty * TR + e / FT_TX, x = tx * FT_TX + e % FT_TX;
         if (y >= a.ny || x >= a.nx) continue;
         float* d = a.dose + ((size_t)z * a.ny + y) * a.nx + x;
         float v = *d;

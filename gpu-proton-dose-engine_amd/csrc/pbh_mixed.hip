@@ -184,14 +184,18 @@ __global__ __launch_bounds__(256) void wsplit_batch_kernel(const BatchFwdItem* _
     wsplit_body<T>(a, items[blockIdx.z].pl.wsplit, items[blockIdx.z].pl.wpar, blockIdx.x, blockIdx.y);
 }
 
-// 4 waves (two workgroups per CU), one-barrier software pipeline over the
-// active layers of each spot chunk, as in the FP32 kernel. Stage A: 16x16 blocks
-// of U^T (3 spot blocks x 2 row blocks); wave w takes spot block w / 2 of row
-// block w % 2 and, for w < 2, spot block 2 of the same row block. Stage B: wave
-// w owns the 32 x 32 dose block at columns 32 w; KB K steps of 16 spots.
+// 4 waves, one-barrier software pipeline over the active layers of each spot
+// chunk, as in the FP32 kernel, on a TY x 128 dose tile (TY = 32 or 64).
+// Stage A: 16x16 blocks of U^T (3 spot blocks x TY / 16 row blocks).
+//   TY = 32: wave w takes spot block w / 2 of row block w % 2 and, for w < 2,
+//            spot block 2 of the same row block;
+//   TY = 64: wave w takes row block w and all three spot blocks, so each
+//            gathered Gy operand feeds three products.
+// Stage B: wave w owns the TY x 32 dose columns 32 w, as TY / 32 blocks of 32 x 32
+// that share each gathered Gx operand; KB K steps of 16 spots.
 //
 // blk: the workgroup's index in the problem's own grid (chunk * tiles + tile).
-template <typename T, int MODE, bool TAB, int KB, int QC, bool LISTS>
+template <typename T, int MODE, bool TAB, int KB, int QC, bool LISTS, int TY>
 __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& pl, double oz, double dz,
                                                const float* r_peak, int blk)
 {
@@ -201,19 +205,22 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
     constexpr int KX = 16 * KB;                 // stage B K (spots in x), padded
     constexpr int NWV = SPLIT ? 2 * WV : WV;    // 16-byte weight vectors per chunk and layer
     constexpr int NV = (NWV + FT_THREADS - 1) / FT_THREADS;
+    static_assert(TY == 32 || TY == 64, "tile rows");
+    constexpr int NCB = TY == 64 ? 3 : 2;       // stage A spot blocks per wave (TY = 32: the second on w < 2)
+    constexpr int NBB = TY / 32;                // stage B row blocks of 32 per wave
 
     __shared__ __attribute__((aligned(16))) T Wh[2][SCH * WT_STRIDE];      // W^T hi [i][j]
     __shared__ __attribute__((aligned(16))) T Wl[2][SPLIT ? SCH * WT_STRIDE : 8];
-    __shared__ __attribute__((aligned(16))) T Uh[2][FT_TY * U_STRIDE];     // c U hi [y][i]
-    __shared__ __attribute__((aligned(16))) T Ul[2][SPLIT ? FT_TY * U_STRIDE : 8];
+    __shared__ __attribute__((aligned(16))) T Uh[2][TY * U_STRIDE];        // c U hi [y][i]
+    __shared__ __attribute__((aligned(16))) T Ul[2][SPLIT ? TY * U_STRIDE : 8];
     // table slots: the interleaved span on the P = 2, Q = QC geometry, FT_TAB otherwise
     constexpr int TXN = !TAB ? 1 : QC ? table_slots(2 * (FT_TX - 1) + QC * (KX - 1) + 1) : FT_TAB;
-    constexpr int TYN = !TAB ? 1 : QC ? table_slots(2 * (FT_TY - 1) + QC * (KYP - 1) + 1) : FT_TAB;
+    constexpr int TYN = !TAB ? 1 : QC ? table_slots(2 * (TY - 1) + QC * (KYP - 1) + 1) : FT_TAB;
     __shared__ unsigned Tx[TAB ? 2 : 1][TXN];
     __shared__ unsigned Ty[TAB ? 2 : 1][TYN];
     __shared__ LayerList ll;
-    // contraction windows of the current spot chunk, per active layer: [0, 1] stage A of
-    // row block 0 / 1, [4 + w] stage B of wave w; k0 | nstep << 6
+    // contraction windows of the current spot chunk, per active layer: [rb] stage A of
+    // row block rb (< TY / 16), [4 + w] stage B of wave w; k0 | nstep << 6
     __shared__ unsigned char win[FT_LB][8];
 
     const int lane = threadIdx.x & 63;
@@ -224,29 +231,29 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
     const int tx = tile % a.tiles_x;
     const int ty = (tile / a.tiles_x) % a.tiles_y;
     const int z = tile / (a.tiles_x * a.tiles_y);
-    const int x0 = tx * FT_TX, y0 = ty * FT_TY;
+    const int x0 = tx * FT_TX, y0 = ty * TY;
     const float zc = (float)(oz + (double)z * dz);
 
-    const int rbA = wave & 1;                       // stage A row block (y)
+    const int rbA = TY == 64 ? wave : (wave & 1);   // stage A row block (y)
     const int yA = 16 * rbA + (lane & 15);          // stage A row of the tile: B-operand column, output row
     const int gA = lane >> 4;                       // stage A k group (8 values)
-    const int cbA0 = wave >> 1;                     // stage A spot blocks: cbA0 and, w < 2, 2
-    const bool twoA = wave < 2;
+    const int cbA0 = TY == 64 ? 0 : wave >> 1;      // TY = 32: stage A spot blocks cbA0 and, w < 2, 2
+    const bool twoA = TY == 64 || wave < 2;
     const int cnB = 32 * wave;                      // stage B columns
     const int aB = cnB + (lane & 31);
     const int gB = lane >> 5;                       // stage B k group (8 values)
     const double yAc = a.oy + (double)(y0 + yA) * a.dy;
     const double xBc = a.ox + (double)(x0 + aB) * a.dx;
     TabStream<QC> sA, sB;
-    sA.init(pl.ty, 8 * pl.ty.Q * gA - pl.ty.P * yA + pl.ty.P * (FT_TY - 1));
+    sA.init(pl.ty, 8 * pl.ty.Q * gA - pl.ty.P * yA + pl.ty.P * (TY - 1));
     sB.init(pl.tx, 8 * pl.tx.Q * gB - pl.tx.P * aB + pl.tx.P * (FT_TX - 1));
-    const int ntY = pl.ty.P * (FT_TY - 1) + pl.ty.Q * (KYP - 1) + 1;
+    const int ntY = pl.ty.P * (TY - 1) + pl.ty.Q * (KYP - 1) + 1;
     const int ntX = pl.tx.P * (FT_TX - 1) + pl.tx.Q * (KX - 1) + 1;
 
     const int ncx = chunk_count(a.nsx), ncy = chunk_count(a.nsy);
     const double inv_dsx = 1.0 / a.dsx, inv_dsy = 1.0 / a.dsy;
 
-    v16f acc = {};
+    v16f acc[NBB] = {};
     int eacc = -1000;     // FP16 forms: the accumulators hold dose 2^-eacc (-1000: none yet)
     u4v wreg[NV];
 
@@ -276,7 +283,9 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                 if (eacc > -1000) {
                     const float r = ldexpf(1.0f, eacc - e);
 #pragma unroll
-                    for (int r16 = 0; r16 < 16; ++r16) acc[r16] *= r;
+                    for (int rb = 0; rb < NBB; ++rb)
+#pragma unroll
+                        for (int r16 = 0; r16 < 16; ++r16) acc[rb][r16] *= r;
                 }
                 eacc = e;
             }
@@ -288,10 +297,10 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                 const int i0 = chunk_begin(a.nsx, ncx, cx), ni = chunk_begin(a.nsx, ncx, cx + 1) - i0;
                 const double Cx = (a.ox + (double)x0 * a.dx) - (a.sx0 + (double)i0 * a.dsx);
                 const TableRow rowx = TAB ? table_row(pl.tx, Cx, FT_TX) : TableRow{0, 0.0f, 0.0f};
-                const TableRow rowy = TAB ? table_row(pl.ty, Cy, FT_TY) : TableRow{0, 0.0f, 0.0f};
+                const TableRow rowy = TAB ? table_row(pl.ty, Cy, TY) : TableRow{0, 0.0f, 0.0f};
                 // QC > 0: this thread's table entries, squared distances formed once per chunk
                 TabShare<QC ? 2 * (FT_TX - 1) + QC * (KX - 1) + 1 : 1> shx;
-                TabShare<QC ? 2 * (FT_TY - 1) + QC * (KYP - 1) + 1 : 1> shy;
+                TabShare<QC ? 2 * (TY - 1) + QC * (KYP - 1) + 1 : 1> shy;
                 if (TAB && QC) {
                     shx.init(pl.tx, rowx);
                     shy.init(pl.ty, rowy);
@@ -300,9 +309,9 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                 // (stage B, K = x spots), in spot units of the chunk; read after the next barrier
                 for (int q = threadIdx.x; q < (m1 - m0) * 8; q += FT_THREADS) {
                     const int m = m0 + (q >> 3), w = q & 7;
-                    if (w == 2 || w == 3) continue;
+                    if (w < 4 && w >= TY / 16) continue;
                     SpotWindow sw;
-                    if (w < 2) {
+                    if (w < 4) {
                         const float lo = (float)((Cy + (double)(16 * w) * a.dy) * inv_dsy);
                         sw = spot_window(lo, lo + (float)(15.0 * a.dy * inv_dsy),
                                               ll.cut[m] * (float)inv_dsy, nj, 32, KYP);
@@ -385,7 +394,11 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                     const int nkA = window(m, rbA, k0);
                     TabStream<QC> sw = sA;
                     sw.shift(pl.ty, k0);
-                    v4f u0 = {}, u1 = {};
+                    // spot blocks of this wave: TY = 32: cbA0 and (w < 2) 2; TY = 64: 0, 1, 2
+                    auto cb_of = [&](int k) { return TY == 64 ? k : (k == 0 ? cbA0 : 2); };
+                    v4f u[NCB];
+#pragma unroll
+                    for (int k = 0; k < NCB; ++k) u[k] = v4f{};
 #pragma unroll
                     for (int ks = 0; ks < 2; ++ks) {
                         if (ks < nkA) {
@@ -408,20 +421,15 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                                 }
                             }
                             const int kofs = k0 + 32 * ks + 8 * gA;
-                            const int off0 = (16 * cbA0 + (lane & 15)) * WT_STRIDE + kofs;
-                            const int off1 = (32 + (lane & 15)) * WT_STRIDE + kofs;
-                            const V8 w0h = *(const V8*)&Wh[b][off0];
-                            u0 = mfma16(w0h, gh, u0);
-                            if (SPLIT && x3) {
-                                u0 = mfma16(w0h, gl, u0);
-                                u0 = mfma16(*(const V8*)&Wl[b][off0], gh, u0);
-                            }
-                            if (twoA) {
-                                const V8 w1h = *(const V8*)&Wh[b][off1];
-                                u1 = mfma16(w1h, gh, u1);
+#pragma unroll
+                            for (int k = 0; k < NCB; ++k) {
+                                if (k > 0 && !twoA) continue;
+                                const int off = (16 * cb_of(k) + (lane & 15)) * WT_STRIDE + kofs;
+                                const V8 wh = *(const V8*)&Wh[b][off];
+                                u[k] = mfma16(wh, gh, u[k]);
                                 if (SPLIT && x3) {
-                                    u1 = mfma16(w1h, gl, u1);
-                                    u1 = mfma16(*(const V8*)&Wl[b][off1], gh, u1);
+                                    u[k] = mfma16(wh, gl, u[k]);
+                                    u[k] = mfma16(*(const V8*)&Wl[b][off], gh, u[k]);
                                 }
                             }
                         }
@@ -445,8 +453,9 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                         *(u2v*)&Uh[b][off] = hv;
                         if (SPLIT) *(u2v*)&Ul[b][off] = lv;
                     };
-                    put(u0, cbA0);
-                    if (twoA) put(u1, 2);
+#pragma unroll
+                    for (int k = 0; k < NCB; ++k)
+                        if (k == 0 || twoA) put(u[k], cb_of(k));
                 };
                 // stage B of layer m: acc += (c U) Gx^T
                 auto stage_b = [&](int m) {
@@ -479,20 +488,22 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                             }
                         }
                     }
-                    // the products accumulate straight into acc (FP16: scaled by 2^-eacc)
-                    v16f p = acc;
+                    // the products accumulate straight into acc (FP16: scaled by 2^-eacc); each
+                    // gathered Gx operand feeds the wave's TY / 32 row blocks
 #pragma unroll
                     for (int ks = 0; ks < KB; ++ks) {
                         if (ks >= nkB) break;
-                        const int off = (lane & 31) * U_STRIDE + k0 + 16 * ks + 8 * gB;
-                        const V8 ah = *(const V8*)&Uh[b][off];
-                        p = mfma32(ah, bh[ks], p);
-                        if (SPLIT && x3) {
-                            p = mfma32(ah, bl[ks], p);
-                            p = mfma32(*(const V8*)&Ul[b][off], bh[ks], p);
+#pragma unroll
+                        for (int rb = 0; rb < NBB; ++rb) {
+                            const int off = (32 * rb + (lane & 31)) * U_STRIDE + k0 + 16 * ks + 8 * gB;
+                            const V8 ah = *(const V8*)&Uh[b][off];
+                            acc[rb] = mfma32(ah, bh[ks], acc[rb]);
+                            if (SPLIT && x3) {
+                                acc[rb] = mfma32(ah, bl[ks], acc[rb]);
+                                acc[rb] = mfma32(*(const V8*)&Ul[b][off], bh[ks], acc[rb]);
+                            }
                         }
                     }
-                    acc = p;
                 };
 
                 load_w(m0);
@@ -524,26 +535,31 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
         }
     }
     if (ch > 0 && base <= lo) return;                   // no layers in this chunk: nothing to add
-    if (F16 && eacc > -1000) {
-        const float r = ldexpf(1.0f, eacc);
 #pragma unroll
-        for (int r16 = 0; r16 < 16; ++r16) acc[r16] *= r;
+    for (int rb = 0; rb < NBB; ++rb) {
+        if (F16 && eacc > -1000) {
+            const float r = ldexpf(1.0f, eacc);
+#pragma unroll
+            for (int r16 = 0; r16 < 16; ++r16) acc[rb][r16] *= r;
+        }
+        store_block(a, pl, tile, ch, z, y0, x0, 32 * rb, cnB, acc[rb], (size_t)tiles);
     }
-    store_block(a, pl, tile, ch, z, y0, x0, 0, cnB, acc, (size_t)tiles);
 }
 
 // Workgroups per CU: on the P = 2, Q = 5 geometry (tables of their interleaved span)
 // three for the split forms (LDS near 52 KB) and four for the single-product forms (31 KB,
 // VGPRs held to 128); two otherwise.
-template <int MODE, int QC> struct MixedOcc {
-    static constexpr int value = !QC ? 2 : (MODE == MODE_PLAIN ? 4 : 3);
+// 64-row tiles (QC = 5): two workgroups per CU for the split forms (LDS near 66 KB),
+// three for the single-product forms.
+template <int MODE, int QC, int TY> struct MixedOcc {
+    static constexpr int value = !QC ? 2 : TY == 64 ? (MODE == MODE_PLAIN ? 3 : 2) : (MODE == MODE_PLAIN ? 4 : 3);
 };
 
-template <typename T, int MODE, bool TAB, int KB, int QC>
-__global__ __launch_bounds__(FT_THREADS, (MixedOcc<MODE, QC>::value))
+template <typename T, int MODE, bool TAB, int KB, int QC, int TY>
+__global__ __launch_bounds__(FT_THREADS, (MixedOcc<MODE, QC, TY>::value))
 void fwd_mixed_kernel(FwdArgs a, FwdPlan pl, double oz, double dz, const float* r_peak)
 {
-    fwd_mixed_body<T, MODE, TAB, KB, QC, true>(a, pl, oz, dz, r_peak, blockIdx.x);
+    fwd_mixed_body<T, MODE, TAB, KB, QC, true, TY>(a, pl, oz, dz, r_peak, blockIdx.x);
 }
 
 // One launch over a batch of problems sharing the template parameters. Its
@@ -551,14 +567,14 @@ void fwd_mixed_kernel(FwdArgs a, FwdPlan pl, double oz, double dz, const float* 
 // lookup and the copy path the kernel needed more than the 168 VGPRs of 3 workgroups
 // per CU and spilled).
 template <typename T, int MODE, bool TAB, int KB, int QC>
-__global__ __launch_bounds__(FT_THREADS, (MixedOcc<MODE, QC>::value))
+__global__ __launch_bounds__(FT_THREADS, (MixedOcc<MODE, QC, 32>::value))
 void fwd_mixed_batch_kernel(const BatchFwdItem* __restrict__ items, const int* __restrict__ wg0, int n)
 {
     const int k = batch_item(wg0, n, blockIdx.x);
     const FwdArgs a = items[k].a;
     const FwdPlan pl = items[k].pl;
-    fwd_mixed_body<T, MODE, TAB, KB, QC, false>(a, pl, items[k].oz, items[k].dz, items[k].r_peak,
-                                         (int)blockIdx.x - wg0[k]);
+    fwd_mixed_body<T, MODE, TAB, KB, QC, false, 32>(a, pl, items[k].oz, items[k].dz, items[k].r_peak,
+                                                 (int)blockIdx.x - wg0[k]);
 }
 
 // r_peak of every layer of every item: one workgroup per item, one wave per layer
@@ -635,7 +651,13 @@ struct SingleLaunch {
     int go() const
     {
         const dim3 grid((unsigned)((long long)a.tiles_x * a.tiles_y * a.nz * pl.nchunk)), block(FT_THREADS);
-        hipLaunchKernelGGL((fwd_mixed_kernel<T, MODE, TAB, KB, QC>), grid, block, 0, s, a, pl, oz, dz, rp);
+        if (a.tile_rows == 64) {
+            // 64-row tiles on the P = 2, Q = 5 table geometry only
+            if (!(TAB && QC == 5)) return PBH_ERR_UNSUPPORTED;
+            hipLaunchKernelGGL((fwd_mixed_kernel<T, MODE, TAB, KB, QC, 64>), grid, block, 0, s, a, pl, oz, dz, rp);
+        } else {
+            hipLaunchKernelGGL((fwd_mixed_kernel<T, MODE, TAB, KB, QC, 32>), grid, block, 0, s, a, pl, oz, dz, rp);
+        }
         return launch_rc("fwd_mixed_kernel");
     }
 };
@@ -669,6 +691,7 @@ int pbh_forward_mixed(const FwdArgs& a, double oz, double dz, int precision, int
     FwdPlan pl;
     int rc = pbh_plan_forward(a, 16 * kb, KYP, ws, ws_bytes, layer_chunk, &pl);
     if (rc != PBH_OK) return rc;
+    if (a.tile_rows == 64 && !mixed_key(pl, kb).q5) return PBH_ERR_UNSUPPORTED;   // before any launch
     float* r_peak = nullptr;
     if (precision == PBH_PREC_BF16X3_PEAK || precision == PBH_PREC_FP16X3_PEAK) {
         r_peak = pbh_forward_ws_peaks(a, ws);   // written by the weight-split launch

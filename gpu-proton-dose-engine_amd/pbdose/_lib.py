@@ -76,8 +76,11 @@ SIGNATURES = {
     "pbh_forward": (C.c_int, [_G, _L, _P, _P, _P, _P, C.c_int, _P, C.c_size_t, _P]),
     "pbh_forward_tuned": (C.c_int, [_G, _L, _P, _P, _P, _P, C.c_int, C.c_int, _P, C.c_size_t, _P]),
     "pbh_forward_choose_chunk": (C.c_int, [_G, _L, C.c_int, _P, C.c_int]),
-    "pbh_forward_profiled": (C.c_int, [_G, _L, _P, _P, _P, _P, C.c_int, C.c_int, _P, C.c_size_t, _P, _P,
-                                       _P]),
+    "pbh_forward_profiled": (C.c_int, [_G, _L, _P, _P, _P, _P, C.c_int, C.c_int, C.c_int, _P, C.c_size_t,
+                                       _P, _P, _P]),
+    "pbh_forward_choose_shape": (C.c_int, [_G, _L, C.c_int, _P, C.c_int, _P]),
+    "pbh_forward_shaped": (C.c_int, [_G, _L, _P, _P, _P, _P, C.c_int, C.c_int, C.c_int, _P, C.c_size_t, _P]),
+    "pbh_forward_workspace_bytes_shaped": (C.c_size_t, [_G, _L, C.c_int, C.c_int, C.c_int]),
     "pbh_forward_noalloc": (C.c_int, [_G, _L, _P, _P, _P, _P, C.c_int, C.c_int, _P]),
     "pbh_adjoint_workspace_bytes": (C.c_size_t, [_G, _L, C.c_int]),
     "pbh_adjoint": (C.c_int, [_G, _L, _P, _P, _P, _P, _P, _P, _P, C.c_int, _P, C.c_size_t, _P]),

@@ -59,6 +59,7 @@ class DeviceProblem:
             act = (self.idd != 0).sum(dim=0).cpu().numpy()
         self._active = np.ascontiguousarray(act, dtype=np.int32)
         self._chunk = {}
+        self._tuned = {}      # form -> (layer chunk, tile rows) measured by autotune()
 
     def auto_chunk(self, precision: int) -> int:
         """Layer chunk of the automatic forward for this problem's active layers
@@ -80,6 +81,85 @@ class DeviceProblem:
             self._chunk[precision] = int(c)
         return self._chunk[precision]
 
+    def auto_shape(self, precision: int) -> tuple:
+        """(layer chunk, tile rows) of the automatic forward for this problem's active
+        layers (pbh_forward_choose_shape: 32- or 64-row tiles; cached like auto_chunk)."""
+        key = ("shape", precision)
+        if key not in self._chunk:
+            g, lat = self.problem.grid, self.problem.lattice
+            ckey = ("shape", g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, g.ox, g.oy, g.oz, lat.n_sx, lat.n_sy,
+                    lat.n_layers, lat.dx_spot, lat.dy_spot, lat.x0, lat.y0, self.dev.index,
+                    precision, self._active.tobytes())
+            c = _CHUNK_CACHE.get(ckey)
+            if c is None:
+                rows = _lib.C.c_int32(0)
+                ch = _lib.lib().pbh_forward_choose_shape(self.grid_s, self.lat_s, precision,
+                                                         self._active.ctypes.data, 0, _lib.C.byref(rows))
+                _lib.check(min(ch, 0), "pbh_forward_choose_shape")
+                if len(_CHUNK_CACHE) > 4096:
+                    _CHUNK_CACHE.clear()
+                c = (int(ch), int(rows.value))
+                _CHUNK_CACHE[ckey] = c
+            self._chunk[key] = c
+        return self._chunk[key]
+
+    def shape_candidates(self, precision: int) -> list:
+        """(layer chunk, tile rows) pairs worth timing: for each tile height the
+        model's chunk and the chunks that cut the busiest slice's active layers into
+        1 to 6 equal parts (where the count of partial tiles changes)."""
+        g, lat = self.problem.grid, self.problem.lattice
+        heights = [32]
+        # 64-row tiles: 16-bit forms with 2 : 5 voxel : spot pitches on both axes
+        if (precision != _lib.PREC["fp32"] and abs(lat.dx_spot / g.dx - 2.5) < 1e-9
+                and abs(lat.dy_spot / g.dy - 2.5) < 1e-9):
+            heights.append(64)
+        amax = max(1, int(self._active.max()) if self._active.size else 1)
+        c_model, r_model = self.auto_shape(precision)
+        cands = set()
+        for rows in heights:
+            cands |= {(max(1, -(-amax // k)), rows) for k in range(1, 7)}
+            cands.add((self.auto_chunk(precision), 32))
+        cands.add((c_model, r_model))
+        return sorted(cands, key=lambda t: (t[1], -t[0]))
+
+    def autotune(self, precision: str, reps: int = 20, use_cache: bool = True) -> tuple:
+        """Measured (layer chunk, tile rows) of the forward for this problem: every
+        pair of shape_candidates is timed over `reps` back-to-back calls, and the
+        fastest is used by later forward() calls with layer_chunk = tile_rows = 0.
+        Winners are cached per (problem shape, active layers, form, device). All
+        candidates compute the same dose up to FP32 summation order."""
+        from . import tuning
+        p = _lib.PREC[precision]
+        g, lat = self.problem.grid, self.problem.lattice
+        key = "fwd|" + "|".join(str(v) for v in (
+            g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, lat.n_sx, lat.n_sy, lat.n_layers, lat.dx_spot,
+            lat.dy_spot, precision, torch.cuda.get_device_name(self.dev), self._active.tobytes().hex()))
+        cache = tuning.load_cache() if use_cache else {}
+        if key in cache:
+            best = tuple(cache[key]["shape"])
+        else:
+            out = torch.empty(self.shape, dtype=torch.float32, device=self.dev)
+            results = []
+            for c, rows in self.shape_candidates(p):
+                self.forward(precision, out=out, layer_chunk=c, tile_rows=rows)
+                torch.cuda.synchronize(self.dev)
+                s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                s.record()
+                for _ in range(reps):
+                    self.forward(precision, out=out, layer_chunk=c, tile_rows=rows)
+                e.record()
+                torch.cuda.synchronize(self.dev)
+                results.append((s.elapsed_time(e) / reps, (c, rows)))
+            best_ms, best = min(results, key=lambda r: r[0])
+            self.tuning_results = results
+            if use_cache:
+                cache = tuning.load_cache()
+                cache[key] = {"shape": list(best), "ms": best_ms,
+                              "timings": [[ms, list(cr)] for ms, cr in results]}
+                tuning.save_cache(cache)
+        self._tuned[p] = (int(best[0]), int(best[1]))
+        return self._tuned[p]
+
     def auto_split(self, precision: int) -> int:
         """Layer split of the automatic adjoint for this problem's active layers
         (pbh_adjoint_choose_split; cached like auto_chunk)."""
@@ -99,12 +179,13 @@ class DeviceProblem:
             self._chunk[key] = int(c)
         return self._chunk[key]
 
-    def workspace(self, kind: str, precision: int, layer_chunk: int = 0) -> torch.Tensor:
-        key = (kind, precision, layer_chunk)
+    def workspace(self, kind: str, precision: int, layer_chunk: int = 0, tile_rows: int = 0) -> torch.Tensor:
+        key = (kind, precision, layer_chunk, tile_rows)
         if key not in self._ws:
             L = _lib.lib()
             if kind == "fwd":
-                n = L.pbh_forward_workspace_bytes_tuned(self.grid_s, self.lat_s, precision, layer_chunk)
+                n = L.pbh_forward_workspace_bytes_shaped(self.grid_s, self.lat_s, precision, layer_chunk,
+                                                         tile_rows)
             elif kind == "obj":
                 n = L.pbh_objective_workspace_bytes(self.grid_s)
             elif kind == "objadj":
@@ -115,14 +196,18 @@ class DeviceProblem:
         return self._ws[key]
 
     def forward(self, precision: str = "fp32", out: Optional[torch.Tensor] = None,
-                w: Optional[torch.Tensor] = None, layer_chunk: int = 0, kernel_events=None) -> torch.Tensor:
+                w: Optional[torch.Tensor] = None, layer_chunk: int = 0, kernel_events=None,
+                tile_rows: int = 0) -> torch.Tensor:
         """Dose (nz, ny, nx) float32, asynchronous on the current stream.
         ``layer_chunk``: at most this many active layers of a depth slice per
-        workgroup (0 = chosen from this problem's active layers per slice by
-        pbh_forward_choose_chunk, >= L = one workgroup per dose tile).
+        workgroup (>= L = one workgroup per dose tile). With layer_chunk and tile_rows
+        both 0, both come from this problem's active layers per slice
+        (pbh_forward_choose_shape); a chunk of 0 alone is pbh_forward_choose_chunk's.
         ``kernel_events``: a pair of recorded torch.cuda.Event(enable_timing=True)
         that pbh_forward_profiled re-records right before and after the fused
-        contraction kernel (for timing that kernel alone)."""
+        contraction kernel (for timing that kernel alone).
+        ``tile_rows``: dose-tile height, 32 or 64 (16-bit forms on the 2 : 5 pitch
+        geometry); 0 = the default."""
         p = _lib.PREC[precision]
         if out is None:
             out = torch.empty(self.shape, dtype=torch.float32, device=self.dev)
@@ -131,22 +216,23 @@ class DeviceProblem:
         wt = self.w if w is None else w
         if wt.dtype != torch.float32 or tuple(wt.shape) != tuple(self.w.shape) or not wt.is_contiguous():
             raise ValueError("weights must be a contiguous float32 (L, nsy, nsx) tensor")
-        if int(layer_chunk) == 0:
+        if int(layer_chunk) == 0 and int(tile_rows) == 0:
+            layer_chunk, tile_rows = self._tuned.get(p) or self.auto_shape(p)
+        elif int(layer_chunk) == 0:
             layer_chunk = self.auto_chunk(p)
-        ws = self.workspace("fwd", p, int(layer_chunk))
+        ws = self.workspace("fwd", p, int(layer_chunk), int(tile_rows))
         if kernel_events is not None:
             e0, e1 = kernel_events
             _lib.check(_lib.lib().pbh_forward_profiled(
                 self.grid_s, self.lat_s, _lib.ptr(wt), _lib.ptr(self.idd), _lib.ptr(self.sigma),
-                _lib.ptr(out), p, int(layer_chunk), _lib.ptr(ws), ws.numel(),
+                _lib.ptr(out), p, int(layer_chunk), int(tile_rows), _lib.ptr(ws), ws.numel(),
                 _lib.C.c_void_p(e0.cuda_event), _lib.C.c_void_p(e1.cuda_event),
                 _lib.stream_of(self.dev)), "pbh_forward_profiled")
             return out
-        _lib.check(_lib.lib().pbh_forward_tuned(self.grid_s, self.lat_s, _lib.ptr(wt),
-                                                _lib.ptr(self.idd), _lib.ptr(self.sigma),
-                                                _lib.ptr(out), p, int(layer_chunk), _lib.ptr(ws),
-                                                ws.numel(), _lib.stream_of(self.dev)),
-                   "pbh_forward_tuned")
+        _lib.check(_lib.lib().pbh_forward_shaped(
+            self.grid_s, self.lat_s, _lib.ptr(wt), _lib.ptr(self.idd), _lib.ptr(self.sigma),
+            _lib.ptr(out), p, int(layer_chunk), int(tile_rows), _lib.ptr(ws), ws.numel(),
+            _lib.stream_of(self.dev)), "pbh_forward_shaped")
         return out
 
 

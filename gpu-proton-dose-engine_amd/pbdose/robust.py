@@ -118,32 +118,7 @@ CANDIDATES = [{"fwd_chunk": f, "adj_split": a, "group": g, "order": o}
               for f in (0, 4096) for a in (2, 4, 8) for g in (0, 1) for o in (0, 1)]
 
 
-def _cache_path() -> str:
-    p = os.environ.get("PBDOSE_AUTOTUNE_CACHE")
-    if p:
-        return p
-    base = os.environ.get("XDG_CACHE_HOME") or os.path.join(os.path.expanduser("~"), ".cache")
-    return os.path.join(base, "pbdose", "autotune.json")
-
-
-def _load_cache() -> dict:
-    try:
-        with open(_cache_path()) as f:
-            return json.load(f)
-    except (OSError, ValueError):
-        return {}
-
-
-def _save_cache(c: dict) -> None:
-    path = _cache_path()
-    try:
-        os.makedirs(os.path.dirname(path), exist_ok=True)
-        tmp = path + f".{os.getpid()}.tmp"
-        with open(tmp, "w") as f:
-            json.dump(c, f, indent=1, sort_keys=True)
-        os.replace(tmp, path)
-    except OSError:
-        pass
+from .tuning import cache_path as _cache_path, load_cache as _load_cache, save_cache as _save_cache  # noqa: E402
 
 
 def tuning_struct(t: dict) -> _lib.PbhBatchTuning:

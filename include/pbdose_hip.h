@@ -208,15 +208,31 @@ int pbh_forward_tuned(const PbhGrid* g, const PbhLattice* lat,
                       float* dose, int precision, int layer_chunk,
                       void* workspace, size_t workspace_bytes, void* stream);
 
-/* pbh_forward_tuned that also records two caller-created HIP events
+/* pbh_forward_tuned with a dose-tile height: tile_rows = 32, or 64 for the 16-bit
+ * forms on grids whose voxel and spot pitches are 2 : 5 on both axes (other
+ * geometries and the FP32 form return PBH_ERR_UNSUPPORTED); 0 = the default
+ * (32). A 64 x 128 tile shares every gathered Gaussian operand between two
+ * row blocks of its MFMA products and copies the weights and builds the x table
+ * once for twice the rows, at fewer workgroups per CU. Results differ from
+ * 32-row tiles only by FP32 summation order. The workspace is
+ * pbh_forward_workspace_bytes_shaped with the same layer_chunk and tile_rows.
+ * No counterpart in the reference (its tiles are cuBLAS's). */
+int pbh_forward_shaped(const PbhGrid* g, const PbhLattice* lat,
+                       const float* w, const double* idd, const double* sigma,
+                       float* dose, int precision, int layer_chunk, int tile_rows,
+                       void* workspace, size_t workspace_bytes, void* stream);
+size_t pbh_forward_workspace_bytes_shaped(const PbhGrid* g, const PbhLattice* lat, int precision,
+                                          int layer_chunk, int tile_rows);
+
+/* pbh_forward_shaped that also records two caller-created HIP events
  * (hipEvent_t, passed as void*) on `stream`: ev_start right before the fused
  * contraction kernel and ev_end right after it, so that the caller can time
  * that kernel alone (the weight split before it and the layer-chunk combine
- * after it fall outside). Results equal pbh_forward_tuned's. No counterpart in
+ * after it fall outside). Results equal pbh_forward_shaped's. No counterpart in
  * the reference (engines.py:208-283 times whole calls, benchmark.py:42-69). */
 int pbh_forward_profiled(const PbhGrid* g, const PbhLattice* lat,
                          const float* w, const double* idd, const double* sigma,
-                         float* dose, int precision, int layer_chunk,
+                         float* dose, int precision, int layer_chunk, int tile_rows,
                          void* workspace, size_t workspace_bytes,
                          void* ev_start, void* ev_end, void* stream);
 
@@ -245,6 +261,14 @@ int pbh_forward_choose_chunk(const PbhGrid* g, const PbhLattice* lat, int precis
 int pbh_forward_noalloc(const PbhGrid* g, const PbhLattice* lat,
                         const float* w, const double* idd, const double* sigma,
                         float* dose, int precision, int layer_chunk, void* stream);
+
+/* Tile height and layer chunk for pbh_forward_shaped from the active layers per
+ * slice (as pbh_forward_choose_chunk): simulates the dispatch of 32- and 64-row
+ * tiles (the latter for the 16-bit forms on the 2 : 5 pitch geometry) and returns
+ * the chunk of the faster one, with its height in *tile_rows, or a negative
+ * PBH_ERR_* code. Host only; no counterpart in the reference. */
+int pbh_forward_choose_shape(const PbhGrid* g, const PbhLattice* lat, int precision,
+                             const int32_t* active_per_z, int compute_units, int32_t* tile_rows);
 
 /* ---------------------------------------------------------------------------
  * Adjoint of the forward map (inverse planning).

@@ -185,8 +185,8 @@ def test_peak_forms_split_near_each_bragg_peak(precision):
         p = make_problem(64, 64, 96, 0.2, 15, 15, 0.5, [E], seed=8)
         dp = engines.DeviceProblem(p)
         D = dp.forward(precision).double().cpu().numpy()
-        chunk = dp.auto_chunk(_lib.PREC[precision])
-        ws = dp.workspace("fwd", _lib.PREC[precision], chunk)
+        chunk, rows = dp.auto_shape(_lib.PREC[precision])
+        ws = dp.workspace("fwd", _lib.PREC[precision], chunk, rows)
         r_peak = ws[:4].view(torch.float32).cpu().numpy()
         z_pk = p.grid.oz + np.argmax(p.idd[0]) * p.grid.dz
         assert abs(float(r_peak[0]) - z_pk) <= 1e-5
@@ -209,6 +209,47 @@ def test_mixed_ragged_and_chunked(precision):
     print(f"case mixed {precision} ragged 70x37x5, lattice 60x50", flush=True)
     p = make_problem(70, 37, 5, 0.25, 60, 50, 0.3, [90.0, 150.0], seed=9, x0=0.2)
     check(p, tol=MIXED_TOL[precision], precision=precision)
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp16", "bf16x3", "fp16x3", "bf16x3_peak", "fp16x3_peak"])
+@pytest.mark.parametrize("case", ["clinical_slab", "offset_ragged", "spot_chunks"])
+def test_tile_heights_agree_bit_for_bit(precision, case):
+    """64-row tiles (2 mm voxels, 5 mm spots) give the 32-row tiles' dose bit for
+    bit: every voxel sums the same MFMA products in the same order (same layers,
+    K steps, windows and FP16 scale), only the sharing of operands differs. Both
+    are within the form's bound of the oracle, ragged tile edges included."""
+    from pbdose import engines
+    print(f"case tile heights {case} {precision}", flush=True)
+    if case == "clinical_slab":
+        p = make_problem(128, 128, 20, 0.2, 41, 41, 0.5, np.linspace(70, 180, 40), seed=2)
+    elif case == "offset_ragged":
+        p = make_problem(100, 90, 7, 0.2, 15, 13, 0.5, [85.0, 120.0, 160.0], x0=1.1, y0=-0.7, seed=3)
+    else:
+        p = make_problem(150, 70, 5, 0.2, 60, 55, 0.5, [90.0, 140.0], seed=5)
+    dp = engines.DeviceProblem(p)
+    ref = oracle_dose(p)
+    for chunk in (3, 40):
+        a = dp.forward(precision, layer_chunk=chunk, tile_rows=32).clone()
+        b = dp.forward(precision, layer_chunk=chunk, tile_rows=64).clone()
+        assert torch.equal(a, b), f"chunk {chunk}: {float((a - b).abs().max()):.3e}"
+        r = O.relative_error(b.double().cpu().numpy(), ref)["max_rel_err_global"]
+        assert r <= MIXED_TOL[precision]
+    from pbdose import _lib
+    c, rows = dp.auto_shape(_lib.PREC[precision])
+    assert rows in (32, 64) and c >= 1
+
+
+def test_tile_height_argument_checks():
+    from pbdose import engines
+    p = make_problem(64, 48, 4, 0.2, 11, 9, 0.5, [100.0, 130.0], seed=6)
+    dp = engines.DeviceProblem(p)
+    with pytest.raises(RuntimeError, match="unsupported"):
+        dp.forward("fp32", layer_chunk=2, tile_rows=64)          # FP32 tiles are 32 rows
+    with pytest.raises(RuntimeError, match="invalid dimension"):
+        dp.forward("bf16x3", layer_chunk=2, tile_rows=48)
+    q = make_problem(64, 48, 4, 0.2, 11, 9, 0.37, [100.0, 130.0], seed=6)
+    with pytest.raises(RuntimeError, match="unsupported"):
+        engines.DeviceProblem(q).forward("bf16x3", layer_chunk=2, tile_rows=64)   # no 2 : 5 table
 
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16x3", "fp16x3"])
@@ -275,3 +316,27 @@ def test_windowed_contraction(precision, case):
     far = ref < 1e-12 * ref.max()
     assert far.any()
     assert np.abs(D[far]).max() <= 1e-7 * ref.max()
+
+
+def test_forward_autotune_picks_a_timed_candidate(tmp_path, monkeypatch):
+    """DeviceProblem.autotune times (layer chunk, tile rows) candidates, 32- and
+    64-row tiles among them, and later automatic forwards use the winner: the
+    dose equals the explicit call with that shape bit for bit, and the cache
+    returns the same winner without timing again."""
+    from pbdose import _lib, engines
+    monkeypatch.setenv("PBDOSE_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
+    print("case forward autotune bf16x3", flush=True)
+    p = make_problem(128, 128, 24, 0.2, 41, 41, 0.5, np.linspace(70, 180, 40), seed=7)
+    dp = engines.DeviceProblem(p)
+    cands = dp.shape_candidates(_lib.PREC["bf16x3"])
+    assert {r for _, r in cands} == {32, 64}
+    best = dp.autotune("bf16x3", reps=3)
+    assert best in [(c, r) for c, r in cands]
+    assert len(dp.tuning_results) == len(cands)
+    a = dp.forward("bf16x3").clone()
+    b = dp.forward("bf16x3", layer_chunk=best[0], tile_rows=best[1])
+    assert torch.equal(a, b)
+    dp2 = engines.DeviceProblem(p)
+    assert dp2.autotune("bf16x3") == best and not hasattr(dp2, "tuning_results")
+    # FP32: 32-row tiles only
+    assert {r for _, r in dp.shape_candidates(_lib.PREC["fp32"])} == {32}
