@@ -62,7 +62,8 @@ int pbh_launch_fwd_fp32_batch(const pbh::BatchFwdItem* items, const int* wg0, in
                               bool tab, bool q5, hipStream_t s);
 bool fp32_q5(const pbh::FwdPlan& pl);
 int pbh_launch_fwd_mixed_batch(const pbh::BatchFwdItem* items, const int* wg0, int n, int total,
-                               int precision, int kb, bool tab, bool q5, hipStream_t s);
+                               int precision, int kb, bool tab, bool q5, int rows, unsigned* counters,
+                               hipStream_t s);
 int pbh_launch_wsplit_batch(const pbh::BatchFwdItem* items, int n, int max_chunks, int precision, hipStream_t s);
 int pbh_mixed_slot_mode(const pbh::FwdPlan& pl);
 bool pbh_adj_mixed_q5(const pbh::AdjArgs& A);

@@ -45,7 +45,7 @@ bool is_peak(int precision)
 
 PbhBatchTuning tuning_or_default(const PbhBatchTuning* t)
 {
-    PbhBatchTuning r{0, 0, 0, 0};
+    PbhBatchTuning r{0, 0, 0, 0, 0, 0};
     if (t) r = *t;
     return r;
 }
@@ -70,7 +70,7 @@ double fwd_work(const PbhBatchItem& it)
 
 struct Layout {
     // offsets in the workspace
-    size_t fwd_items, fwd_wg0, cmb_wg0, peak_items, obj_items, adj_items, adj_wg0, red_items, red_wg0;
+    size_t fwd_items, fwd_wg0, cmb_wg0, peak_items, obj_items, adj_items, adj_wg0, red_items, red_wg0, counters;
     size_t table_end;
     std::vector<size_t> fwd_ws, R, adj_part, obj_part;
     std::vector<size_t> fwd_ws_bytes;
@@ -99,10 +99,23 @@ int check_tuning(const PbhBatchTuning& t)
         return PBH_ERR_UNSUPPORTED;
     if (t.group != 0 && t.group != 1) return PBH_ERR_UNSUPPORTED;
     if (t.order != 0 && t.order != 1) return PBH_ERR_UNSUPPORTED;
+    if (t.tile_rows != 0 && t.tile_rows != 32 && t.tile_rows != 64) return PBH_ERR_UNSUPPORTED;
+    if (t.persistent != 0 && t.persistent != 1) return PBH_ERR_UNSUPPORTED;
     return PBH_OK;
 }
 
-Layout make_layout(const PbhBatchItem* items, int n, const PbhBatchTuning& t)
+// Dose-tile rows of an item's forward: 64 when requested for a 16-bit form on the
+// P = 2, Q = 5 table geometry, else 32.
+int item_rows(const PbhBatchItem& it, int precision, const PbhBatchTuning& t)
+{
+    if (t.tile_rows != 64 || !is_mixed(precision)) return FT_TY;
+    const FwdArgs a = make_fwd_args(&it.grid, &it.lattice, nullptr, nullptr, nullptr, nullptr, FT_TY);
+    FwdPlan pl;
+    if (pbh_plan_forward(a, 48, 64, nullptr, 0, a.L, &pl) != PBH_OK) return FT_TY;
+    return (pl.table && pl.tx.P == 2 && pl.tx.Q == 5 && pl.ty.P == 2 && pl.ty.Q == 5) ? 64 : FT_TY;
+}
+
+Layout make_layout(const PbhBatchItem* items, int n, int precision, const PbhBatchTuning& t)
 {
     Layout L;
     int n_obj = 0;
@@ -119,6 +132,9 @@ Layout make_layout(const PbhBatchItem* items, int n, const PbhBatchTuning& t)
     L.adj_wg0 = off; off = align256(off + (size_t)2 * std::max(n_obj, 1) * sizeof(int) + sizeof(int));
     L.red_items = off; off = align256(off + (size_t)std::max(n_obj, 1) * sizeof(BatchRedItem));
     L.red_wg0 = off; off = align256(off + (size_t)(std::max(n_obj, 1) + 1) * sizeof(int));
+    // claim counters of persistent forward launches, two per group (zero-filled by the
+    // caller with the workspace, and left at zero by every launch)
+    L.counters = off; off = align256(off + (size_t)2 * n * sizeof(unsigned));
     L.table_end = off;
     L.fwd_ws.resize(n);
     L.fwd_ws_bytes.resize(n);
@@ -127,7 +143,8 @@ Layout make_layout(const PbhBatchItem* items, int n, const PbhBatchTuning& t)
     L.obj_part.assign(n, 0);
     for (int k = 0; k < n; ++k) {
         const PbhBatchItem& it = items[k];
-        const FwdArgs a = make_fwd_args(&it.grid, &it.lattice, nullptr, nullptr, nullptr, nullptr);
+        const FwdArgs a = make_fwd_args(&it.grid, &it.lattice, nullptr, nullptr, nullptr, nullptr,
+                                        item_rows(it, precision, t));
         const size_t fb = pbh_forward_ws_bytes(a, t.fwd_chunk);
         L.fwd_ws[k] = off;
         L.fwd_ws_bytes[k] = fb;
@@ -163,6 +180,8 @@ struct PbhBatch {
     int* d_red_wg0 = nullptr;
     int red_total = 0;
     std::vector<Launch> fwd, adj;
+    bool persistent = false;
+    unsigned* d_counters = nullptr;    // two per forward group (persistent launches)
 };
 
 // Splits items (already in launch order) into groups of equal key and mode.
@@ -189,7 +208,7 @@ extern "C" size_t pbh_batch_workspace_bytes(const PbhBatchItem* items, int n_ite
     if (check_items(items, n_items, precision) != PBH_OK) return 0;
     const PbhBatchTuning t = tuning_or_default(tuning);
     if (check_tuning(t) != PBH_OK) return 0;
-    return make_layout(items, n_items, t).total;
+    return make_layout(items, n_items, precision, t).total;
 }
 
 extern "C" int pbh_batch_create(const PbhBatchItem* items, int n, int precision, const PbhBatchTuning* tuning,
@@ -202,7 +221,7 @@ extern "C" int pbh_batch_create(const PbhBatchItem* items, int n, int precision,
     const PbhBatchTuning t = tuning_or_default(tuning);
     if ((rc = check_tuning(t)) != PBH_OK) return rc;
     if (!workspace) return PBH_ERR_NULL_ARG;
-    const Layout L = make_layout(items, n, t);
+    const Layout L = make_layout(items, n, precision, t);
     if (workspace_bytes < L.total) return PBH_ERR_WORKSPACE;
     char* ws = (char*)workspace;
 
@@ -213,8 +232,11 @@ extern "C" int pbh_batch_create(const PbhBatchItem* items, int n, int precision,
     if (t.group == 0) std::fill(key.begin(), key.end(), kmax);
     std::vector<int> order(n);
     for (int k = 0; k < n; ++k) order[k] = k;
+    std::vector<int> rows(n);
+    for (int k = 0; k < n; ++k) rows[k] = item_rows(items[k], precision, t);
     std::stable_sort(order.begin(), order.end(), [&](int p, int q) {
         if (key[p] != key[q]) return key[p] > key[q];
+        if (rows[p] != rows[q]) return rows[p] > rows[q];
         return t.order == 1 && fwd_work(items[p]) > fwd_work(items[q]);
     });
 
@@ -229,7 +251,7 @@ extern "C" int pbh_batch_create(const PbhBatchItem* items, int n, int precision,
         const int k = order[p];
         const PbhBatchItem& it = items[k];
         BatchFwdItem& f = hf[p];
-        f.a = make_fwd_args(&it.grid, &it.lattice, it.w, it.idd, it.sigma, it.dose);
+        f.a = make_fwd_args(&it.grid, &it.lattice, it.w, it.idd, it.sigma, it.dose, item_rows(it, precision, t));
         const int kx = mixed ? 16 * key[k] : key[k];
         const int ky = mixed ? 64 : key[k];
         rc = pbh_plan_forward(f.a, kx, ky, ws + L.fwd_ws[k], L.fwd_ws_bytes[k], t.fwd_chunk, &f.pl);
@@ -244,7 +266,9 @@ extern "C" int pbh_batch_create(const PbhBatchItem* items, int n, int precision,
         f.r_peak = is_peak(precision) ? pbh_forward_ws_peaks(f.a, ws + L.fwd_ws[k]) : nullptr;
         hp[p] = BatchPeakItem{it.idd, it.lattice.n_layers, it.grid.nz, it.grid.oz, it.grid.dz, (float*)f.r_peak};
         keys[p] = key[k];
-        tabs[p] = (char)(mixed ? pbh_mixed_slot_mode(f.pl) : ((f.pl.table ? 1 : 0) | (fp32_q5(f.pl) ? 2 : 0)));
+        // forward mode: bit 0 table factors, bit 1 compile-time slots, bit 2 64-row tiles
+        tabs[p] = (char)((mixed ? pbh_mixed_slot_mode(f.pl) : ((f.pl.table ? 1 : 0) | (fp32_q5(f.pl) ? 2 : 0))) |
+                         (f.a.tile_rows == 64 ? 4 : 0));
         const long long w = (long long)f.a.tiles_x * f.a.tiles_y * f.a.nz * f.pl.nchunk;
         if (w > 0x3fffffffLL) return PBH_ERR_UNSUPPORTED;
         wgs[p] = (int)w;
@@ -372,6 +396,8 @@ extern "C" int pbh_batch_create(const PbhBatchItem* items, int n, int precision,
     b->d_adj_wg0 = (int*)(ws + L.adj_wg0);
     b->d_red = (BatchRedItem*)(ws + L.red_items);
     b->d_red_wg0 = (int*)(ws + L.red_wg0);
+    b->d_counters = (unsigned*)(ws + L.counters);
+    b->persistent = t.persistent != 0 && is_mixed(precision);
     std::vector<char> host(L.table_end, 0);
     std::memcpy(host.data() + L.fwd_items, hf.data(), hf.size() * sizeof(BatchFwdItem));
     std::memcpy(host.data() + L.fwd_wg0, fwd_wg0.data(), fwd_wg0.size() * sizeof(int));
@@ -408,7 +434,9 @@ extern "C" int pbh_batch_forward_adjoint(const PbhBatch* b, void* stream)
                                            g.mode & 1, (g.mode & 2) != 0, s);
         else
             rc = pbh_launch_fwd_mixed_batch(b->d_fwd + g.first, b->d_fwd_wg0 + g.wg0_off, g.count, g.total,
-                                            b->precision, g.key, g.mode & 1, (g.mode & 2) != 0, s);
+                                            b->precision, g.key, g.mode & 1, (g.mode & 2) != 0,
+                                            (g.mode & 4) ? 64 : 32,
+                                            b->persistent ? b->d_counters + 2 * (&g - b->fwd.data()) : nullptr, s);
         if (rc != PBH_OK) return rc;
     }
     if (b->cmb_total > 0 && (rc = pbh_launch_combine_batch(b->d_fwd, b->d_cmb_wg0, b->n, b->cmb_total, s)) != PBH_OK)

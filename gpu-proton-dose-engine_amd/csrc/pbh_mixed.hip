@@ -30,6 +30,8 @@
 // dose scaled by 2^-eacc: every layer's U is stored below 2^14 with no reduction
 // in the kernel, and its products accumulate straight into the accumulators, as
 // for BF16; the tile is scaled back by 2^eacc when it is stored.
+#include <algorithm>
+
 #include "pbh_mixed_core.h"
 
 namespace pbh {
@@ -566,15 +568,41 @@ void fwd_mixed_kernel(FwdArgs a, FwdPlan pl, double oz, double dz, const float* 
 // workgroups scan their slices' layers (no precomputed lists: with both the item
 // lookup and the copy path the kernel needed more than the 168 VGPRs of 3 workgroups
 // per CU and spilled).
-template <typename T, int MODE, bool TAB, int KB, int QC>
-__global__ __launch_bounds__(FT_THREADS, (MixedOcc<MODE, QC, 32>::value))
-void fwd_mixed_batch_kernel(const BatchFwdItem* __restrict__ items, const int* __restrict__ wg0, int n)
+//
+// counters == null: workgroup u of the launch runs work unit u (u < total).
+// counters != null (persistent launch, grid = the workgroups that fit at once): each
+// workgroup claims units from counters[0] until they run out; the last workgroup to
+// leave (counters[1]) sets both counters back to 0 for the next launch. Every unit
+// is claimed once, so the results equal the one-workgroup-per-unit launch's.
+template <typename T, int MODE, bool TAB, int KB, int QC, int TY>
+__global__ __launch_bounds__(FT_THREADS, (MixedOcc<MODE, QC, TY>::value))
+void fwd_mixed_batch_kernel(const BatchFwdItem* __restrict__ items, const int* __restrict__ wg0, int n, int total,
+                            unsigned* counters)
 {
-    const int k = batch_item(wg0, n, blockIdx.x);
-    const FwdArgs a = items[k].a;
-    const FwdPlan pl = items[k].pl;
-    fwd_mixed_body<T, MODE, TAB, KB, QC, false, 32>(a, pl, items[k].oz, items[k].dz, items[k].r_peak,
-                                                 (int)blockIdx.x - wg0[k]);
+    auto run = [&](int u) {
+        const int k = batch_item(wg0, n, u);
+        const FwdArgs a = items[k].a;
+        const FwdPlan pl = items[k].pl;
+        fwd_mixed_body<T, MODE, TAB, KB, QC, false, TY>(a, pl, items[k].oz, items[k].dz, items[k].r_peak,
+                                                      u - wg0[k]);
+    };
+    if (!counters) {
+        run((int)blockIdx.x);
+        return;
+    }
+    __shared__ int next;
+    for (;;) {
+        if (threadIdx.x == 0) next = (int)atomicAdd(&counters[0], 1u);
+        __syncthreads();
+        const int u = next;
+        __syncthreads();           // every wave has read `next` (and left the last unit)
+        if (u >= total) break;
+        run(u);
+    }
+    if (threadIdx.x == 0 && atomicAdd(&counters[1], 1u) == gridDim.x - 1) {
+        atomicExch(&counters[0], 0u);
+        atomicExch(&counters[1], 0u);
+    }
 }
 
 // r_peak of every layer of every item: one workgroup per item, one wave per layer
@@ -665,13 +693,25 @@ struct SingleLaunch {
 struct BatchLaunch {
     const BatchFwdItem* items;
     const int* wg0;
-    int n, total;
+    int n, total, rows;
+    unsigned* counters;     // persistent launch: claim counters; null: one workgroup per unit
     hipStream_t s;
     template <typename T, int MODE, bool TAB, int KB, int QC>
     int go() const
     {
-        hipLaunchKernelGGL((fwd_mixed_batch_kernel<T, MODE, TAB, KB, QC>), dim3((unsigned)total), dim3(FT_THREADS),
-                           0, s, items, wg0, n);
+        if (rows == 64 && !(TAB && QC == 5)) return PBH_ERR_UNSUPPORTED;
+        // a persistent launch has as many workgroups as are resident at once
+        const int occ = rows == 64 ? MixedOcc<MODE, QC, 64>::value : MixedOcc<MODE, QC, 32>::value;
+        const long long resident = (long long)occ * pbh_device_cus();
+        const unsigned grid = counters ? (unsigned)std::min<long long>(total, resident) : (unsigned)total;
+        if (rows == 64) {
+            if constexpr (TAB && QC == 5)
+                hipLaunchKernelGGL((fwd_mixed_batch_kernel<T, MODE, TAB, KB, QC, 64>), dim3(grid), dim3(FT_THREADS),
+                                   0, s, items, wg0, n, total, counters);
+        } else {
+            hipLaunchKernelGGL((fwd_mixed_batch_kernel<T, MODE, TAB, KB, QC, 32>), dim3(grid), dim3(FT_THREADS), 0, s,
+                               items, wg0, n, total, counters);
+        }
         return launch_rc("fwd_mixed_batch_kernel");
     }
 };
@@ -730,10 +770,11 @@ int pbh_launch_wsplit_batch(const BatchFwdItem* items, int n, int max_chunks, in
 // Batched mixed-precision forward: items[0..n) share kb (stage B K steps), the
 // table mode and the slot mode (q5); total = wg0[n].
 int pbh_launch_fwd_mixed_batch(const BatchFwdItem* items, const int* wg0, int n, int total, int precision,
-                               int kb, bool tab, bool q5, hipStream_t s)
+                               int kb, bool tab, bool q5, int rows, unsigned* counters, hipStream_t s)
 {
     if (kb < 1 || kb > 3) return PBH_ERR_UNSUPPORTED;
-    return dispatch_precision(precision, MixedKey{kb, tab, q5 && tab}, BatchLaunch{items, wg0, n, total, s});
+    return dispatch_precision(precision, MixedKey{kb, tab, q5 && tab},
+                              BatchLaunch{items, wg0, n, total, rows, counters, s});
 }
 
 // Template key of a planned item, for grouping a batch: bit 0 = table, bit 1 = q5.

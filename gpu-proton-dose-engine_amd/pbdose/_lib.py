@@ -48,7 +48,7 @@ class PbhBatchItem(C.Structure):
 
 class PbhBatchTuning(C.Structure):
     _fields_ = [("fwd_chunk", C.c_int32), ("adj_split", C.c_int32), ("group", C.c_int32),
-                ("order", C.c_int32)]
+                ("order", C.c_int32), ("tile_rows", C.c_int32), ("persistent", C.c_int32)]
 
 
 _P = C.c_void_p

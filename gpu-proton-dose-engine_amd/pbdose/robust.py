@@ -27,8 +27,6 @@ from __future__ import annotations
 
 import ctypes as C
 import hashlib
-import json
-import os
 from dataclasses import dataclass
 from typing import List, Optional, Sequence
 
@@ -110,19 +108,23 @@ def central_cube(grid: DoseGrid, half: float = 3.0, dose_value: float = 2.0, w_i
 # ---------------------------------------------------------------------------
 # tuning
 # ---------------------------------------------------------------------------
-DEFAULT_TUNING = {"fwd_chunk": 0, "adj_split": 0, "group": 1, "order": 1}
+DEFAULT_TUNING = {"fwd_chunk": 0, "adj_split": 0, "group": 1, "order": 1, "tile_rows": 0, "persistent": 0}
 
 # forward: automatic layer chunks per item, or one chunk (all layers of a tile
-# in one workgroup); adjoint: workgroups per depth slice
-CANDIDATES = [{"fwd_chunk": f, "adj_split": a, "group": g, "order": o}
-              for f in (0, 4096) for a in (2, 4, 8) for g in (0, 1) for o in (0, 1)]
+# in one workgroup); 32- or 64-row dose tiles; one workgroup per work unit, or a
+# persistent launch claiming units from a counter; adjoint: workgroups per depth
+# slice; items in the given order or largest first
+CANDIDATES = [{"fwd_chunk": f, "adj_split": a, "group": g, "order": o, "tile_rows": r, "persistent": p}
+              for f in (0, 4096) for a in (2, 4, 8) for g in (0, 1) for o in (0, 1)
+              for r in (32, 64) for p in (0, 1)]
 
 
 from .tuning import cache_path as _cache_path, load_cache as _load_cache, save_cache as _save_cache  # noqa: E402
 
 
 def tuning_struct(t: dict) -> _lib.PbhBatchTuning:
-    return _lib.PbhBatchTuning(int(t["fwd_chunk"]), int(t["adj_split"]), int(t["group"]), int(t["order"]))
+    return _lib.PbhBatchTuning(int(t["fwd_chunk"]), int(t["adj_split"]), int(t["group"]), int(t["order"]),
+                               int(t.get("tile_rows", 0)), int(t.get("persistent", 0)))
 
 
 # ---------------------------------------------------------------------------

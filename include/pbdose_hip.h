@@ -365,6 +365,14 @@ typedef struct PbhBatchTuning {
     int32_t group;           /* 0: one forward launch at the widest spot chunk;
                                 1: one forward launch per chunk width */
     int32_t order;           /* 0: items in the given order; 1: largest work first */
+    int32_t tile_rows;       /* forward dose-tile rows: 0 or 32, or 64 for the 16-bit forms
+                                (items without 2 : 5 pitch tables keep 32) */
+    int32_t persistent;      /* 0: one workgroup per (item, tile, layer chunk) of a forward
+                                launch; 1: a persistent forward launch of as many workgroups
+                                as fit on the GPU at once, each claiming the next
+                                (item, tile, chunk) of the group's work list from a global
+                                atomic counter (the last workgroup out resets it, so the
+                                launch can be replayed from a graph) */
 } PbhBatchTuning;
 
 typedef struct PbhBatch PbhBatch;   /* opaque host-side plan (host memory only) */

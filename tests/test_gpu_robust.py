@@ -34,6 +34,16 @@ def small_fields():
     return [fa, fb]
 
 
+def fields_with_25_pitch():
+    """small_fields plus a field on 2 mm voxels / 5 mm spots, the geometry of the
+    64-row mixed tiles."""
+    from pbdose import model, robust
+    g = model.DoseGrid(72, 70, 20, 0.2, 0.2, 0.2, -(71) * 0.2 / 2, -(69) * 0.2 / 2, 0.0)
+    fc = robust.Field(g, model.SpotLattice(13, 11, 3, 0.5, 0.5, 0.2, 0.1, energies=np.array([85.0, 110.0, 135.0]),
+                                           weights=np.random.default_rng(2).lognormal(0, 0.5, (3, 11, 13))))
+    return small_fields() + [fc]
+
+
 def small_scenarios():
     from pbdose import robust
     S = robust.Scenario
@@ -98,7 +108,8 @@ def check_items(b, fields, scen, precision):
     """Every item of a run batch vs the FP64 oracle (dose, dJ/dw, J) and vs the
     single-problem path."""
     # the dose bounds of the single-problem forms (tests/test_gpu_forward.py)
-    tol = {"fp32": 1e-6, "bf16x3": 2e-5, "fp16x3": 2e-6, "bf16x3_peak": 5e-3}.get(precision, 2e-5)
+    tol = {"fp32": 1e-6, "bf16x3": 2e-5, "fp16x3": 2e-6, "bf16x3_peak": 5e-3, "bf16": 5e-3,
+           "fp16": 1e-3}[precision]
     worst = [0.0] * 5
     for k, (fi, si) in enumerate(b.items):
         D = b.dose[k].double().cpu().numpy()
@@ -168,11 +179,14 @@ def test_forward_only_batch_and_lut_gradients():
         assert rel(bl.grad_w[k].cpu(), gw.cpu()) <= 1e-6
 
 
-def test_graph_replay_equals_eager_runs():
+@pytest.mark.parametrize("variant", [{}, {"tile_rows": 64, "persistent": 1, "fwd_chunk": 2}])
+def test_graph_replay_equals_eager_runs(variant):
+    """Graph replays equal eager runs bit for bit, also with the persistent forward
+    launch, whose claim counters every launch leaves at zero for the next."""
     from pbdose import robust
-    print("case robust batch graph replays", flush=True)
-    fields, scen = small_fields(), small_scenarios()
-    b = robust.RobustBatch(fields, scen, precision="bf16x3", tuning=robust.DEFAULT_TUNING)
+    print(f"case robust batch graph replays {variant}", flush=True)
+    fields, scen = fields_with_25_pitch(), small_scenarios()
+    b = robust.RobustBatch(fields, scen, precision="bf16x3", tuning={**robust.DEFAULT_TUNING, **variant})
     b.capture()
     for rep in range(3):
         # change the inputs between replays: the graph must read them live
@@ -192,6 +206,31 @@ def test_graph_replay_equals_eager_runs():
         assert all(torch.equal(x, y) for x, y in zip(b.grad_w, eager[1])), rep
         assert torch.equal(b.J, eager[2]), rep
     print(f"  3 replays bit-identical to eager runs ({b.launch_count} kernel nodes)", flush=True)
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16x3", "bf16"])
+def test_tile_height_and_persistent_launch_change_no_bit(precision):
+    """The forward of a batch with 64-row tiles and/or a persistent launch (work
+    units claimed from a counter) equals the 32-row, one-workgroup-per-unit launch
+    bit for bit, at the same layer chunk; each is checked against the oracle."""
+    from pbdose import robust
+    print(f"case batch tile heights / persistent launch {precision}", flush=True)
+    fields, scen = fields_with_25_pitch(), small_scenarios()
+    ref = None
+    for rows in (32, 64):
+        for pers in (0, 1):
+            t = {**robust.DEFAULT_TUNING, "fwd_chunk": 2, "tile_rows": rows, "persistent": pers}
+            b = robust.RobustBatch(fields, scen, precision=precision, tuning=t)
+            b.run_eager()
+            b.run_eager()            # a second launch starts from the counters the first reset
+            torch.cuda.synchronize()
+            dose = [d.clone() for d in b.dose]
+            if ref is None:
+                ref = dose
+                check_items(b, fields, scen, precision)
+            else:
+                assert all(torch.equal(x, y) for x, y in zip(dose, ref)), (rows, pers)
+            b.close()
 
 
 def test_autotune_picks_a_measured_candidate(tmp_path, monkeypatch):
@@ -226,6 +265,19 @@ def test_robust_workload_128cube_63_items():
         assert abs(float(b.J[k]) - float(J1)) <= 1e-6 * abs(float(J1))
     print(f"  batch vs single path: dose {ws:.1e}, dJ/dw {wg:.1e}", flush=True)
     assert ws <= 1e-6 and wg <= 1e-6
+    # one persistent launch of 64-row tiles over the same 63 items: the same doses
+    t = {**robust.DEFAULT_TUNING, "fwd_chunk": b.tuning["fwd_chunk"] or 4096, "group": 0, "tile_rows": 64,
+         "persistent": 1}
+    b2 = robust.RobustBatch(fields, scen, precision="bf16x3", tuning=t)
+    b2.run()
+    torch.cuda.synchronize()
+    b3 = robust.RobustBatch(fields, scen, precision="bf16x3", tuning={**t, "tile_rows": 32, "persistent": 0})
+    b3.run()
+    torch.cuda.synchronize()
+    assert all(torch.equal(x, y) for x, y in zip(b2.dose, b3.dose))
+    print(f"  persistent 64-row forward: {b2.launch_count} launches, doses equal the 32-row launch", flush=True)
+    b2.close()
+    b3.close()
     zs = np.arange(0, 128, 16)
     for k in (1, 30, 62):
         fi, si = b.items[k]
