@@ -46,15 +46,18 @@ import torch  # noqa: E402
 # events: weight split, forward, layer-chunk combine), from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE (KiB per dispatch, summed over the call's kernels) scaled by the same
 # counters on a 256 MiB copy (fetch x 2.000, write x 1.000):
-# profiles/counters_forward_adjoint_128cube_67240spots.md
-TRAFFIC_KIB = {"fp32": (5185.5, 15232.0), "bf16x3": (8600.5, 20943.6), "bf16": (8121.8, 27087.6)}
+# profiles/counters_forward_adjoint_128cube_67240spots.md (the shapes bench.py's autotuner picks:
+# 64-row tiles, layer chunk 14 for the 16-bit forms; 32 rows, chunk 26 for FP32)
+TRAFFIC_KIB = {"fp32": (5189.5, 15232.0), "bf16x3": (9564.4, 22927.6), "bf16": (8595.1, 22927.6),
+               "fp16x3": (9579.6, 22928.9)}
 CALL_KERNELS = {"fp32": "fwd_fp32_kernel + combine_kernel"}
 KERNEL_NAME = {"fp32": "fwd_fp32_kernel"}
 # MFMA FLOP the contraction kernel issues per launch on this workload (not the
 # algorithmic count: windows skip K steps, the x3 forms run three products), from
 # the rocprofv3 counter SQ_INSTS_VALU_MFMA_MOPS_* (units of 512 FLOP) in the
 # counter profile under profiles/; None where not profiled
-MFMA_FLOP_ISSUED: dict = {}
+MFMA_FLOP_ISSUED = {"fp32": 11348832 * 512.0, "bf16x3": 22984032 * 512.0, "bf16": 7661344 * 512.0,
+                    "fp16x3": 22984032 * 512.0}
 FETCH_SCALE, WRITE_SCALE = 2.000, 1.000
 
 # published dense MFMA peaks of MI355X (TFLOP/s) for the arithmetic each form runs on
@@ -277,17 +280,21 @@ def cpu_leg(prob, budget_s=12.0, max_reps=15):
 
 
 def profile_kernels(forms, reps=5):
-    """Short run for rocprofv3: forwards of the given forms, objective + adjoint,
-    and three 256 MiB device copies (known bytes, to calibrate HBM counters)."""
+    """Short run for rocprofv3: forwards of the given forms (at bench.py's autotuned
+    tile shapes), objective + adjoint, and three 256 MiB device copies (known bytes,
+    to calibrate HBM counters)."""
     from pbdose import planning
     dev = torch.device("cuda", 0)
     prob, dp = build_field(dev)
     out = torch.empty(dp.shape, dtype=torch.float32, device=dev)
     for f in forms:
+        # the shape bench.py times (measured, or read from $PBDOSE_AUTOTUNE_CACHE when a
+        # previous run of this mode filled it, so that a profiled run times nothing else)
+        chunk, rows = dp.autotune(f)
         for _ in range(reps):
             dp.forward(f, out=out)
         torch.cuda.synchronize()
-        print(f"forward {f}: {reps} launches", flush=True)
+        print(f"forward {f}: {reps} launches ({rows}-row tiles, layer chunk {chunk})", flush=True)
     Dp, wv = planning.central_cube(prob.grid, device=dev)
     D = dp.forward("fp32")
     for aform in sorted({adjoint_form(f) for f in forms} | {"fp32"}):

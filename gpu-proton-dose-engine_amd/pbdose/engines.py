@@ -133,7 +133,7 @@ class DeviceProblem:
         g, lat = self.problem.grid, self.problem.lattice
         key = "fwd|" + "|".join(str(v) for v in (
             g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, lat.n_sx, lat.n_sy, lat.n_layers, lat.dx_spot,
-            lat.dy_spot, precision, torch.cuda.get_device_name(self.dev), self._active.tobytes().hex()))
+            lat.dy_spot, precision, tuning.device_key(self.dev), self._active.tobytes().hex()))
         cache = tuning.load_cache() if use_cache else {}
         if key in cache:
             best = tuple(cache[key]["shape"])

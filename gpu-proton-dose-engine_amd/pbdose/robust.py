@@ -249,7 +249,8 @@ class RobustBatch:
 
     def signature(self) -> str:
         """Key of this batch's shape for the tuning cache."""
-        parts = [self.precision, str(self.objective), torch.cuda.get_device_name(self.dev)]
+        from .tuning import device_key
+        parts = [self.precision, str(self.objective), device_key(self.dev)]
         for p in self.problems:
             g, lat = p.grid, p.lattice
             parts.append(f"{g.nx}x{g.ny}x{g.nz}@{g.dx:g},{g.dy:g}:{lat.n_sx}x{lat.n_sy}x{lat.n_layers}"

@@ -9,6 +9,8 @@ from __future__ import annotations
 import json
 import os
 
+import torch
+
 
 def cache_path() -> str:
     p = os.environ.get("PBDOSE_AUTOTUNE_CACHE")
@@ -36,3 +38,10 @@ def save_cache(c: dict) -> None:
         os.replace(tmp, path)
     except OSError:
         pass
+
+
+def device_key(dev) -> str:
+    """Identity of a GPU for the cache: architecture and compute-unit count (the
+    marketing name is not reported the same way in every process)."""
+    p = torch.cuda.get_device_properties(dev)
+    return f"{getattr(p, 'gcnArchName', p.name)}/{p.multi_processor_count}"
