@@ -11,7 +11,9 @@ kernel and synchronises, returning a (nz, ny, nx) float32 device tensor.
 """
 from __future__ import annotations
 
+import dataclasses
 import time
+import warnings
 from typing import Optional
 
 import numpy as np
@@ -247,23 +249,54 @@ def torch_separable(problem: PBProblem, device: str = "cuda", dtype=None,
                   mixed forms selected by ``precision``).
     allow_tf32    accepted for compatibility; gfx950 has no TF32 mode, so it
                   has no effect (the reference measured none either).
-    per_layer     must be True: the averaged-sigma approximation is not offered.
+    per_layer     True (default): sigma exact per layer. False: the reference's
+                  single-sigma approximation (engines.py:243-255), on the GPU:
+                  one width per slice, the layer-weight average sigma_bar(z)
+                  (engines.py:75-80), applied to A(z) = sum_l IDD_l(z) w_l.
     layer_chunk   accepted and ignored: no layer-sized intermediate exists.
     precision     'fp32' (default) or one of 'bf16', 'fp16', 'bf16x3',
                   'fp16x3', 'bf16x3_peak', 'fp16x3_peak'.
     Returns D (nz, ny, nx) on the device, or (D, wall_seconds).
     """
-    if not per_layer:
-        raise NotImplementedError("per_layer=False (single averaged sigma) is not supported: "
-                                  "sigma is exact per layer")
     if dtype not in (None, torch.float32):
         raise ValueError(f"dtype {dtype} not supported: the dose is computed in float32")
+    if return_intermediates:
+        warnings.warn("return_intermediates is ignored: no intermediate is materialised", stacklevel=2)
     t0 = time.perf_counter()
-    dp = DeviceProblem(problem, device)
+    dp = DeviceProblem(problem, device) if per_layer else averaged_problem(problem, device)
     D = dp.forward(precision)
     torch.cuda.synchronize(dp.dev)
     dt = time.perf_counter() - t0
     return (D, dt) if return_timing else D
+
+
+def averaged_problem(problem: PBProblem, device="cuda") -> DeviceProblem:
+    """The single-sigma approximation (per_layer=False, engines.py:184-191,
+    :243-255) as a DeviceProblem of nz layers, written on the device by
+    pbh_average_sigma: layer k carries the amplitude map A(z_k) = sum_l IDD_l(z_k)
+    w_l, an IDD of 1 at slice k only, and sigma_bar on every slice, so that the
+    forward gives slice z the dose Gy A(z) Gx^T with width sigma_bar(z). The
+    sigma_bar(z) of the reference (engines.py:75-80) is ``.sigma[0]``."""
+    dev = _lib.require_cuda(device)
+    g, lat = problem.grid, problem.lattice
+    w = torch.as_tensor(lat.ensure_weights(), dtype=torch.float64, device=dev).contiguous()
+    idd = torch.as_tensor(problem.idd, dtype=torch.float64, device=dev).contiguous()
+    sig = torch.as_tensor(problem.sigma, dtype=torch.float64, device=dev).contiguous()
+    L, nz = lat.n_layers, g.nz
+    if tuple(w.shape) != (L, lat.n_sy, lat.n_sx) or tuple(idd.shape) != (L, nz) or tuple(sig.shape) != (L, nz):
+        raise ValueError("weights must be (L, nsy, nsx) and idd/sigma (L, nz)")
+    wa = torch.empty((nz, lat.n_sy, lat.n_sx), dtype=torch.float32, device=dev)
+    ia = torch.empty((nz, nz), dtype=torch.float64, device=dev)
+    sa = torch.empty((nz, nz), dtype=torch.float64, device=dev)
+    wl = torch.empty((L,), dtype=torch.float64, device=dev)
+    _lib.check(_lib.lib().pbh_average_sigma(_lib.ptr(w), _lib.ptr(idd), _lib.ptr(sig), L, nz, lat.n_sy, lat.n_sx,
+                                            _lib.ptr(wa), _lib.ptr(ia), _lib.ptr(sa), _lib.ptr(wl),
+                                            _lib.stream_of(dev)), "pbh_average_sigma")
+    lat_a = dataclasses.replace(lat, n_layers=nz, energies=None, weights=None)
+    pa = PBProblem(g, lat_a, problem.idd, problem.sigma, dict(problem.meta), problem.fluence_normalized)
+    dp = DeviceProblem(pa, dev, weights=wa, idd=ia, sigma=sa)
+    dp.layer_sums = wl
+    return dp
 
 
 def separable_hip(problem: PBProblem, device="cuda", precision: str = "fp32") -> torch.Tensor:

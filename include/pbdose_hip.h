@@ -270,6 +270,20 @@ int pbh_forward_noalloc(const PbhGrid* g, const PbhLattice* lat,
 int pbh_forward_choose_shape(const PbhGrid* g, const PbhLattice* lat, int precision,
                              const int32_t* active_per_z, int compute_units, int32_t* tile_rows);
 
+/* The single-sigma approximation (per_layer=False, engines.py:184-191, :243-254)
+ * as an ordinary problem for the forward entry points: from weights w (L, nsy,
+ * nsx) FP64 and tables idd, sigma (L, nz) FP64 it writes a problem of nz layers,
+ * w_avg (nz, nsy, nsx) FP32 = A(z) = sum_l IDD_l(z) w_l (engines.py:63-72),
+ * idd_avg (nz, nz) FP64 = the identity, sigma_avg (nz, nz) FP64 = sigma_bar(z)
+ * on every row, sigma_bar(z) = sum_l (W_l / sum W) sigma_l(z) with W_l the sum of
+ * layer l's weights (engines.py:75-80; layer_sums[L] receives the W_l). A
+ * forward of that problem on the same grid gives slice z the dose
+ * Gy A(z) Gx^T with both factors of width sigma_bar(z). FP64 sums in layer
+ * order. */
+int pbh_average_sigma(const double* w, const double* idd, const double* sigma, int n_layers, int nz,
+                      int nsy, int nsx, float* w_avg, double* idd_avg, double* sigma_avg,
+                      double* layer_sums, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Adjoint of the forward map (inverse planning).
  * ------------------------------------------------------------------------- */

@@ -450,10 +450,16 @@ def sigma_bar(w, sigma):
 
 
 def separable_dose_averaged(x, y, sx, sy, w, idd, sigma):
-    """FP64 single-sigma approximation (per_layer=False). engines.py:184-191
+    """FP64 single-sigma approximation (per_layer=False) as the PyTorch engine
+    computes it, engines.py:243-254: A(z) = sum_l IDD_l(z) W_l (nz, nsy, nsx) and
+    one Gaussian width sigma_bar(z) per slice, D[z] = Gy(z) A(z) Gx(z)^T, shape
+    (nz, ny, nx).
 
-    A(z) = sum_l IDD_l(z) W_l and one Gaussian width sigma_bar(z) per slice:
-    D[z] = Gy(z) A(z) Gx(z)^T."""
+    The NumPy engine's version (engines.py:184-191) pairs A's x-spot index with
+    Gy's spot index in its first einsum and returns (nz, nx, ny) after its
+    transpose; the two slips cancel when the x and y voxel axes and spot
+    positions coincide (square centred fields, such as its fixture here), and
+    the PyTorch engine is the drop-in target, so its order is the one kept."""
     w = np.asarray(w, dtype=np.float64)
     idd = np.asarray(idd, dtype=np.float64)
     L, nz = idd.shape
@@ -461,8 +467,8 @@ def separable_dose_averaged(x, y, sx, sy, w, idd, sigma):
     sb = sigma_bar(w, sigma)
     Gx = lateral(x, sx, sb)                                  # (nz, nx, nsx)
     Gy = lateral(y, sy, sb)                                  # (nz, ny, nsy)
-    T = np.einsum("zij,zvj->ziv", A, Gy)
-    return np.einsum("zai,ziy->zya", Gx, T).transpose(0, 2, 1)
+    T = np.einsum("zji,zvj->ziv", A, Gy)                     # (nz, nsx, ny)
+    return np.einsum("zai,ziv->zva", Gx, T)                  # (nz, ny, nx)
 
 
 def gaussian_field_weights(n_layers, n_spot=41, spot_spacing=0.5, spot_sigma=0.28, seed=0):

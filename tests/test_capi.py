@@ -63,13 +63,29 @@ def test_forward_rejects_null_and_bad_precision_without_launching():
     assert L.pbh_forward(g, lat, one, one, one, one, 99, nul, 0, nul) == -5
     # the profiled forward needs both events; the no-workspace forward checks its
     # arguments and its scratch limit before touching the device
-    assert L.pbh_forward_profiled(g, lat, one, one, one, one, 3, 0, nul, 0, nul, one, nul) == -1
+    assert L.pbh_forward_profiled(g, lat, one, one, one, one, 3, 0, 0, nul, 0, nul, one, nul) == -1
     assert L.pbh_forward_noalloc(g, lat, nul, nul, nul, nul, 3, 0, nul) == -1
     assert L.pbh_forward_noalloc(g, lat, one, one, one, one, 99, 0, nul) == -5
     big = _lib.PbhGrid(512, 512, 512, 0, 0.2, 0.2, 0.2, 0, 0, 0)
     blat = _lib.PbhLattice(41, 41, 40, 0, 0.5, 0.5, 0.0, 0.0)
     assert L.pbh_forward_workspace_bytes_tuned(big, blat, 3, 1) > _lib.PBH_NOALLOC_SCRATCH_BYTES
     assert L.pbh_forward_noalloc(big, blat, one, one, one, one, 3, 1, nul) == -8
+
+
+def test_average_sigma_rejects_bad_arguments_without_launching():
+    L = _lib.lib()
+    nul, one = C.c_void_p(0), C.c_void_p(16)
+    # stand-in pointers, never dereferenced: each call below fails a check first
+    args = [one, one, one, 3, 16, 9, 9, one, one, one, one, nul]
+    bad = list(args); bad[0] = nul
+    assert L.pbh_average_sigma(*bad) == -1
+    bad = list(args); bad[10] = nul
+    assert L.pbh_average_sigma(*bad) == -1
+    for k, v in ((3, 0), (4, 0), (5, -1), (6, 0), (4, 100000), (6, 5000)):
+        bad = list(args); bad[k] = v
+        assert L.pbh_average_sigma(*bad) == -2, (k, v)
+    bad = list(args); bad[4], bad[5], bad[6] = 4096, 1024, 1024      # nz x spots > 2^31
+    assert L.pbh_average_sigma(*bad) == -5
 
 
 def test_forward_workspace_query():

@@ -98,3 +98,43 @@ def test_other_straggling_models(ref_physics, name):
     _, _, info = physics.build_luts(len(z), float(z[0]), float(z[1] - z[0]), ref_physics[f"{name}_E"],
                                     idd_model=mdl)
     np.testing.assert_allclose(info[:, 1].cpu().numpy(), ref_physics[f"{name}_sigR"], rtol=1e-13)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-6), ("bf16x3", 2e-5)])
+def test_averaged_sigma_matches_reference(ref_physics, precision, tol):
+    """per_layer=False (engines.py:184-191, :243-255): sigma_bar(z) and the dose
+    against the reference's own values on its fixture field (16^3 at 4 mm, 9 x 9
+    spots, 3 layers), then a 2 : 5 pitch field against the oracle's FP64
+    single-sigma dose."""
+    from pbdose import engines, model
+    print(f"case averaged sigma, {precision}", flush=True)
+    g = model.DoseGrid.centered(16, 0.4)
+    p = model.gaussian_impt_field(np.array([100.0, 120.0, 140.0]), g, n_spot=9, spot_spacing=0.5, seed=3)
+    dp = engines.averaged_problem(p, "cuda")
+    np.testing.assert_allclose(dp.sigma.cpu().numpy(), np.tile(ref_physics["avg_sigma_bar"], (16, 1)),
+                               rtol=1e-12, atol=0)
+    np.testing.assert_array_equal(dp.idd.cpu().numpy(), np.eye(16))
+    D = engines.torch_separable(p, "cuda", per_layer=False, precision=precision)
+    assert O.relative_error(D.double().cpu().numpy(), ref_physics["avg_dose"])["max_rel_err_global"] <= tol
+
+    g = model.DoseGrid(64, 48, 96, 0.2, 0.2, 0.2, -6.3, -4.7, 0.0)
+    p = model.gaussian_impt_field(np.array([90.0, 115.0, 140.0, 160.0]), g, n_spot=21, spot_spacing=0.5, seed=7)
+    D = engines.torch_separable(p, "cuda", per_layer=False, precision=precision).double().cpu().numpy()
+    ref = O.separable_dose_averaged(g.x, g.y, p.lattice.spot_x, p.lattice.spot_y, p.lattice.weights,
+                                    p.idd, p.sigma)
+    err = O.relative_error(D, ref)["max_rel_err_global"]
+    print(f"  64 x 48 x 96, 21 x 21 spots, 4 layers: {err:.2e}", flush=True)
+    assert err <= tol
+
+
+def test_averaged_sigma_equals_exact_when_widths_agree():
+    """With every layer's sigma row equal, sigma_bar is that row and the
+    single-sigma dose equals the per-layer-exact dose (FP32 both)."""
+    from pbdose import engines, model
+    print("case averaged sigma = exact for equal widths", flush=True)
+    g = model.DoseGrid(40, 36, 30, 0.2, 0.2, 0.3, -3.9, -3.5, 0.0)
+    p = model.gaussian_impt_field(np.array([80.0, 100.0, 120.0]), g, n_spot=11, spot_spacing=0.5, seed=2)
+    p.sigma = np.tile(p.sigma[1], (3, 1))
+    Da = engines.torch_separable(p, "cuda", per_layer=False).double().cpu().numpy()
+    De = engines.torch_separable(p, "cuda").double().cpu().numpy()
+    assert O.relative_error(Da, De)["max_rel_err_global"] <= 1e-6

@@ -436,6 +436,15 @@ def test_averaged_sigma_dose_matches_reference(ref_physics):
     De = O.separable_dose(x, y, s, s, w, ref_physics["gauss_idd"], same)
     Da = O.separable_dose_averaged(x, y, s, s, w, ref_physics["gauss_idd"], same)
     assert np.abs(De - Da).max() <= 1e-13 * De.max()
+    # ... also off the square centred geometry, where the NumPy engine's index
+    # pairing would not cancel: 20 x 12 voxels, 7 x 5 spots, an offset lattice
+    xr, yr = O.axis(20, 0.3, -2.9), O.axis(12, 0.25, -1.2)
+    sx, sy = O.spot_axis(7, 0.5, 0.4), O.spot_axis(5, 0.45, -0.3)
+    wr = np.random.default_rng(5).uniform(0.2, 1.0, (3, 5, 7))
+    sig = np.broadcast_to(ref_physics["gauss_sigma"][2], ref_physics["gauss_sigma"].shape)
+    De = O.separable_dose(xr, yr, sx, sy, wr, ref_physics["gauss_idd"], sig)
+    Da = O.separable_dose_averaged(xr, yr, sx, sy, wr, ref_physics["gauss_idd"], sig)
+    assert Da.shape == (16, 12, 20) and np.abs(De - Da).max() <= 1e-13 * De.max()
 
 
 # ------------------------------------------------------------- error metrics
