@@ -218,6 +218,15 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
     // table slots: the interleaved span on the P = 2, Q = QC geometry, FT_TAB otherwise
     constexpr int TXN = !TAB ? 1 : QC ? table_slots(2 * (FT_TX - 1) + QC * (KX - 1) + 1) : FT_TAB;
     constexpr int TYN = !TAB ? 1 : QC ? table_slots(2 * (TY - 1) + QC * (KYP - 1) + 1) : FT_TAB;
+    // single-product forms on 64-row tiles: operand-pair tables (build_pairs, no
+    // unpacking at the gathers); otherwise (hi, lo) packed per entry. (Pair planes for
+    // the split forms, hi and lo, were measured 1 us slower: twice the table writes
+    // and LDS for 40 fewer VALU instructions per layer and wave.)
+    constexpr bool PAIRS = TAB && QC > 0 && TY == 64 && !SPLIT;
+    // FP16 split forms on 64-row tiles (two workgroups per CU by LDS, VGPRs to spare):
+    // each stage loads all its operands before its products (fp16x3 36.1 -> 34.9 us on
+    // the 128^3 / 41 x 41 x 40-layer field; the BF16 split form measured no gain)
+    constexpr bool PRE = TAB && QC > 0 && TY == 64 && SPLIT && F16;
     __shared__ unsigned Tx[TAB ? 2 : 1][TXN];
     __shared__ unsigned Ty[TAB ? 2 : 1][TYN];
     __shared__ LayerList ll;
@@ -307,6 +316,9 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                     shx.init(pl.tx, rowx);
                     shy.init(pl.ty, rowy);
                 }
+                // PAIRS: slot of entry threadIdx.x - Q, the pair slot of each thread's entries
+                const int s1x = PAIRS ? table_slot(pl.tx, (int)threadIdx.x - QC) : 0;
+                const int s1y = PAIRS ? table_slot(pl.ty, (int)threadIdx.x - QC) : 0;
                 // spots within reach of each wave's rows (stage A, K = y spots) and columns
                 // (stage B, K = x spots), in spot units of the chunk; read after the next barrier
                 for (int q = threadIdx.x; q < (m1 - m0) * 8; q += FT_THREADS) {
@@ -365,6 +377,10 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                 };
                 auto build_tx = [&](int m) {
                     if (!TAB) return;
+                    if (PAIRS) {
+                        build_pairs<T, QC>(shx, s1x, Tx[m & 1], ll.k2[m]);
+                        return;
+                    }
                     if (QC) {
                         build_packed<T>(shx, Tx[m & 1], ll.k2[m]);
                         return;
@@ -377,6 +393,10 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                 };
                 auto build_ty = [&](int m) {
                     if (!TAB) return;
+                    if (PAIRS) {
+                        build_pairs<T, QC>(shy, s1y, Ty[m & 1], ll.k2[m]);
+                        return;
+                    }
                     if (QC) {
                         build_packed<T>(shy, Ty[m & 1], ll.k2[m]);
                         return;
@@ -401,11 +421,50 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                     v4f u[NCB];
 #pragma unroll
                     for (int k = 0; k < NCB; ++k) u[k] = v4f{};
+                    if (PRE) {
+                        // every operand of the layer's products first: the LDS reads of both
+                        // K steps issue back to back and the products wait once
+                        V8 gh[2], gl[2], wh[2][NCB], wl[2][NCB];
 #pragma unroll
-                    for (int ks = 0; ks < 2; ++ks) {
+                        for (int ks = 0; ks < 2; ++ks) {
+                            if (ks < nkA) {
+                                unsigned t8[8];
+#pragma unroll
+                                for (int e = 0; e < 8; ++e) t8[e] = Ty[b][sw.slot(pl.ty, 32 * ks + e)];
+                                unpack8(t8, gh[ks], gl[ks]);
+                                const int kofs = k0 + 32 * ks + 8 * gA;
+#pragma unroll
+                                for (int k = 0; k < NCB; ++k) {
+                                    const int off = (16 * k + (lane & 15)) * WT_STRIDE + kofs;
+                                    wh[ks][k] = *(const V8*)&Wh[b][off];
+                                    if (SPLIT) wl[ks][k] = *(const V8*)&Wl[b][off];
+                                }
+                            }
+                        }
+#pragma unroll
+                        for (int ks = 0; ks < 2; ++ks) {
+                            if (ks < nkA) {
+#pragma unroll
+                                for (int k = 0; k < NCB; ++k) {
+                                    u[k] = mfma16(wh[ks][k], gh[ks], u[k]);
+                                    if (SPLIT && x3) {
+                                        u[k] = mfma16(wh[ks][k], gl[ks], u[k]);
+                                        u[k] = mfma16(wl[ks][k], gh[ks], u[k]);
+                                    }
+                                }
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int ks = 0; ks < 2 && !PRE; ++ks) {
                         if (ks < nkA) {
                             V8 gh, gl;      // Gy^T operand: j = k0 + 32 ks + 8 gA + e, y = yA
-                            if (TAB) {
+                            if (PAIRS) {
+                                u4v h;
+#pragma unroll
+                                for (int q = 0; q < 4; ++q) h[q] = Ty[b][sw.slot(pl.ty, 32 * ks + 2 * q)];
+                                gh = __builtin_bit_cast(V8, h);
+                            } else if (TAB) {
                                 unsigned t8[8];
 #pragma unroll
                                 for (int e = 0; e < 8; ++e) t8[e] = Ty[b][sw.slot(pl.ty, 32 * ks + e)];
@@ -469,10 +528,49 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                     TabStream<QC> sw = sB;
                     sw.shift(pl.tx, k0);
                     V8 bh[KB], bl[KB];
+                    if (PRE) {
+                        // every operand first (Gx gathers and the c U blocks of both row
+                        // blocks), then the products
+                        V8 ah[KB][NBB], al[KB][NBB];
+#pragma unroll
+                        for (int ks = 0; ks < KB; ++ks) {
+                            if (ks < nkB) {
+                                unsigned t8[8];
+#pragma unroll
+                                for (int e = 0; e < 8; ++e) t8[e] = Tx[b][sw.slot(pl.tx, 16 * ks + e)];
+                                unpack8(t8, bh[ks], bl[ks]);
+#pragma unroll
+                                for (int rb = 0; rb < NBB; ++rb) {
+                                    const int off = (32 * rb + (lane & 31)) * U_STRIDE + k0 + 16 * ks + 8 * gB;
+                                    ah[ks][rb] = *(const V8*)&Uh[b][off];
+                                    if (SPLIT) al[ks][rb] = *(const V8*)&Ul[b][off];
+                                }
+                            }
+                        }
+#pragma unroll
+                        for (int ks = 0; ks < KB; ++ks) {
+                            if (ks < nkB) {
+#pragma unroll
+                                for (int rb = 0; rb < NBB; ++rb) {
+                                    acc[rb] = mfma32(ah[ks][rb], bh[ks], acc[rb]);
+                                    if (SPLIT && x3) {
+                                        acc[rb] = mfma32(ah[ks][rb], bl[ks], acc[rb]);
+                                        acc[rb] = mfma32(al[ks][rb], bh[ks], acc[rb]);
+                                    }
+                                }
+                            }
+                        }
+                        return;
+                    }
 #pragma unroll
                     for (int ks = 0; ks < KB; ++ks) {
                         if (ks >= nkB) break;
-                        if (TAB) {
+                        if (PAIRS) {
+                            u4v h;
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) h[q] = Tx[b][sw.slot(pl.tx, 16 * ks + 2 * q)];
+                            bh[ks] = __builtin_bit_cast(V8, h);
+                        } else if (TAB) {
                             unsigned t8[8];
 #pragma unroll
                             for (int e = 0; e < 8; ++e) t8[e] = Tx[b][sw.slot(pl.tx, 16 * ks + e)];

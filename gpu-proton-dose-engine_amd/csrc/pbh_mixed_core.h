@@ -115,6 +115,35 @@ __device__ __forceinline__ void build_packed(const TabShare<NT>& sh, unsigned* t
     }
 }
 
+// Operand-pair tables (single-product forms): entry t of a thread's share is
+// written as the first half of slot(t) and the second half of slot(t - Q), so that
+// slot(t) holds (hi(t), hi(t + Q)). A lane's eight operand entries t0 + Q e (e < 8)
+// are then the four 32-bit slots of t0 + 2 Q q, read straight into the MFMA operand
+// with no unpacking. s1 = slot of entry threadIdx.x - Q (negative for the first Q
+// threads: their first entry has no pair slot). Entry values equal build_packed's
+// hi halves.
+template <typename T, int QC, int NT>
+__device__ __forceinline__ void build_pairs(const TabShare<NT>& sh, int s1, unsigned* A, float k2)
+{
+    typedef T V2 __attribute__((ext_vector_type(2)));
+    typedef float F2 __attribute__((ext_vector_type(2)));
+    const float nk2 = -k2;
+    unsigned short* a16 = (unsigned short*)A;
+    auto put = [&](int k, T h) {
+        if (!TabShare<NT>::ok(k)) return;
+        a16[2 * (sh.s0 + FT_THREADS / 2 * k)] = bits16(h);
+        if (k > 0 || (int)threadIdx.x >= QC) a16[2 * (s1 + FT_THREADS / 2 * k) + 1] = bits16(h);
+    };
+#pragma unroll
+    for (int k = 0; k < TabShare<NT>::N; k += 2) {
+        const float x0 = __builtin_amdgcn_exp2f(sh.d2[k] * nk2);
+        const float x1 = k + 1 < TabShare<NT>::N ? __builtin_amdgcn_exp2f(sh.d2[k + 1] * nk2) : 0.0f;
+        const V2 h = __builtin_convertvector((F2){x0, x1}, V2);
+        put(k, h[0]);
+        if (k + 1 < TabShare<NT>::N) put(k + 1, h[1]);
+    }
+}
+
 // Eight packed entries -> hi and lo operand vectors.
 template <typename V8>
 __device__ __forceinline__ void unpack8(const unsigned* t, V8& hi, V8& lo)
