@@ -199,6 +199,29 @@ def time_adjoint(dp, D, steps, warmup, precision="fp32"):
     return wall, call_events(call, steps), g
 
 
+def time_planning_step(dp, steps, warmup, fwd="bf16x3", adj="bf16x3"):
+    """One inverse-planning step per call (pbh_forward_objective_adjoint: forward dose,
+    J and dJ/dw, with the objective formed in the forward's combine): wall ms per call
+    over `steps` back-to-back calls, and dJ/dw of the last call."""
+    from pbdose import planning
+    Dp, wv = planning.central_cube(dp.problem.grid, device=dp.dev)
+    out = torch.empty(dp.shape, dtype=torch.float32, device=dp.dev)
+    g = None
+
+    def call():
+        nonlocal g
+        _, _, g = planning.forward_objective_gradient(dp, Dp, wv, precision=adj, forward_precision=fwd, out=out)
+
+    for _ in range(warmup):
+        call()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        call()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps, g
+
+
 def time_robust(precision, steps, objective):
     """Robust-scenario batch (3 fields x 21 scenarios on 128^3), one graph replay
     per step; launch parameters from the measured autotuner."""
@@ -368,6 +391,9 @@ def main():
         adj_err = float((gmix.double() - g32.double()).abs().max() / g32.double().abs().max())
     else:
         adj_ms, adj_ev, adj_err = adj32_ms, adj32_ev, 0.0
+    # forward + objective + adjoint as one planning-step call, for the headline form and FP32
+    step_ms, _ = time_planning_step(dp, max(5, args.steps // 4), 3, args.precision, aform)
+    step32_ms, _ = time_planning_step(dp, max(5, args.steps // 4), 3, "fp32", "fp32")
     pcie_ms = time_torch_separable(prob, args.precision, max(5, args.steps // 10))
     bt = base_type(args.precision)
     peak = PEAK[bt]
@@ -404,6 +430,10 @@ def main():
         "fwd_adj_ms": ms_step + adj_ms,
         "fwd_adj_ms_fp32": ms32 + adj32_ms,
         "fwd_adj_speedup_vs_fp32": (ms32 + adj32_ms) / (ms_step + adj_ms),
+        "planning_step": {"call": "pbh_forward_objective_adjoint (forward, J and dJ/dw in one call; the "
+                                  "objective formed in the forward's layer-chunk combine)",
+                          "forms": f"{args.precision} forward, {aform} adjoint",
+                          "ms": step_ms, "ms_fp32": step32_ms, "speedup_vs_fp32": step32_ms / step_ms},
         "torch_separable_ms_incl_host_copies": pcie_ms,
         "roofline": {"bound": "mfma",
                      "kernel": KERNEL_NAME.get(args.precision, "fwd_mixed_kernel"),

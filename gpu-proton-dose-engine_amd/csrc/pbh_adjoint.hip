@@ -422,10 +422,8 @@ __device__ __forceinline__ void objective_body(const float* D, const float* Dp, 
     __shared__ double red[OBJ_THREADS / 64];
     double acc = 0.0;
     auto one = [&](float d, float p, float wf, float* r_out) {
-        const double r = (double)d - (double)p;
-        const double w = (double)wf;
-        acc += w * r * r;
-        if (r_out) *r_out = (float)(2.0 * w * r);
+        const float r = objective_term(d, p, wf, acc);
+        if (r_out) *r_out = r;
     };
     const bool vec = (((uintptr_t)D | (uintptr_t)Dp | (uintptr_t)wv | (uintptr_t)R) & 15) == 0;
     size_t done = 0;
@@ -759,6 +757,55 @@ extern "C" int pbh_objective_adjoint(const PbhGrid* g, const PbhLattice* lat, co
     hipLaunchKernelGGL(objective_kernel, dim3(OBJ_BLOCKS), dim3(OBJ_THREADS), 0, s, dose, presc, vox_w, nv, R, jpart);
     if ((rc = launch_rc("objective_kernel")) != PBH_OK) return rc;
     return adjoint_launch(g, lat, A, idd, grad_w, nullptr, precision, jpart, OBJ_BLOCKS, j_out, s);
+}
+
+// Workspace of pbh_forward_objective_adjoint: the forward's, then the objective +
+// adjoint's, then the J partial sums of the fused combine (FT_OBJ_PARTS per dose tile).
+extern "C" size_t pbh_forward_objective_adjoint_workspace_bytes(const PbhGrid* g, const PbhLattice* lat,
+                                                               int fwd_precision, int layer_chunk, int tile_rows,
+                                                               int adj_precision)
+{
+    const size_t f = pbh_forward_workspace_bytes_shaped(g, lat, fwd_precision, layer_chunk, tile_rows);
+    const size_t o = pbh_objective_adjoint_workspace_bytes(g, lat, adj_precision);
+    if (f == 0 || o == 0) return 0;
+    const long long rows = tile_rows ? tile_rows : FT_TY;
+    const long long tiles = ((g->nx + FT_TX - 1) / FT_TX) * ((g->ny + rows - 1) / rows) * (long long)g->nz;
+    return align_up(f) + align_up(o) + (size_t)tiles * FT_OBJ_PARTS * sizeof(double);
+}
+
+extern "C" int pbh_forward_objective_adjoint(const PbhGrid* g, const PbhLattice* lat, const float* w,
+                                             const double* idd, const double* sigma, float* dose,
+                                             const float* presc, const float* vox_w, float* grad_w,
+                                             double* j_out, int fwd_precision, int layer_chunk, int tile_rows,
+                                             int adj_precision, int split, void* workspace,
+                                             size_t workspace_bytes, void* stream)
+{
+    int rc = pbh_validate(g, lat);
+    if (rc != PBH_OK) return rc;
+    if (!w || !idd || !sigma || !dose || !presc || !grad_w || !j_out || !workspace) return PBH_ERR_NULL_ARG;
+    const bool mixed = adj_precision == PBH_PREC_BF16 || adj_precision == PBH_PREC_BF16X3;
+    if (adj_precision != PBH_PREC_FP32 && !mixed) return PBH_ERR_UNSUPPORTED;
+    if (split < 0 || layer_chunk < 0) return PBH_ERR_BAD_DIM;
+    const size_t need = pbh_forward_objective_adjoint_workspace_bytes(g, lat, fwd_precision, layer_chunk,
+                                                                      tile_rows, adj_precision);
+    if (need == 0) return PBH_ERR_UNSUPPORTED;
+    if (workspace_bytes < need) return PBH_ERR_WORKSPACE;
+    hipStream_t s = (hipStream_t)stream;
+    char* ws = (char*)workspace;
+    const size_t fbytes = pbh_forward_workspace_bytes_shaped(g, lat, fwd_precision, layer_chunk, tile_rows);
+    char* ows = ws + align_up(fbytes);
+    float* part = (float*)ows;
+    float* R = (float*)(ows + align_up(pbh_adjoint_workspace_bytes(g, lat, adj_precision)));
+    double* jpart = (double*)(ows + align_up(pbh_objective_adjoint_workspace_bytes(g, lat, adj_precision)));
+    AdjArgs A;
+    if ((rc = prepare_adjoint(g, lat, w, idd, sigma, R, part, nullptr, nullptr, adj_precision, split, &A)) != PBH_OK)
+        return rc;
+    const ObjFuse fuse{presc, vox_w, R, jpart};
+    int njpart = 0;
+    if ((rc = pbh_forward_fused(g, lat, w, idd, sigma, dose, fwd_precision, layer_chunk, tile_rows, ws, fbytes, s,
+                                &fuse, &njpart)) != PBH_OK)
+        return rc;
+    return adjoint_launch(g, lat, A, idd, grad_w, nullptr, adj_precision, jpart, njpart, j_out, s);
 }
 
 extern "C" int pbh_lut_chain(const double* grad_idd, const double* grad_sigma, const double* didd,

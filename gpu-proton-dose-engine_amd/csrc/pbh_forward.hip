@@ -268,6 +268,11 @@ __global__ __launch_bounds__(256) void combine_kernel(FwdArgs a, FwdPlan pl)
     combine_body(a, pl, blockIdx.x);
 }
 
+__global__ __launch_bounds__(256) void combine_objective_kernel(FwdArgs a, FwdPlan pl, ObjFuse f)
+{
+    combine_objective_body(a, pl, f, blockIdx.x / FT_OBJ_PARTS, blockIdx.x % FT_OBJ_PARTS);
+}
+
 __global__ __launch_bounds__(256) void combine_batch_kernel(const BatchFwdItem* __restrict__ items,
                                                             const int* __restrict__ wg0, int n)
 {
@@ -504,8 +509,13 @@ int pbh_plan_forward(const FwdArgs& a, int kx_pad, int ky_pad, void* ws, size_t 
     return PBH_OK;
 }
 
-int pbh_launch_combine(const FwdArgs& a, const FwdPlan& pl, hipStream_t s)
+int pbh_launch_combine(const FwdArgs& a, const FwdPlan& pl, hipStream_t s, const ObjFuse* fuse)
 {
+    if (fuse) {
+        hipLaunchKernelGGL(combine_objective_kernel, dim3((unsigned)(tile_count(a) * FT_OBJ_PARTS)), dim3(256), 0, s,
+                           a, pl, *fuse);
+        return launch_rc("combine_objective_kernel");
+    }
     if (pl.nchunk <= 1) return PBH_OK;
     hipLaunchKernelGGL(combine_kernel, dim3((unsigned)tile_count(a)), dim3(256), 0, s, a, pl);
     return launch_rc("combine_kernel");
@@ -518,7 +528,8 @@ int pbh_launch_combine_batch(const BatchFwdItem* items, const int* wg0, int n, i
 }
 
 int pbh_forward_mixed(const FwdArgs& a, double oz, double dz, int precision, int layer_chunk,
-                      void* ws, size_t ws_bytes, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1);
+                      void* ws, size_t ws_bytes, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1,
+                      const ObjFuse* fuse);
 
 extern "C" size_t pbh_forward_workspace_bytes_tuned(const PbhGrid* g, const PbhLattice* lat, int precision,
                                                    int layer_chunk)
@@ -618,7 +629,8 @@ static int tile_rows_of(int precision, int tile_rows)
 // stream right before and right after the contraction kernel.
 static int forward_impl(const PbhGrid* g, const PbhLattice* lat, const float* w, const double* idd,
                         const double* sigma, float* dose, int precision, int layer_chunk, int tile_rows,
-                        void* workspace, size_t workspace_bytes, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
+                        void* workspace, size_t workspace_bytes, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1,
+                        const ObjFuse* fuse = nullptr)
 {
     int rc = validate_problem(g, lat);
     if (rc != PBH_OK) return rc;
@@ -629,7 +641,8 @@ static int forward_impl(const PbhGrid* g, const PbhLattice* lat, const float* w,
     if (rows < 0) return rows;
     const FwdArgs a = make_fwd_args(g, lat, w, idd, sigma, dose, rows);
     if (precision != PBH_PREC_FP32)
-        return pbh_forward_mixed(a, g->oz, g->dz, precision, layer_chunk, workspace, workspace_bytes, s, ev0, ev1);
+        return pbh_forward_mixed(a, g->oz, g->dz, precision, layer_chunk, workspace, workspace_bytes, s, ev0, ev1,
+                                 fuse);
     FwdPlan pl;
     rc = pbh_plan_forward(a, chunk_pad(a.nsx, a.nsy), chunk_pad(a.nsx, a.nsy), workspace, workspace_bytes,
                           layer_chunk, &pl);
@@ -654,7 +667,20 @@ static int forward_impl(const PbhGrid* g, const PbhLattice* lat, const float* w,
     }
     if ((rc = launch_rc("fwd_fp32_kernel")) != PBH_OK) return rc;
     if (ev1 && (rc = hip_rc(hipEventRecord(ev1, s), "hipEventRecord")) != PBH_OK) return rc;
-    return pbh_launch_combine(a, pl, s);
+    return pbh_launch_combine(a, pl, s, fuse);
+}
+
+int pbh_forward_fused(const PbhGrid* g, const PbhLattice* lat, const float* w, const double* idd,
+                      const double* sigma, float* dose, int precision, int layer_chunk, int tile_rows,
+                      void* workspace, size_t workspace_bytes, hipStream_t s, const ObjFuse* fuse, int* njpart)
+{
+    if (!fuse || !fuse->presc || !fuse->R || !fuse->jpart || !njpart) return PBH_ERR_NULL_ARG;
+    const int rows = tile_rows_of(precision, tile_rows);
+    if (rows < 0) return rows;
+    if (validate_problem(g, lat) != PBH_OK) return PBH_ERR_BAD_DIM;
+    *njpart = (int)tile_count(make_fwd_args(g, lat, nullptr, nullptr, nullptr, nullptr, rows)) * FT_OBJ_PARTS;
+    return forward_impl(g, lat, w, idd, sigma, dose, precision, layer_chunk, tile_rows, workspace, workspace_bytes,
+                        s, nullptr, nullptr, fuse);
 }
 
 extern "C" int pbh_forward_tuned(const PbhGrid* g, const PbhLattice* lat, const float* w,

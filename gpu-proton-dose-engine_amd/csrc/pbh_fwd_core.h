@@ -232,6 +232,98 @@ __device__ inline void combine_body(const FwdArgs& a, const FwdPlan& pl, int til
     }
 }
 
+// One voxel of the dose objective J = sum wv (D - Dp)^2 (FP64 terms): adds its term
+// to acc and returns its residual dJ/dD = 2 wv (D - Dp) as FP32. Shared by the
+// objective kernel and the combine that forms the objective of a planning step, so
+// that both give the same residual bit for bit.
+__device__ __forceinline__ float objective_term(float d, float p, float wf, double& acc)
+{
+    const double r = (double)d - (double)p;
+    const double w = (double)wf;
+    acc += w * r * r;
+    return (float)(2.0 * w * r);
+}
+
+// The objective fused into the layer-chunk combine (one planning step): R and the
+// partial sums of J, one per combine workgroup (FT_OBJ_PARTS per dose tile).
+struct ObjFuse {
+    const float* presc;    // Dp (nz, ny, nx)
+    const float* vox_w;    // wv (nz, ny, nx), or null: all 1
+    float* R;              // out: 2 wv (D - Dp)
+    double* jpart;         // out: tiles * FT_OBJ_PARTS partial sums of J
+};
+constexpr int FT_OBJ_PARTS = 4;    // row bands of a dose tile, one combine workgroup each
+
+// Row band `band` of dose tile `tile`: the dose (plus the partial tiles of chunks
+// 1 .. nchunk(z) - 1 in chunk order, written back when there are any), then R and this
+// workgroup's partial sum of J over the band's voxels inside the grid.
+__device__ inline void combine_objective_body(const FwdArgs& a, const FwdPlan& pl, const ObjFuse& f, int tile,
+                                              int band)
+{
+    __shared__ int cnt[4];
+    __shared__ double red[4];
+    const size_t tiles = (size_t)a.tiles_x * a.tiles_y * a.nz;
+    const int tx = tile % a.tiles_x;
+    const int ty = (tile / a.tiles_x) % a.tiles_y;
+    const int z = tile / (a.tiles_x * a.tiles_y);
+    const int nact = pl.nlayers ? pl.nlayers[z] : count_active(a, z, cnt);
+    const int nch = nact > pl.chunk ? (nact + pl.chunk - 1) / pl.chunk : 1;
+    const int TR = a.tile_rows, BR = TR / FT_OBJ_PARTS;
+    const float* part0 = nch > 1 ? pl.part + (size_t)tile * ((size_t)TR * FT_TX) : nullptr;
+    const size_t cstride = tiles * ((size_t)TR * FT_TX);
+    double acc = 0.0;
+    const bool vec = (a.nx & 3) == 0 && tx * FT_TX + FT_TX <= a.nx &&
+                     (((uintptr_t)a.dose | (uintptr_t)f.presc | (uintptr_t)f.vox_w | (uintptr_t)f.R) & 15) == 0;
+    if (vec) {
+        // whole 128-column rows inside the grid: four consecutive voxels per thread
+        for (int e = BR * band * FT_TX + 4 * threadIdx.x; e < BR * (band + 1) * FT_TX; e += 4 * blockDim.x) {
+            const int y = ty * TR + e / FT_TX, x = tx * FT_TX + e % FT_TX;
+            if (y >= a.ny) break;
+            const size_t i = ((size_t)z * a.ny + y) * a.nx + x;
+            float4* d = (float4*)(a.dose + i);
+            float4 v = *d;
+            if (nch > 1) {
+                for (int c = 1; c < nch; ++c) {
+                    const float4 p = *(const float4*)(part0 + (size_t)(c - 1) * cstride + e);
+                    v.x += p.x;
+                    v.y += p.y;
+                    v.z += p.z;
+                    v.w += p.w;
+                }
+                *d = v;
+            }
+            const float4 p = *(const float4*)(f.presc + i);
+            const float4 w = f.vox_w ? *(const float4*)(f.vox_w + i) : make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+            float4 r;
+            r.x = objective_term(v.x, p.x, w.x, acc);
+            r.y = objective_term(v.y, p.y, w.y, acc);
+            r.z = objective_term(v.z, p.z, w.z, acc);
+            r.w = objective_term(v.w, p.w, w.w, acc);
+            *(float4*)(f.R + i) = r;
+        }
+    } else {
+        for (int e = BR * band * FT_TX + threadIdx.x; e < BR * (band + 1) * FT_TX; e += blockDim.x) {
+            const int y = ty * TR + e / FT_TX, x = tx * FT_TX + e % FT_TX;
+            if (y >= a.ny || x >= a.nx) continue;
+            const size_t i = ((size_t)z * a.ny + y) * a.nx + x;
+            float v = a.dose[i];
+            if (nch > 1) {
+                for (int c = 1; c < nch; ++c) v += part0[(size_t)(c - 1) * cstride + e];
+                a.dose[i] = v;
+            }
+            f.R[i] = objective_term(v, f.presc[i], f.vox_w ? f.vox_w[i] : 1.0f, acc);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+        f.jpart[(size_t)tile * FT_OBJ_PARTS + band] = t;
+    }
+}
+
 constexpr int FT_LB = 64;    // layers gathered per pass
 
 // Window of the contraction: a Gaussian factor exp(-d^2 / (2 s^2)) below
@@ -394,5 +486,14 @@ int pbh_plan_forward(const pbh::FwdArgs& a, int kx_pad, int ky_pad, void* ws, si
                      int layer_chunk, pbh::FwdPlan* plan);
 size_t pbh_forward_ws_bytes(const pbh::FwdArgs& a, int layer_chunk);
 int pbh_auto_chunk(const pbh::FwdArgs& a);
-int pbh_launch_combine(const pbh::FwdArgs& a, const pbh::FwdPlan& pl, hipStream_t s);
+// The layer-chunk combine of a forward call; with `fuse`, the combine over every dose
+// tile that also forms the objective (combine_objective_body).
+int pbh_launch_combine(const pbh::FwdArgs& a, const pbh::FwdPlan& pl, hipStream_t s,
+                       const pbh::ObjFuse* fuse = nullptr);
+// A forward call (pbh_forward_shaped) whose combine forms the objective (`fuse`, not
+// null); *njpart receives the number of J partial sums written.
+int pbh_forward_fused(const PbhGrid* g, const PbhLattice* lat, const float* w, const double* idd,
+                      const double* sigma, float* dose, int precision, int layer_chunk, int tile_rows,
+                      void* workspace, size_t workspace_bytes, hipStream_t s, const pbh::ObjFuse* fuse,
+                      int* njpart);
 int pbh_device_cus();

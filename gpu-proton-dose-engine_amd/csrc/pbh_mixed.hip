@@ -821,7 +821,7 @@ using namespace pbh;
 float* pbh_forward_ws_peaks(const FwdArgs& a, void* ws);
 
 int pbh_forward_mixed(const FwdArgs& a, double oz, double dz, int precision, int layer_chunk, void* ws,
-                      size_t ws_bytes, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
+                      size_t ws_bytes, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, const ObjFuse* fuse)
 {
     const int kb = (chunk_pad(a.nsx, 1) + 15) / 16;   // stage B K steps of 16 spots
     if (!ws) return PBH_ERR_NULL_ARG;
@@ -845,7 +845,7 @@ int pbh_forward_mixed(const FwdArgs& a, double oz, double dz, int precision, int
     if ((rc = dispatch_precision(precision, mixed_key(pl, kb), SingleLaunch{a, pl, oz, dz, r_peak, s})) != PBH_OK)
         return rc;
     if (ev1 && (rc = hip_rc(hipEventRecord(ev1, s), "hipEventRecord")) != PBH_OK) return rc;
-    return pbh_launch_combine(a, pl, s);
+    return pbh_launch_combine(a, pl, s, fuse);
 }
 
 int pbh_launch_peak_batch(const BatchPeakItem* items, int n, hipStream_t s)

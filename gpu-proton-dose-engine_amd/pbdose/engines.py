@@ -181,11 +181,17 @@ class DeviceProblem:
             self._chunk[key] = int(c)
         return self._chunk[key]
 
-    def workspace(self, kind: str, precision: int, layer_chunk: int = 0, tile_rows: int = 0) -> torch.Tensor:
-        key = (kind, precision, layer_chunk, tile_rows)
+    def workspace(self, kind: str, precision: int, layer_chunk: int = 0, tile_rows: int = 0,
+                  adj_precision: int = 0) -> torch.Tensor:
+        key = (kind, precision, layer_chunk, tile_rows, adj_precision)
         if key not in self._ws:
             L = _lib.lib()
-            if kind == "fwd":
+            if kind == "fwdobjadj":
+                n = L.pbh_forward_objective_adjoint_workspace_bytes(self.grid_s, self.lat_s, precision, layer_chunk,
+                                                                    tile_rows, adj_precision)
+                if n == 0:
+                    raise ValueError("no planning-step workspace for these forms and shape")
+            elif kind == "fwd":
                 n = L.pbh_forward_workspace_bytes_shaped(self.grid_s, self.lat_s, precision, layer_chunk,
                                                          tile_rows)
             elif kind == "obj":
@@ -196,6 +202,14 @@ class DeviceProblem:
                 n = L.pbh_adjoint_workspace_bytes(self.grid_s, self.lat_s, precision)
             self._ws[key] = _lib.workspace(n, self.dev)
         return self._ws[key]
+
+    def shape_of(self, p: int, layer_chunk: int = 0, tile_rows: int = 0) -> tuple:
+        """The (layer chunk, tile rows) a forward of form p runs with (as ``forward``)."""
+        if int(layer_chunk) == 0 and int(tile_rows) == 0:
+            return self._tuned.get(p) or self.auto_shape(p)
+        if int(layer_chunk) == 0:
+            return self.auto_chunk(p), int(tile_rows)
+        return int(layer_chunk), int(tile_rows)
 
     def forward(self, precision: str = "fp32", out: Optional[torch.Tensor] = None,
                 w: Optional[torch.Tensor] = None, layer_chunk: int = 0, kernel_events=None,
@@ -218,10 +232,7 @@ class DeviceProblem:
         wt = self.w if w is None else w
         if wt.dtype != torch.float32 or tuple(wt.shape) != tuple(self.w.shape) or not wt.is_contiguous():
             raise ValueError("weights must be a contiguous float32 (L, nsy, nsx) tensor")
-        if int(layer_chunk) == 0 and int(tile_rows) == 0:
-            layer_chunk, tile_rows = self._tuned.get(p) or self.auto_shape(p)
-        elif int(layer_chunk) == 0:
-            layer_chunk = self.auto_chunk(p)
+        layer_chunk, tile_rows = self.shape_of(p, layer_chunk, tile_rows)
         ws = self.workspace("fwd", p, int(layer_chunk), int(tile_rows))
         if kernel_events is not None:
             e0, e1 = kernel_events

@@ -173,6 +173,41 @@ def objective_gradient(dp: DeviceProblem, dose: torch.Tensor, presc: torch.Tenso
     return j[0], gw
 
 
+def forward_objective_gradient(dp: DeviceProblem, presc: torch.Tensor, vox_w: Optional[torch.Tensor] = None,
+                               w: Optional[torch.Tensor] = None, precision: str = "bf16x3",
+                               forward_precision: Optional[str] = None, out: Optional[torch.Tensor] = None,
+                               split: int = 0, layer_chunk: int = 0, tile_rows: int = 0):
+    """One inverse-planning step in one call (pbh_forward_objective_adjoint): the dose
+    of w, J = sum wv (D - Dp)^2 and dJ/dw. Returns (D, J, dJ/dw); D and dJ/dw equal
+    ``dp.forward`` followed by ``objective_gradient`` bit for bit (J up to the grouping
+    of its FP64 sum), with the objective formed inside the forward's combine.
+    ``precision`` is the adjoint's ('fp32', 'bf16', 'bf16x3'); the forward's defaults
+    to it ('fp32' forward for an 'fp32' adjoint)."""
+    if precision not in ("fp32", "bf16", "bf16x3"):
+        raise ValueError(f"adjoint precision {precision!r}: 'fp32', 'bf16' or 'bf16x3'")
+    fwd = forward_precision or precision
+    for name, t in (("presc", presc), ("vox_w", vox_w), ("out", out)):
+        if t is not None and (t.dtype != torch.float32 or tuple(t.shape) != tuple(dp.shape) or
+                              not t.is_contiguous()):
+            raise ValueError(f"{name} must be a contiguous float32 tensor with the dose shape")
+    L = _lib.lib()
+    pa, pf = _lib.PREC[precision], _lib.PREC[fwd]
+    chunk, rows = dp.shape_of(pf, layer_chunk, tile_rows)
+    wt = dp.w if w is None else w.contiguous()
+    D = torch.empty(dp.shape, dtype=torch.float32, device=dp.dev) if out is None else out
+    gw = torch.empty_like(dp.w)
+    j = torch.empty(1, dtype=torch.float64, device=dp.dev)
+    if int(split) == 0:
+        split = dp.auto_split(pa)
+    ws = dp.workspace("fwdobjadj", pf, int(chunk), int(rows), pa)
+    _lib.check(L.pbh_forward_objective_adjoint(dp.grid_s, dp.lat_s, _lib.ptr(wt), _lib.ptr(dp.idd),
+                                               _lib.ptr(dp.sigma), _lib.ptr(D), _lib.ptr(presc), _lib.ptr(vox_w),
+                                               _lib.ptr(gw), _lib.ptr(j), pf, int(chunk), int(rows), pa,
+                                               int(split), _lib.ptr(ws), ws.numel(), _lib.stream_of(dp.dev)),
+               "pbh_forward_objective_adjoint")
+    return D, j[0], gw
+
+
 def lut_chain(grad_idd, grad_sigma, didd, dsigma) -> torch.Tensor:
     """Per-layer sum over depth of grad_idd * didd + grad_sigma * dsigma (FP64)."""
     Lr, nz = grad_idd.shape

@@ -344,6 +344,26 @@ int pbh_objective_adjoint(const PbhGrid* g, const PbhLattice* lat,
 /* out[l] = sum_z (grad_idd[l,z] didd[l,z] + grad_sigma[l,z] dsigma[l,z]) for
  * (L, nz) FP64 arrays: the chain rule from the look-up tables to a per-layer
  * parameter (e.g. a range shift, with didd/dsigma the tables' derivatives). */
+/* One step of inverse planning in one call: the forward dose of w
+ * (pbh_forward_shaped with fwd_precision, layer_chunk, tile_rows) into dose, J =
+ * sum vox_w (dose - presc)^2 (FP64) into *j_out and dJ/dw (adj_precision FP32, BF16
+ * or BF16X3, layer split as pbh_adjoint_tuned) into grad_w. The layer-chunk combine
+ * of the forward runs over every dose tile and also forms R = dJ/dD and J's partial
+ * sums, so no launch re-reads the dose: the objective kernel of
+ * pbh_objective_adjoint is not needed. dose and grad_w equal pbh_forward_shaped +
+ * pbh_objective_adjoint's bit for bit; J differs by the grouping of its FP64 sum.
+ * Workspace: pbh_forward_objective_adjoint_workspace_bytes. No counterpart in the
+ * reference (its engines are forward-only). */
+size_t pbh_forward_objective_adjoint_workspace_bytes(const PbhGrid* g, const PbhLattice* lat,
+                                                     int fwd_precision, int layer_chunk, int tile_rows,
+                                                     int adj_precision);
+int pbh_forward_objective_adjoint(const PbhGrid* g, const PbhLattice* lat, const float* w,
+                                  const double* idd, const double* sigma, float* dose,
+                                  const float* presc, const float* vox_w, float* grad_w,
+                                  double* j_out, int fwd_precision, int layer_chunk, int tile_rows,
+                                  int adj_precision, int split, void* workspace,
+                                  size_t workspace_bytes, void* stream);
+
 int pbh_lut_chain(const double* grad_idd, const double* grad_sigma, const double* didd,
                   const double* dsigma, int n_layers, int nz, double* out, void* stream);
 

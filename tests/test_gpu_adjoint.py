@@ -317,6 +317,49 @@ def test_objective_gradient_equals_objective_then_adjoint(precision):
             assert torch.equal(g1, g0)
 
 
+@pytest.mark.parametrize("fwd,adj,rows", [("fp32", "fp32", 32), ("bf16x3", "bf16x3", 32), ("bf16x3", "bf16x3", 64),
+                                           ("fp16x3", "bf16x3", 64), ("bf16", "bf16", 64)])
+def test_planning_step_in_one_call(fwd, adj, rows):
+    """pbh_forward_objective_adjoint (the objective formed in the forward's combine over
+    every tile) against pbh_forward_shaped + pbh_objective_adjoint: the dose and
+    dJ/dw bit for bit, J to 1e-12 (its FP64 sum is grouped by tile bands instead of by
+    grid stride). Layer chunks of 2 (partial tiles on every slice with more than two
+    active layers) and of all layers, with and without voxel weights, a grid whose x
+    and y are not multiples of the tile (scalar edge path) and one that is, twice in a
+    row on the same workspace."""
+    from pbdose import engines, planning
+    for nx, ny in ((100, 76), (128, 64)):
+        print(f"case planning step {fwd}/{adj} {rows} rows {nx}x{ny}", flush=True)
+        p = small_problem(nx=nx, ny=ny, nz=30, d=0.2, nsx=15, nsy=13, ds=0.5,
+                          energies=tuple(np.linspace(75.0, 125.0, 7)))
+        dp = engines.DeviceProblem(p)
+        D = dp.forward()
+        Dp, wv = planning.central_cube(p.grid, half=1.5, dose_value=float(D.max()) * 0.5)
+        for chunk in (2, 40):
+            D0 = dp.forward(fwd, layer_chunk=chunk, tile_rows=rows).clone()
+            for vw in (wv, None):
+                J0, g0 = planning.objective_gradient(dp, D0, Dp, vw, precision=adj)
+                for _ in range(2):
+                    D1, J1, g1 = planning.forward_objective_gradient(dp, Dp, vw, precision=adj, forward_precision=fwd,
+                                                                     layer_chunk=chunk, tile_rows=rows)
+                    assert torch.equal(D1, D0)
+                    assert torch.equal(g1, g0)
+                    assert abs(float(J1) / float(J0) - 1.0) <= 1e-12
+
+
+def test_planning_step_rejects_bad_arguments():
+    from pbdose import engines, planning
+    p = small_problem()
+    dp = engines.DeviceProblem(p)
+    Dp, wv = planning.central_cube(p.grid)
+    with pytest.raises(ValueError):
+        planning.forward_objective_gradient(dp, Dp, wv, precision="fp16x3")
+    with pytest.raises(ValueError):
+        planning.forward_objective_gradient(dp, Dp[:1].contiguous(), wv)
+    with pytest.raises(RuntimeError):       # 64-row tiles need the 2 : 5 pitch geometry
+        planning.forward_objective_gradient(dp, Dp, wv, precision="bf16x3", tile_rows=64)
+
+
 @pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
 @pytest.mark.parametrize("shape", [((1, 1, 1), (1, 1), (120.0,)), ((48, 40, 1), (9, 7), (90.0, 120.0, 150.0)),
                                    ((1, 40, 6), (5, 9), (100.0, 130.0)), ((40, 1, 6), (9, 5), (100.0, 130.0))])
