@@ -233,6 +233,33 @@ def test_tile_height_and_persistent_launch_change_no_bit(precision):
             b.close()
 
 
+@pytest.mark.skip(reason="the objective formed at the batch forward's tile store was taken out: a GPU run "
+                         "of the bf16x3 / 64-row case wrote nothing for 180 s and was stopped, and no cause "
+                         "was found by reading the code; batches keep the objective kernel")
+@pytest.mark.parametrize("precision,rows", [("fp32", 32), ("bf16x3", 64), ("bf16x3", 32)])
+def test_objective_at_the_tile_store(precision, rows):
+    """Items forwarded in one layer chunk would form R and J's partial sums where the
+    forward stores its dose tiles: no objective launch, J summed in the depth
+    reduction's launch. Every item against the FP64 oracle and the single-problem path
+    (which runs the objective kernel), with and without a persistent launch, on two runs
+    in a row."""
+    from pbdose import robust
+    print(f"case objective at the tile store {precision} {rows} rows", flush=True)
+    fields, scen = fields_with_25_pitch(), small_scenarios()
+    for pers in (0, 1):
+        t = {**robust.DEFAULT_TUNING, "fwd_chunk": 4096, "tile_rows": rows, "persistent": pers}
+        b = robust.RobustBatch(fields, scen, precision=precision, tuning=t)
+        b0 = robust.RobustBatch(fields, scen, precision=precision, tuning={**t, "fwd_chunk": 1})
+        # against one layer per chunk: no combine, objective and J-sum launches
+        assert b.launch_count == b0.launch_count - 3, (b.launch_count, b0.launch_count)
+        b0.close()
+        for _ in range(2):
+            b.run_eager()
+        torch.cuda.synchronize()
+        check_items(b, fields, scen, precision)
+        b.close()
+
+
 def test_autotune_picks_a_measured_candidate(tmp_path, monkeypatch):
     from pbdose import robust
     monkeypatch.setenv("PBDOSE_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
