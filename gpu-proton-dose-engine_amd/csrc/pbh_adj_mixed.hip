@@ -52,7 +52,8 @@ __device__ __forceinline__ void adj_mixed_body(const AdjArgs& A, int blk)
     constexpr int TXN = AmTab<QC, AM_TX, AM_SC>::N, TYN = AmTab<QC, AM_TY, AM_SC>::N;
     const FwdArgs& a = A.f;
 
-    __shared__ unsigned Tx[TXN], Ty[AM_NT][TYN];                   // y tables per tile of a group
+    // tables of two layers (built one layer ahead, during stage 2); y tables per tile
+    __shared__ unsigned Tx[2][TXN], Ty[2][AM_NT][TYN];
     __shared__ __attribute__((aligned(16))) T Qh[AM_NT][AM_SC * AM_QS];
     __shared__ __attribute__((aligned(16))) T Ql[AM_NT][SPLIT ? AM_SC * AM_QS : 8];
 
@@ -212,29 +213,37 @@ __device__ __forceinline__ void adj_mixed_body(const AdjArgs& A, int blk)
 #pragma unroll
                         for (int u = 0; u < AM_NT; ++u) shy[u].init(gy, rowy[u]);
                     }
-                    for (int q = 0; q < nmine; ++q) {
+                    // the tables of layer q into buffer b
+                    auto build = [&](int q, int b) {
                         const float k2 = (float)lk2[q];
-                        __syncthreads();        // previous layer's readers of the tables and Q are done
                         if (QC) {
-                            build_packed<T>(shx, Tx, k2);
+                            build_packed<T>(shx, Tx[b], k2);
 #pragma unroll
                             for (int u = 0; u < AM_NT; ++u)
-                                if (u < nt) build_packed<T>(shy[u], Ty[u], k2);
+                                if (u < nt) build_packed<T>(shy[u], Ty[b][u], k2);
                         } else {
                             for (int t = threadIdx.x; t < ntX; t += FT_THREADS) {
                                 const float d = table_distance_f(rowx, t);
-                                Tx[table_slot(gx, t)] = pack_split<T>(__builtin_amdgcn_exp2f(-(d * d) * k2));
+                                Tx[b][table_slot(gx, t)] = pack_split<T>(__builtin_amdgcn_exp2f(-(d * d) * k2));
                             }
 #pragma unroll
                             for (int u = 0; u < AM_NT; ++u) {
                                 if (u >= nt) break;
                                 for (int t = threadIdx.x; t < ntY; t += FT_THREADS) {
                                     const float d = table_distance_f(rowy[u], t);
-                                    Ty[u][table_slot(gy, t)] = pack_split<T>(__builtin_amdgcn_exp2f(-(d * d) * k2));
+                                    Ty[b][u][table_slot(gy, t)] =
+                                        pack_split<T>(__builtin_amdgcn_exp2f(-(d * d) * k2));
                                 }
                             }
                         }
-                        __syncthreads();
+                    };
+                    __syncthreads();            // the previous group's readers of buffer 0 are done
+                    build(0, 0);
+                    __syncthreads();
+                    // per layer q: stage 1 (Tx[b] -> Q), barrier, stage 2 (Ty[b], Q) with the
+                    // tables of layer q + 1 built into the other buffer, barrier
+                    for (int q = 0; q < nmine; ++q) {
+                        const int bt = q & 1;
                         // stage 1 of both tiles of the group: Q[y][i], this wave's 16 rows of each,
                         // 3 spot blocks; the Gx operand of a (spot block, K step) is gathered once
                         // and used for both tiles
@@ -250,7 +259,7 @@ __device__ __forceinline__ void adj_mixed_body(const AdjArgs& A, int blk)
                                 if (ks < k_lo || ks >= k_hi) continue;
                                 unsigned t8[8];
 #pragma unroll
-                                for (int e = 0; e < 8; ++e) t8[e] = Tx[slot1(nb, ks, e)];
+                                for (int e = 0; e < 8; ++e) t8[e] = Tx[bt][slot1(nb, ks, e)];
                                 V8 bh, bl;
                                 unpack8(t8, bh, bl);
 #pragma unroll
@@ -300,7 +309,7 @@ __device__ __forceinline__ void adj_mixed_body(const AdjArgs& A, int blk)
                                     if (ks < k_lo || ks >= k_hi) continue;
                                     unsigned t8[8];
 #pragma unroll
-                                    for (int e = 0; e < 8; ++e) t8[e] = Ty[u][slot2(mb, ks, e)];
+                                    for (int e = 0; e < 8; ++e) t8[e] = Ty[bt][u][slot2(mb, ks, e)];
                                     V8 ah, al;
                                     unpack8(t8, ah, al);
                                     const int off = (16 * nb + c16) * AM_QS + 32 * ks + 8 * g;
@@ -323,6 +332,8 @@ __device__ __forceinline__ void adj_mixed_body(const AdjArgs& A, int blk)
                                 }
                             }
                         }
+                        if (q + 1 < nmine) build(q + 1, bt ^ 1);
+                        __syncthreads();        // stage 2 has read Q and Ty[bt]; the next tables are built
                     }
                 }
             }

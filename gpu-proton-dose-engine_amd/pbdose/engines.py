@@ -162,8 +162,48 @@ class DeviceProblem:
         self._tuned[p] = (int(best[0]), int(best[1]))
         return self._tuned[p]
 
+    def autotune_split(self, precision: str, reps: int = 20, use_cache: bool = True) -> int:
+        """Measured layer split of the weight adjoint (``precision`` 'fp32', 'bf16' or
+        'bf16x3'): the dispatch model's split and its neighbours (splits 2 to 16) are
+        timed over `reps` objective + adjoint calls, and the fastest is used by later
+        adjoint calls with split 0. Cached like ``autotune``. Every split gives the same
+        dJ/dw bit for bit."""
+        from . import tuning
+        p = _lib.PREC[precision]
+        g, lat = self.problem.grid, self.problem.lattice
+        key = "adj|" + "|".join(str(v) for v in (
+            g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, lat.n_sx, lat.n_sy, lat.n_layers, lat.dx_spot,
+            lat.dy_spot, precision, tuning.device_key(self.dev), self._active.tobytes().hex()))
+        cache = tuning.load_cache() if use_cache else {}
+        if key in cache:
+            best = int(cache[key]["split"])
+        else:
+            from . import planning
+            zero = torch.zeros(self.shape, dtype=torch.float32, device=self.dev)
+            model = self.auto_split(p)
+            results = []
+            for sp in sorted({max(1, min(16, model + d)) for d in (-3, -2, -1, 0, 1, 2, 3)} | {2, 4, 8}):
+                planning.objective_gradient(self, zero, zero, precision=precision, split=sp)
+                torch.cuda.synchronize(self.dev)
+                s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                s.record()
+                for _ in range(reps):
+                    planning.objective_gradient(self, zero, zero, precision=precision, split=sp)
+                e.record()
+                torch.cuda.synchronize(self.dev)
+                results.append((s.elapsed_time(e) / reps, sp))
+            best_ms, best = min(results)
+            self.split_results = results
+            if use_cache:
+                cache = tuning.load_cache()
+                cache[key] = {"split": best, "ms": best_ms, "timings": [[ms, sp] for ms, sp in results]}
+                tuning.save_cache(cache)
+        self._chunk[("split", p)] = int(best)
+        return int(best)
+
     def auto_split(self, precision: int) -> int:
-        """Layer split of the automatic adjoint for this problem's active layers
+        """Layer split of the automatic adjoint: the measured one after autotune_split,
+        else the dispatch model's for this problem's active layers
         (pbh_adjoint_choose_split; cached like auto_chunk)."""
         key = ("split", precision)
         if key not in self._chunk:

@@ -383,3 +383,25 @@ def test_adjoint_degenerate_shapes(shape, precision):
     e = rel(g.cpu(), ref)
     print(f"  vs oracle {e:.2e}", flush=True)
     assert e <= (2e-5 if precision == "fp32" else MIXED_ADJ_TOL[precision])
+
+
+def test_adjoint_split_autotune(tmp_path, monkeypatch):
+    """DeviceProblem.autotune_split times the dispatch model's split and its
+    neighbours; later automatic adjoints use the winner (dJ/dw equals the explicit
+    split's bit for bit, as every split does), and the cache returns it without
+    timing again."""
+    from pbdose import engines, planning
+    monkeypatch.setenv("PBDOSE_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
+    print("case adjoint split autotune bf16x3", flush=True)
+    p = small_problem(nx=64, ny=48, nz=40, d=0.2, nsx=13, nsy=11, ds=0.5,
+                      energies=tuple(np.linspace(70.0, 130.0, 8)))
+    dp = engines.DeviceProblem(p)
+    D = dp.forward()
+    Dp, wv = planning.central_cube(p.grid, half=1.5, dose_value=float(D.max()) * 0.5)
+    best = dp.autotune_split("bf16x3", reps=3)
+    assert best in [sp for _, sp in dp.split_results] and 1 <= best <= 16
+    J0, g0 = planning.objective_gradient(dp, D, Dp, wv, precision="bf16x3")
+    J1, g1 = planning.objective_gradient(dp, D, Dp, wv, precision="bf16x3", split=best)
+    assert torch.equal(g0, g1) and float(J0) == float(J1)
+    dp2 = engines.DeviceProblem(p)
+    assert dp2.autotune_split("bf16x3") == best and not hasattr(dp2, "split_results")
