@@ -299,12 +299,13 @@ def torch_separable(problem: PBProblem, device: str = "cuda", dtype=None,
     dtype         None or torch.float32 (the kernels compute in FP32 or in the
                   mixed forms selected by ``precision``).
     allow_tf32    accepted for compatibility; gfx950 has no TF32 mode, so it
-                  has no effect (the reference measured none either).
+                  has no effect (the reference measured none either) and True warns.
     per_layer     True (default): sigma exact per layer. False: the reference's
                   single-sigma approximation (engines.py:243-255), on the GPU:
                   one width per slice, the layer-weight average sigma_bar(z)
                   (engines.py:75-80), applied to A(z) = sum_l IDD_l(z) w_l.
-    layer_chunk   accepted and ignored: no layer-sized intermediate exists.
+    layer_chunk   accepted and ignored (with a warning when given): no
+                  layer-sized intermediate exists.
     precision     'fp32' (default) or one of 'bf16', 'fp16', 'bf16x3',
                   'fp16x3', 'bf16x3_peak', 'fp16x3_peak'.
     Returns D (nz, ny, nx) on the device, or (D, wall_seconds).
@@ -313,6 +314,12 @@ def torch_separable(problem: PBProblem, device: str = "cuda", dtype=None,
         raise ValueError(f"dtype {dtype} not supported: the dose is computed in float32")
     if return_intermediates:
         warnings.warn("return_intermediates is ignored: no intermediate is materialised", stacklevel=2)
+    if layer_chunk is not None:
+        warnings.warn("layer_chunk is ignored: no layer-sized intermediate exists (the kernels' own "
+                      "layer chunks are chosen per problem)", stacklevel=2)
+    if allow_tf32:
+        warnings.warn("allow_tf32 has no effect: gfx950 has no TF32 mode; the dose is computed in the "
+                      "form `precision` names", stacklevel=2)
     t0 = time.perf_counter()
     dp = DeviceProblem(problem, device) if per_layer else averaged_problem(problem, device)
     D = dp.forward(precision)

@@ -142,6 +142,11 @@ def test_product_refuses_cpu_device():
     prob = model.PBProblem(g, lat, np.ones((1, 4)), np.ones((1, 4)))
     with pytest.raises(ValueError, match="GPU only"):
         engines.torch_separable(prob, "cpu")
+    # options the reference accepts that have no effect here warn before anything runs
+    for kw, text in (({"layer_chunk": 4}, "layer_chunk"), ({"allow_tf32": True}, "allow_tf32"),
+                     ({"return_intermediates": True}, "return_intermediates")):
+        with pytest.warns(UserWarning, match=text), pytest.raises(ValueError, match="GPU only"):
+            engines.torch_separable(prob, "cpu", **kw)
 
 
 def _batch_items(n=3):
