@@ -126,10 +126,11 @@ class DeviceProblem:
 
     def autotune(self, precision: str, reps: int = 20, use_cache: bool = True) -> tuple:
         """Measured (layer chunk, tile rows) of the forward for this problem: every
-        pair of shape_candidates is timed over `reps` back-to-back calls, and the
-        fastest is used by later forward() calls with layer_chunk = tile_rows = 0.
-        Winners are cached per (problem shape, active layers, form, device). All
-        candidates compute the same dose up to FP32 summation order."""
+        pair of shape_candidates is timed over `reps` back-to-back calls, then the
+        chunks within 2 of the fastest at its tile height, and the fastest of all is
+        used by later forward() calls with layer_chunk = tile_rows = 0. Winners are
+        cached per (problem shape, active layers, form, device). All candidates
+        compute the same dose up to FP32 summation order."""
         from . import tuning
         p = _lib.PREC[precision]
         g, lat = self.problem.grid, self.problem.lattice
@@ -142,7 +143,8 @@ class DeviceProblem:
         else:
             out = torch.empty(self.shape, dtype=torch.float32, device=self.dev)
             results = []
-            for c, rows in self.shape_candidates(p):
+
+            def time_shape(c, rows):
                 self.forward(precision, out=out, layer_chunk=c, tile_rows=rows)
                 torch.cuda.synchronize(self.dev)
                 s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -152,6 +154,16 @@ class DeviceProblem:
                 e.record()
                 torch.cuda.synchronize(self.dev)
                 results.append((s.elapsed_time(e) / reps, (c, rows)))
+
+            for c, rows in self.shape_candidates(p):
+                time_shape(c, rows)
+            # refine: the neighbouring chunks of the fastest at its tile height
+            _, (c0, r0) = min(results, key=lambda r: r[0])
+            tried = {cr for _, cr in results}
+            amax = max(1, int(self._active.max()) if self._active.size else 1)
+            for c in range(max(1, c0 - 2), min(amax, c0 + 2) + 1):
+                if (c, r0) not in tried:
+                    time_shape(c, r0)
             best_ms, best = min(results, key=lambda r: r[0])
             self.tuning_results = results
             if use_cache:
