@@ -59,6 +59,7 @@ KERNEL_NAME = {"fp32": "fwd_fp32_kernel"}
 MFMA_FLOP_ISSUED = {"fp32": 11348832 * 512.0, "bf16x3": 22984032 * 512.0, "bf16": 7661344 * 512.0,
                     "fp16x3": 22984032 * 512.0}
 FETCH_SCALE, WRITE_SCALE = 2.000, 1.000
+HBM_PEAK_GBS = 8000.0
 
 # published dense MFMA peaks of MI355X (TFLOP/s) for the arithmetic each form runs on
 PEAK = {"fp32": 157.3, "bf16": 2500.0, "fp16": 2500.0}
@@ -457,6 +458,17 @@ def main():
                           "achieved": achieved_c, "peak": peak, "unit": "TFLOP/s", "frac": achieved_c / peak,
                           "launch_ms": launch_ms},
     }
+    # HBM rate of the forward call (profiled bytes per call / event-timed call) against the
+    # 8 TB/s peak, and MFMA utilisation (MFMA work issued / the contraction kernel's time
+    # against the dense peak of its type)
+    tb = traffic_bytes(args.precision)
+    line["hbm"] = {"bytes_per_call": tb, "gb_s": tb / (launch_ms * 1e-3) / 1e9 if tb else None,
+                   "peak_gb_s": HBM_PEAK_GBS,
+                   "frac": tb / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if tb else None,
+                   # what a call must move: dose (FP32), spot weights (FP32), the two (L, nz) FP64 tables
+                   "algorithmic_bytes_per_call": g.nx * g.ny * g.nz * 4 + lat.n_layers * lat.n_sy * lat.n_sx * 4 +
+                                                 2 * lat.n_layers * g.nz * 8}
+    line["mfma_utilisation"] = issued / (kernel_ms * 1e-3) / 1e12 / peak if issued else None
     if not args.no_robust:
         rs = max(5, args.steps // 10)
         line["robust_batch"] = {
