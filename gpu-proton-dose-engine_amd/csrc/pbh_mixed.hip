@@ -233,6 +233,10 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
     // contraction windows of the current spot chunk, per active layer: [rb] stage A of
     // row block rb (< TY / 16), [4 + w] stage B of wave w; k0 | nstep << 6
     __shared__ unsigned char win[FT_LB][8];
+    // FP16 forms: the largest exponent of each wave's bounds (pass set-up), and the
+    // factor c 2^e 2^-eacc of every active layer on the current spot chunk
+    __shared__ int emax[F16 ? FT_THREADS / 64 : 1];
+    __shared__ float ufac[F16 ? FT_LB : 1];
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -273,6 +277,8 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
     // LISTS: passes over 64 active layers of the slice's precomputed list (written by
     // the weight-split launch); otherwise passes over 64 layers, scanned here
     const int npass = LISTS ? pl.nlayers[z] : a.L;
+    const int nq = ncy * ncx;                           // spot chunks
+    const size_t wlayer = (size_t)nq * FT_WSPLIT;       // split weights per layer
     for (int lb = 0; lb < npass && base < hi; lb += FT_LB) {
         const int nact = LISTS ? copy_layers(pl, a.L, z, lb, zc, r_peak, ll)
                                : collect_layers(a, lb, z, zc, r_peak, ll);
@@ -282,14 +288,19 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
         if (F16) {
             // the accumulators hold dose 2^-eacc: eacc covers the largest bound of this
             // pass's layers on every spot chunk (the accumulated part is rescaled exactly
-            // when a later pass needs a larger one)
+            // when a later pass needs a larger one). The (layer, chunk) bounds are read
+            // in parallel, one per thread, and maximised over the workgroup.
             int e = -1000;
-            for (int m = m0; m < m1; ++m)
-                for (int q = 0; q < ncy * ncx; ++q) {
-                    const float2 wp = pl.wpar[(size_t)ll.lay[m] * ncy * ncx + q];
-                    const float bound = ll.c[m] * wp.x * wp.y;
-                    if (bound > 0.0f && bound <= 3.0e38f) e = max(e, ilogbf(bound) - 13);
-                }
+            for (int t = threadIdx.x; t < (m1 - m0) * nq; t += FT_THREADS) {
+                const int m = m0 + t / nq;
+                const float2 wp = pl.wpar[(size_t)ll.lay[m] * nq + t % nq];
+                const float bound = ll.c[m] * wp.x * wp.y;
+                if (bound > 0.0f && bound <= 3.0e38f) e = max(e, ilogbf(bound) - 13);
+            }
+            for (int o = 32; o > 0; o >>= 1) e = max(e, __shfl_xor(e, o, 64));
+            if (lane == 0) emax[wave] = e;
+            __syncthreads();
+            e = max(max(emax[0], emax[1]), max(emax[2], emax[3]));
             if (e > eacc) {
                 if (eacc > -1000) {
                     const float r = ldexpf(1.0f, eacc - e);
@@ -336,21 +347,23 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                     }
                     win[m][w] = (unsigned char)(sw.k0 | (sw.nstep << 6));
                 }
+                // FP16 forms: U of layer m on this chunk is stored as c 2^e 2^-eacc U_s, below
+                // 2^14 by the weight-split bound c 2^e b of |c Gy W| (U_s = Gy W 2^-e); the
+                // factors of the chunk's layers, read after the next barrier (eacc still
+                // unset: every bound of the pass is 0, all weights 0)
+                if (F16) {
+                    for (int m = m0 + (int)threadIdx.x; m < m1; m += FT_THREADS) {
+                        const float2 wp = pl.wpar[((size_t)ll.lay[m] * ncy + cy) * ncx + cx];
+                        ufac[m] = eacc > -1000 ? ldexpf(ll.c[m] * wp.x, -eacc) : 0.0f;
+                    }
+                }
                 auto window = [&](int m, int w, int& k0) {
                     const int code = __builtin_amdgcn_readfirstlane((int)win[m][w]);
                     k0 = code & 63;
                     return code >> 6;
                 };
                 const unsigned short* wchunk = pl.wsplit + ((size_t)cy * ncx + cx) * FT_WSPLIT;
-                const size_t wlayer = (size_t)ncy * ncx * FT_WSPLIT;
 
-                // FP16 forms: U of layer m on this chunk is stored as c 2^e 2^-eacc U_s, below
-                // 2^14 by the weight-split bound c 2^e b of |c Gy W| (U_s = Gy W 2^-e)
-                auto u_factor = [&](int m) {
-                    const float2 wp = pl.wpar[((size_t)ll.lay[m] * ncy + cy) * ncx + cx];
-                    // eacc still unset: every bound of the pass is 0 (all weights 0)
-                    return eacc > -1000 ? ldexpf(ll.c[m] * wp.x, -eacc) : 0.0f;
-                };
                 auto is_x3 = [&](int m) {
                     if (MODE == MODE_X3) return true;
                     if (MODE == MODE_PLAIN) return false;
@@ -497,7 +510,7 @@ __device__ __forceinline__ void fwd_mixed_body(const FwdArgs& a, const FwdPlan& 
                     }
                     // FP16: U is stored as (c 2^e 2^-eU) U_s with the layer's power of two 2^eU,
                     // from the weight-split bound, putting |U| below 2^14 (no reduction needed)
-                    const float cs = F16 ? u_factor(m) : c;
+                    const float cs = F16 ? ufac[m] : c;
                     // C layout of U^T (16x16): spots 16 cb + 4 gA + r (r < 4), row yA:
                     // four consecutive spots of U's row yA, one 8-byte store per half
                     auto put = [&](v4f u, int cb) {
