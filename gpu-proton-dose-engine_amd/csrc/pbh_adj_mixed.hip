@@ -245,9 +245,9 @@ __device__ __forceinline__ void adj_mixed_body(const AdjArgs& A, int blk)
                     for (int q = 0; q < nmine; ++q) {
                         const int bt = q & 1;
                         // stage 1 of both tiles of the group: Q[y][i], this wave's 16 rows of each,
-                        // 3 spot blocks; the Gx operand of a (spot block, K step) is gathered once
-                        // and used for both tiles
-#pragma unroll 1
+                        // 3 spot blocks (unrolled: their product chains interleave); the Gx operand
+                        // of a (spot block, K step) is gathered once and used for both tiles
+#pragma unroll
                         for (int nb = 0; nb < 3; ++nb) {
                             const int wc = __builtin_amdgcn_readfirstlane((int)awin[par][q][nb]);
                             const int k_lo = wc & 15, k_hi = wc >> 4;
