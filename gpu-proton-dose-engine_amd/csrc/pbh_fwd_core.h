@@ -188,6 +188,42 @@ __device__ inline int count_active(const FwdArgs& a, int z, int* scratch)
     return t;
 }
 
+// Rows of one dose tile (from row y0, column x0, NIT x 1024 voxels) += NP partial tiles
+// in chunk order, for a 256-thread workgroup: thread t takes voxels 4 t + 1024 it (it <
+// NIT), loads all of them first, then adds and stores. The tile's columns lie inside the
+// grid and the dose rows are 16-byte aligned; rows beyond the grid are skipped.
+template <int NIT, int NP>
+__device__ __forceinline__ void combine_rows(const FwdArgs& a, const float* part0, size_t cstride, int z, int y0,
+                                             int x0)
+{
+    float4 v[NIT], p[NP][NIT];
+    int n = 0;
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int e = 4 * (int)threadIdx.x + 1024 * it;
+        const int y = y0 + e / FT_TX;
+        if (y >= a.ny) break;
+        n = it + 1;
+        v[it] = *(const float4*)(a.dose + ((size_t)z * a.ny + y) * a.nx + x0 + e % FT_TX);
+#pragma unroll
+        for (int c = 0; c < NP; ++c) p[c][it] = *(const float4*)(part0 + (size_t)c * cstride + e);
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        if (it >= n) break;
+        const int e = 4 * (int)threadIdx.x + 1024 * it;
+        float4 r = v[it];
+#pragma unroll
+        for (int c = 0; c < NP; ++c) {
+            r.x += p[c][it].x;
+            r.y += p[c][it].y;
+            r.z += p[c][it].z;
+            r.w += p[c][it].w;
+        }
+        *(float4*)(a.dose + ((size_t)z * a.ny + y0 + e / FT_TX) * a.nx + x0 + e % FT_TX) = r;
+    }
+}
+
 // dose tile += partial tiles of chunks 1 .. nchunk(z) - 1, in chunk order.
 // One workgroup of 256 threads per dose tile.
 __device__ inline void combine_body(const FwdArgs& a, const FwdPlan& pl, int tile)
@@ -204,6 +240,22 @@ __device__ inline void combine_body(const FwdArgs& a, const FwdPlan& pl, int til
     const float* part0 = pl.part + (size_t)tile * ((size_t)TR * FT_TX);
     const size_t cstride = tiles * ((size_t)TR * FT_TX);
     const bool vec = (a.nx & 3) == 0 && tx * FT_TX + FT_TX <= a.nx && ((uintptr_t)a.dose & 15) == 0;
+    if (vec && blockDim.x == 256 && nch <= 4) {
+        // whole 128-column rows inside the grid, four consecutive voxels per thread; the
+        // thread's float4s of the dose and of every partial are loaded before any store,
+        // so that their latencies overlap (one load, add and store at a time paid one
+        // memory latency per four voxels)
+        if (TR == 64) {
+            if (nch == 2) combine_rows<8, 1>(a, part0, cstride, z, ty * TR, tx * FT_TX);
+            else if (nch == 3) combine_rows<8, 2>(a, part0, cstride, z, ty * TR, tx * FT_TX);
+            else combine_rows<8, 3>(a, part0, cstride, z, ty * TR, tx * FT_TX);
+        } else {
+            if (nch == 2) combine_rows<4, 1>(a, part0, cstride, z, ty * TR, tx * FT_TX);
+            else if (nch == 3) combine_rows<4, 2>(a, part0, cstride, z, ty * TR, tx * FT_TX);
+            else combine_rows<4, 3>(a, part0, cstride, z, ty * TR, tx * FT_TX);
+        }
+        return;
+    }
     if (vec) {
         // whole 128-column rows inside the grid: four consecutive voxels per thread
         for (int e = 4 * threadIdx.x; e < TR * FT_TX; e += 4 * blockDim.x) {
