@@ -271,6 +271,27 @@ def test_layer_chunk_parity_and_determinism(precision):
         assert r <= tol
 
 
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16x3", "bf16"])
+@pytest.mark.parametrize("rows", [32, 64])
+def test_combine_of_two_to_six_chunks(precision, rows):
+    """Layer chunks that leave 1, 2 or 3 partial tiles per dose tile (the combine's
+    unrolled paths, every load before the first store) and 5 (its loop), at both tile
+    heights, on a grid whose last band of tile rows is cut by the grid (ny = 100): each
+    within the form's bound of the oracle and repeatable bit for bit."""
+    from pbdose import engines
+    print(f"case combine chunks {precision} {rows} rows", flush=True)
+    p = make_problem(128, 100, 6, 0.2, 41, 33, 0.5, np.linspace(80, 170, 16), seed=31)
+    dp = engines.DeviceProblem(p)
+    ref = oracle_dose(p)
+    for chunk in (8, 6, 4, 3):            # 16 active layers per slice: 2, 3, 4, 6 chunks
+        a = dp.forward(precision, layer_chunk=chunk, tile_rows=rows).clone()
+        b = dp.forward(precision, layer_chunk=chunk, tile_rows=rows)
+        assert torch.equal(a, b), f"layer chunk {chunk} not deterministic"
+        r = O.relative_error(a.double().cpu().numpy(), ref)["max_rel_err_global"]
+        print(f"  layer chunk {chunk}: {r:.3e}", flush=True)
+        assert r <= MIXED_TOL[precision]
+
+
 @pytest.mark.parametrize("precision", ["fp32", "bf16x3", "fp16x3", "bf16"])
 def test_direct_factor_path(precision):
     # spot pitch 0.37 cm against 0.2 cm voxels has no small integer ratio:
