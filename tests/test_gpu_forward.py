@@ -56,6 +56,27 @@ def test_engine_check_problem():
     assert np.abs(D[:4] - B).max() <= TOL_FP32 * np.abs(B).max()
 
 
+@pytest.mark.parametrize("precision,tol", [("fp32", TOL_FP32), ("bf16x3", 2e-5), ("fp16x3", 2e-6)])
+def test_triton_check_problem(precision, tol):
+    # triton_check2.py:14-22 geometry: 32x32x40 @ 2.5 mm, 3 layers (90, 140, 190 MeV) of
+    # 5x5 spots, seed 3; per-layer exact against separable_numpy(per_layer=True) and the
+    # single-sigma path against separable_numpy(per_layer=False), as that script checks
+    print(f"case triton_check 32x32x40 {precision}", flush=True)
+    from pbdose import engines, model
+    g = model.DoseGrid(32, 32, 40, 0.25, 0.25, 0.25, -4.0, -4.0, 0.0)
+    E = np.array([90.0, 140.0, 190.0])
+    w = np.random.default_rng(3).random((3, 5, 5))
+    idd, sig = O.layer_luts(g.z, E)
+    lat = model.SpotLattice(5, 5, 3, 0.5, 0.5, energies=E, weights=w)
+    p = model.PBProblem(g, lat, idd, sig)
+    check(p, tol=tol, precision=precision)
+    Da = engines.torch_separable(p, "cuda", per_layer=False, precision=precision).double().cpu().numpy()
+    ref = O.separable_dose_averaged(g.x, g.y, lat.spot_x, lat.spot_y, w, idd, sig)
+    r = O.relative_error(Da, ref)["max_rel_err_global"]
+    print(f"  single sigma: {r:.3e}", flush=True)
+    assert r <= tol
+
+
 def test_config_small_64cube():
     # 64^3 @ 2 mm, 21x21 @ 5 mm, 10 layers 100-150 MeV, w = 1
     print("case 64^3 / 21x21 / 10 layers", flush=True)
