@@ -347,8 +347,8 @@ def main():
                     help="comma-separated forms: run a short kernel sequence for rocprofv3 and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build runs the "
-              "whole field on a single MI355X", file=sys.stderr)
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}): the engine computes the "
+              "whole field on a single MI355X and does not shard it over GPUs", file=sys.stderr)
         sys.exit(2)
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
