@@ -6,7 +6,8 @@ engine, backed by the fused HIP kernels of libpbdose_hip.so.
 
 ``torch_separable`` keeps the reference signature (engines.py:208-283): it
 copies the problem's host arrays to the device on every call, runs the fused
-kernel and synchronises, returning a (nz, ny, nx) float32 device tensor.
+kernel and synchronises, returning a (nz, ny, nx) float32 device tensor
+(float64 from the FP64 kernel with dtype=torch.float64).
 ``DeviceProblem`` is the device-resident path with no per-call copies.
 """
 from __future__ import annotations
@@ -41,6 +42,8 @@ class DeviceProblem:
         self.lat_s = _lib.lattice_struct(lat)
         self.shape = g.shape
         w = lat.ensure_weights() if weights is None else weights
+        self._w_src = w          # the weights as given, for the FP64 form
+        self._w64 = None
         self.w = torch.as_tensor(w, dtype=torch.float32, device=self.dev).contiguous()
         self.idd = torch.as_tensor(problem.idd if idd is None else idd, dtype=torch.float64,
                                    device=self.dev).contiguous()
@@ -301,6 +304,26 @@ class DeviceProblem:
         return out
 
 
+    def forward_fp64(self, out: Optional[torch.Tensor] = None, w: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """Dose (nz, ny, nx) float64 from FP64 weights and tables (pbh_forward_fp64),
+        asynchronous on the current stream. ``w``: FP64 (L, nsy, nsx) weights on the
+        device; default the problem's weights at full precision."""
+        if out is None:
+            out = torch.empty(self.shape, dtype=torch.float64, device=self.dev)
+        elif out.dtype != torch.float64 or tuple(out.shape) != self.shape or not out.is_contiguous():
+            raise ValueError("out must be a contiguous float64 (nz, ny, nx) tensor")
+        if w is None:
+            if self._w64 is None:
+                self._w64 = torch.as_tensor(self._w_src, dtype=torch.float64, device=self.dev).contiguous()
+            w = self._w64
+        if w.dtype != torch.float64 or tuple(w.shape) != tuple(self.w.shape) or not w.is_contiguous():
+            raise ValueError("weights must be a contiguous float64 (L, nsy, nsx) tensor")
+        _lib.check(_lib.lib().pbh_forward_fp64(self.grid_s, self.lat_s, _lib.ptr(w), _lib.ptr(self.idd),
+                                               _lib.ptr(self.sigma), _lib.ptr(out), _lib.stream_of(self.dev)),
+                   "pbh_forward_fp64")
+        return out
+
+
 def torch_separable(problem: PBProblem, device: str = "cuda", dtype=None,
                     allow_tf32: bool = False, per_layer: bool = True,
                     layer_chunk: Optional[int] = None, return_timing: bool = False,
@@ -308,8 +331,9 @@ def torch_separable(problem: PBProblem, device: str = "cuda", dtype=None,
     """Per-layer-exact dose on the GPU, drop-in for engines.py:208-283.
 
     device        must be a CUDA (ROCm) device; there is no CPU path.
-    dtype         None or torch.float32 (the kernels compute in FP32 or in the
-                  mixed forms selected by ``precision``).
+    dtype         None or torch.float32: the dose in FP32, computed in FP32 or in
+                  the mixed form ``precision`` selects. torch.float64: FP64
+                  weights, tables, arithmetic and dose (per_layer=True only).
     allow_tf32    accepted for compatibility; gfx950 has no TF32 mode, so it
                   has no effect (the reference measured none either) and True warns.
     per_layer     True (default): sigma exact per layer. False: the reference's
@@ -322,8 +346,12 @@ def torch_separable(problem: PBProblem, device: str = "cuda", dtype=None,
                   'fp16x3', 'bf16x3_peak', 'fp16x3_peak'.
     Returns D (nz, ny, nx) on the device, or (D, wall_seconds).
     """
-    if dtype not in (None, torch.float32):
-        raise ValueError(f"dtype {dtype} not supported: the dose is computed in float32")
+    if dtype not in (None, torch.float32, torch.float64):
+        raise ValueError(f"dtype {dtype} not supported: float32 (default) or float64")
+    if dtype == torch.float64 and not per_layer:
+        raise ValueError("dtype=float64 computes the per-layer-exact dose; per_layer=False is float32 only")
+    if dtype == torch.float64 and precision != "fp32":
+        raise ValueError(f"precision {precision!r} selects a float32 form; dtype=float64 computes in FP64")
     if return_intermediates:
         warnings.warn("return_intermediates is ignored: no intermediate is materialised", stacklevel=2)
     if layer_chunk is not None:
@@ -334,7 +362,7 @@ def torch_separable(problem: PBProblem, device: str = "cuda", dtype=None,
                       "form `precision` names", stacklevel=2)
     t0 = time.perf_counter()
     dp = DeviceProblem(problem, device) if per_layer else averaged_problem(problem, device)
-    D = dp.forward(precision)
+    D = dp.forward_fp64() if dtype == torch.float64 else dp.forward(precision)
     torch.cuda.synchronize(dp.dev)
     dt = time.perf_counter() - t0
     return (D, dt) if return_timing else D

@@ -196,6 +196,15 @@ int pbh_forward(const PbhGrid* g, const PbhLattice* lat,
                 float* dose, int precision,
                 void* workspace, size_t workspace_bytes, void* stream);
 
+/* The forward dose in FP64: weights w (L, nsy, nsx), tables idd, sigma (L, nz)
+ * and dose (nz, ny, nx) all FP64, FP64 FMA on the vector ALUs, every voxel's
+ * layers and spots summed in a fixed order (deterministic). No workspace; the
+ * dose is overwritten. engines.py:208-283 torch_separable(dtype=torch.float64)
+ * on a GPU (per_layer=True); model of engines.py:172-202. */
+int pbh_forward_fp64(const PbhGrid* g, const PbhLattice* lat,
+                     const double* w, const double* idd, const double* sigma,
+                     double* dose, void* stream);
+
 /* pbh_forward with an explicit layer chunk: the active layers (IDD != 0) of
  * each depth slice are cut into chunks of at most `layer_chunk` layers (0 =
  * automatic; >= L: one chunk), each computed by its own workgroup; a second

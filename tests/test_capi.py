@@ -70,6 +70,9 @@ def test_forward_rejects_null_and_bad_precision_without_launching():
     blat = _lib.PbhLattice(41, 41, 40, 0, 0.5, 0.5, 0.0, 0.0)
     assert L.pbh_forward_workspace_bytes_tuned(big, blat, 3, 1) > _lib.PBH_NOALLOC_SCRATCH_BYTES
     assert L.pbh_forward_noalloc(big, blat, one, one, one, one, 3, 1, nul) == -8
+    assert L.pbh_forward_fp64(g, lat, one, nul, one, one, nul) == -1
+    empty = _lib.PbhGrid(0, 8, 8, 0, 0.2, 0.2, 0.2, 0, 0, 0)
+    assert L.pbh_forward_fp64(empty, lat, one, one, one, one, nul) == -2
 
 
 def test_average_sigma_rejects_bad_arguments_without_launching():
@@ -136,6 +139,7 @@ def test_lut_workspace_query():
 
 def test_product_refuses_cpu_device():
     import numpy as np
+    import torch
     from pbdose import engines, model
     g = model.DoseGrid(4, 4, 4, 0.2, 0.2, 0.2)
     lat = model.SpotLattice(2, 2, 1, 0.5, 0.5, weights=np.ones((1, 2, 2)))
@@ -147,6 +151,12 @@ def test_product_refuses_cpu_device():
                      ({"return_intermediates": True}, "return_intermediates")):
         with pytest.warns(UserWarning, match=text), pytest.raises(ValueError, match="GPU only"):
             engines.torch_separable(prob, "cpu", **kw)
+    # float64 is the per-layer-exact FP64 form; other dtypes and FP32-only options refuse
+    for kw, text in (({"dtype": torch.float16}, "not supported"),
+                     ({"dtype": torch.float64, "per_layer": False}, "per_layer=False"),
+                     ({"dtype": torch.float64, "precision": "bf16x3"}, "FP64")):
+        with pytest.raises(ValueError, match=text):
+            engines.torch_separable(prob, "cuda", **kw)
 
 
 def _batch_items(n=3):
