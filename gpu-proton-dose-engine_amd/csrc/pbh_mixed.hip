@@ -80,7 +80,12 @@ __device__ __forceinline__ void wsplit_body(const FwdArgs& a, unsigned short* ds
     const int i0 = chunk_begin(a.nsx, ncx, cx), ni = chunk_begin(a.nsx, ncx, cx + 1) - i0;
     const int j0 = chunk_begin(a.nsy, ncy, cy), nj = chunk_begin(a.nsy, ncy, cy + 1) - j0;
     const float* wl = a.w + (size_t)l * a.nsy * a.nsx;
-    float sc = 1.0f;
+    // this thread's pair (loaded before the FP16 scale is known, so that the loads
+    // overlap the reduction below)
+    const int p = part * 256 + threadIdx.x;               // pair index: i = p / 32, j = 2 (p % 32) + {0, 1}
+    const int i = p / (KYP / 2), j = 2 * (p % (KYP / 2));
+    float v0 = (i < ni && j < nj) ? wl[(size_t)(j0 + j) * a.nsx + i0 + i] : 0.0f;
+    float v1 = (i < ni && j + 1 < nj) ? wl[(size_t)(j0 + j + 1) * a.nsx + i0 + i] : 0.0f;
     if (F16) {
         // FP16's exponent range: the chunk's weights are scaled by 2^-e so that their
         // largest magnitude lies in [1, 2); every part of the chunk forms the same
@@ -115,15 +120,13 @@ __device__ __forceinline__ void wsplit_body(const FwdArgs& a, unsigned short* ds
 #pragma unroll
         for (int q = 0; q < FT_THREADS / 64; ++q) wmax = fmaxf(wmax, part_max[q]);
         const int e = (wmax > 0.0f && wmax <= 3.0e38f) ? ilogbf(wmax) : 0;
-        sc = ldexpf(1.0f, -e);
+        const float sc = ldexpf(1.0f, -e);
         if (part == 0 && threadIdx.x == 0) wpar[blk] = make_float2(ldexpf(1.0f, e), cmax * sc);
+        v0 *= sc;
+        v1 *= sc;
     }
     unsigned* hi = (unsigned*)(dst + (size_t)blk * FT_WSPLIT);
     unsigned* lo = hi + WS_PAIRS;
-    const int p = part * 256 + threadIdx.x;               // pair index: i = p / 32, j = 2 (p % 32) + {0, 1}
-    const int i = p / (KYP / 2), j = 2 * (p % (KYP / 2));
-    const float v0 = (i < ni && j < nj) ? wl[(size_t)(j0 + j) * a.nsx + i0 + i] * sc : 0.0f;
-    const float v1 = (i < ni && j + 1 < nj) ? wl[(size_t)(j0 + j + 1) * a.nsx + i0 + i] * sc : 0.0f;
     T h0, r0, h1, r1;
     split<T>(v0, h0, r0);
     split<T>(v1, h1, r1);
