@@ -88,37 +88,25 @@ __device__ __forceinline__ void wsplit_body(const FwdArgs& a, unsigned short* ds
     float v1 = (i < ni && j + 1 < nj) ? wl[(size_t)(j0 + j + 1) * a.nsx + i0 + i] : 0.0f;
     if (F16) {
         // FP16's exponent range: the chunk's weights are scaled by 2^-e so that their
-        // largest magnitude lies in [1, 2); every part of the chunk forms the same
-        // maxima (same order), and part 0 records 2^e and the column-sum bound
-        // wave g sums rows j = g, g + 4, ... of column i = lane; the four partial
-        // column sums are added in wave order, then maximised over the columns
-        __shared__ float part_sum[FT_THREADS / 64][64];
-        __shared__ float part_max[FT_THREADS / 64];
-        const int col = threadIdx.x & 63, g = threadIdx.x >> 6;
-        float csum = 0.0f, mx = 0.0f;
+        // largest magnitude lies in [1, 2), and part 0 records 2^e and the column-sum
+        // bound. Every wave forms the chunk's maxima by itself (lane i sums column i over
+        // the chunk's rows in row order; maxima over the lanes), so every wave of every
+        // part has the same values and no barrier is needed.
+        const int col = threadIdx.x & 63;
+        float cmax = 0.0f, wmax = 0.0f;
         if (col < ni) {
-            for (int j = g; j < nj; j += FT_THREADS / 64) {
-                const float v = fabsf(wl[(size_t)(j0 + j) * a.nsx + i0 + col]);
-                csum += v;
-                mx = fmaxf(mx, v);
+            const float* wc = wl + (size_t)j0 * a.nsx + i0 + col;
+#pragma unroll 8
+            for (int r = 0; r < nj; ++r) {
+                const float v = fabsf(wc[(size_t)r * a.nsx]);
+                cmax += v;
+                wmax = fmaxf(wmax, v);
             }
         }
-        part_sum[g][col] = csum;
-        for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-        if (col == 0) part_max[g] = mx;
-        __syncthreads();
-        float cmax = 0.0f, wmax = 0.0f;
-        if (g == 0) {
-            float t = 0.0f;
-#pragma unroll
-            for (int q = 0; q < FT_THREADS / 64; ++q) t += part_sum[q][col];
-            for (int o = 32; o > 0; o >>= 1) t = fmaxf(t, __shfl_xor(t, o, 64));
-            if (col == 0) part_sum[0][0] = t;
+        for (int o = 32; o > 0; o >>= 1) {
+            cmax = fmaxf(cmax, __shfl_xor(cmax, o, 64));
+            wmax = fmaxf(wmax, __shfl_xor(wmax, o, 64));
         }
-        __syncthreads();
-        cmax = part_sum[0][0];
-#pragma unroll
-        for (int q = 0; q < FT_THREADS / 64; ++q) wmax = fmaxf(wmax, part_max[q]);
         const int e = (wmax > 0.0f && wmax <= 3.0e38f) ? ilogbf(wmax) : 0;
         const float sc = ldexpf(1.0f, -e);
         if (part == 0 && threadIdx.x == 0) wpar[blk] = make_float2(ldexpf(1.0f, e), cmax * sc);
