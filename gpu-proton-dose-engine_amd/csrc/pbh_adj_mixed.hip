@@ -345,7 +345,10 @@ template <int MODE, int QC>
 __global__ __launch_bounds__(FT_THREADS, 2)
 void adj_mixed_kernel(AdjArgs A)
 {
-    adj_mixed_body<MODE, QC>(A, blockIdx.x);
+    int z, sh;
+    xcd_unit((int)blockIdx.x, A.split, z, sh);
+    if (z >= A.f.nz) return;
+    adj_mixed_body<MODE, QC>(A, z * A.split + sh);
 }
 
 template <int MODE, int QC>
@@ -379,7 +382,7 @@ static bool split_fits(const AdjArgs& A)
 int pbh_launch_adj_mixed(const AdjArgs& A, int precision, int nz, hipStream_t s)
 {
     if (!split_fits(A)) return PBH_ERR_UNSUPPORTED;
-    const dim3 grid((unsigned)(nz * A.split)), block(FT_THREADS);
+    const dim3 grid((unsigned)xcd_grid(nz, A.split)), block(FT_THREADS);
     const bool x3 = precision != PBH_PREC_BF16;
     const bool q5 = pbh_adj_mixed_q5(A);
     if (x3) {

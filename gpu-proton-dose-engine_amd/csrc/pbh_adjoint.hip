@@ -306,7 +306,10 @@ template <bool TAB, bool SIG, int QC>
 __global__ __launch_bounds__(AJ_THREADS, 2)
 void adj_fp32_kernel(AdjArgs A)
 {
-    adj_fp32_body<TAB, SIG, QC>(A, blockIdx.x);
+    int z, sh;
+    xcd_unit((int)blockIdx.x, A.split, z, sh);
+    if (z >= A.f.nz) return;
+    adj_fp32_body<TAB, SIG, QC>(A, z * A.split + sh);
 }
 
 // One launch over a batch of problems sharing the template parameters.
@@ -605,7 +608,7 @@ int pbh_make_adj_args(const PbhGrid* g, const PbhLattice* lat, const float* w, c
 {
     if (split == 0) split = pbh_adj_default_split(g, lat);
     if (split < 1 || split > AJ_MAX_SPLIT) return PBH_ERR_UNSUPPORTED;
-    if ((long long)g->nz * split > 0x7fffffffLL) return PBH_ERR_UNSUPPORTED;
+    if ((long long)kXcds * split * cdiv(g->nz, kXcds) > 0x7fffffffLL) return PBH_ERR_UNSUPPORTED;
     AdjArgs& A = *out;
     A.f = make_fwd_args(g, lat, w, idd, sigma, nullptr);
     A.R = R;
@@ -648,7 +651,7 @@ static int adjoint_launch(const PbhGrid* g, const PbhLattice* lat, const AdjArgs
     if (mixed) {
         if ((rc = pbh_launch_adj_mixed(A, precision, g->nz, s)) != PBH_OK) return rc;
     } else {
-        const dim3 grid((unsigned)(g->nz * A.split)), block(AJ_THREADS);
+        const dim3 grid((unsigned)xcd_grid(g->nz, A.split)), block(AJ_THREADS);
         const bool sig = grad_sigma != nullptr;
         if (pbh_adj_mixed_q5(A)) {
             if (sig) hipLaunchKernelGGL((adj_fp32_kernel<true, true, 5>), grid, block, 0, s, A);

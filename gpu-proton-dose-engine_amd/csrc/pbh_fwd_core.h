@@ -499,6 +499,24 @@ __host__ __device__ inline int chunk_pad(int nsx, int nsy)
     return (m + 3) & ~3;
 }
 
+// The workgroups of a launch are dispatched to the 8 XCDs in turn (workgroup b to XCD
+// b mod 8), and each XCD has its own L2. xcd_unit maps launch index b to the work
+// unit (slice z, share s) of a launch of split workgroups per slice so that all shares
+// of a slice run on one XCD and read the slice's data through one L2. Round r gives
+// the XCDs slices 8 r .. 8 r + 7, in reverse XCD order on odd rounds, so that the work
+// per slice, which falls with depth, evens out over the XCDs. The grid is
+// xcd_grid(nz, split); indices with z >= nz have no work.
+constexpr int kXcds = 8;
+
+__host__ __device__ inline int xcd_grid(int nz, int split) { return kXcds * split * cdiv(nz, kXcds); }
+
+__device__ inline void xcd_unit(int b, int split, int& z, int& s)
+{
+    const int k = b / kXcds, x = b % kXcds, r = k / split;
+    s = k % split;
+    z = kXcds * r + ((r & 1) ? kXcds - 1 - x : x);
+}
+
 // ---- batches of independent problems (robust scenarios) ----
 // The workgroups of one batched launch are split among its items by a prefix
 // table: item k owns workgroups wg0[k] .. wg0[k+1]-1 and runs them as the
