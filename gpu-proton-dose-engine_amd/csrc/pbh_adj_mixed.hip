@@ -357,7 +357,10 @@ void adj_mixed_batch_kernel(const AdjArgs* __restrict__ items, const int* __rest
 {
     const int k = batch_item(wg0, n, blockIdx.x);
     const AdjArgs A = items[k];
-    adj_mixed_body<MODE, QC>(A, (int)blockIdx.x - wg0[k]);
+    int z, sh;     // item ranges start at multiples of 8: launch index b and b - wg0[k] share an XCD
+    xcd_unit((int)blockIdx.x - wg0[k], A.split, z, sh);
+    if (z >= A.f.nz) return;
+    adj_mixed_body<MODE, QC>(A, z * A.split + sh);
 }
 
 }  // namespace pbh

@@ -319,7 +319,10 @@ void adj_fp32_batch_kernel(const AdjArgs* __restrict__ items, const int* __restr
 {
     const int k = batch_item(wg0, n, blockIdx.x);
     const AdjArgs A = items[k];
-    adj_fp32_body<TAB, SIG, QC>(A, (int)blockIdx.x - wg0[k]);
+    int z, sh;     // item ranges start at multiples of 8: launch index b and b - wg0[k] share an XCD
+    xcd_unit((int)blockIdx.x - wg0[k], A.split, z, sh);
+    if (z >= A.f.nz) return;
+    adj_fp32_body<TAB, SIG, QC>(A, z * A.split + sh);
 }
 
 // grad_w[e] = sum_z part[z][e] in FP64, increasing z (8 loads in flight per thread)

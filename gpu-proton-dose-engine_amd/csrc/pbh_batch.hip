@@ -339,7 +339,7 @@ extern "C" int pbh_batch_create(const PbhBatchItem* items, int n, int precision,
         const bool mix = b->adj_prec != PBH_PREC_FP32 && A.tab && !it.grad_idd && !it.grad_sigma &&
                          pbh_adj_mixed_fits(A);
         atabs.push_back((char)((mix ? 3 : (A.tab ? 1 : 0)) | (pbh_adj_mixed_q5(A) ? 4 : 0)));
-        aw.push_back(it.grid.nz * A.split);
+        aw.push_back(xcd_grid(it.grid.nz, A.split));     // a multiple of 8 (XCD-aware slice order)
         const size_t ne = (size_t)it.lattice.n_layers * it.lattice.n_sy * it.lattice.n_sx;
         hr.push_back(BatchRedItem{part, it.idd, it.grad_w, ne, it.grid.nz,
                                   it.lattice.n_sy * it.lattice.n_sx});
@@ -367,6 +367,10 @@ extern "C" int pbh_batch_create(const PbhBatchItem* items, int n, int precision,
             for (int q = 0; q < g.count; ++q) {
                 adj_wg0[g.wg0_off + q] = (int)acc;
                 acc += aw2[g.first + q];
+            }
+            if (acc > 0x7fffffffLL) {
+                delete b;
+                return PBH_ERR_UNSUPPORTED;
             }
             adj_wg0[g.wg0_off + g.count] = (int)acc;
             g.total = (int)acc;
