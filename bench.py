@@ -48,8 +48,8 @@ import torch  # noqa: E402
 # counters on a 256 MiB copy (fetch x 2.000, write x 1.000):
 # profiles/counters_forward_adjoint_128cube_67240spots.md (the shapes bench.py's autotuner picked
 # there: 64-row tiles, layer chunk 14 for bf16x3 and fp16x3, 10 for bf16; 32 rows, chunk 28 for FP32)
-TRAFFIC_KIB = {"fp32": (4898.6, 14592.0), "bf16x3": (9621.6, 22927.6), "bf16": (11477.6, 28559.6),
-               "fp16x3": (9640.6, 22928.9)}
+TRAFFIC_KIB = {"fp32": (5026.4, 14848.0), "bf16x3": (9624.1, 22927.6), "bf16": (11477.6, 28559.6),
+               "fp16x3": (9639.1, 22928.9)}
 CALL_KERNELS = {"fp32": "fwd_fp32_kernel + combine_kernel"}
 KERNEL_NAME = {"fp32": "fwd_fp32_kernel"}
 # MFMA FLOP the contraction kernel issues per launch on this workload (not the
