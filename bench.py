@@ -62,7 +62,7 @@ FETCH_SCALE, WRITE_SCALE = 2.000, 1.000
 HBM_PEAK_GBS = 8000.0
 
 # published dense MFMA peaks of MI355X (TFLOP/s) for the arithmetic each form runs on
-PEAK = {"fp32": 157.3, "bf16": 2500.0, "fp16": 2500.0}
+PEAK = {"fp32": 157.3, "bf16": 2500.0, "fp16": 2500.0, "fp64": 78.6}
 DTYPE = {"fp32": "f32", "bf16": "bf16", "fp16": "f16"}
 
 
@@ -139,6 +139,25 @@ def time_forward(dp, precision, steps, warmup):
     launch_ms = call_events(lambda: dp.forward(precision, out=out), steps)
     kernel_ms = kernel_events(dp, precision, steps, out)
     return out, wall * 1e3 / steps, launch_ms, kernel_ms
+
+
+def time_forward_fp64(dp, steps, warmup):
+    """The FP64 form (torch_separable(dtype=torch.float64)): (dose, wall ms per step over
+    back-to-back steps, ms per call from per-call events; the call is one kernel). Its
+    weights are the FP32 weights every other form sees, held in FP64, so that the CPU
+    leg's FP64 dose is its reference too."""
+    out = torch.empty(dp.shape, dtype=torch.float64, device=dp.dev)
+    w = dp.w.double().contiguous()
+    for _ in range(warmup):
+        dp.forward_fp64(out=out, w=w)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dp.forward_fp64(out=out, w=w)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    launch_ms = call_events(lambda: dp.forward_fp64(out=out, w=w), steps)
+    return out, wall * 1e3 / steps, launch_ms
 
 
 def accuracy(ref64, D):
@@ -382,6 +401,16 @@ def main():
                        "kernel_achieved_tflop_s": flops / (kern_ms * 1e-3) / 1e12,
                        "kernel_frac_of_peak": flops / (kern_ms * 1e-3) / 1e12 / PEAK[base_type(f)],
                        "accuracy_vs_fp64": None}
+    # the FP64 form: FP64 weights, tables, arithmetic and dose on the vector ALUs (one kernel)
+    steps64 = max(20, args.steps // 4)
+    D64, ms64, call64 = time_forward_fp64(dp, steps64, args.warmup)
+    doses["fp64"] = D64
+    per_form["fp64"] = {"ms_per_step": ms64, "fields_per_s": 1e3 / ms64, "call_us": call64 * 1e3,
+                        "tile": "32 x 128 voxels, all layers (FP64 FMA on the vector ALUs)",
+                        "kernel_us": call64 * 1e3, "steps": steps64,
+                        "kernel_achieved_tflop_s": flops / (call64 * 1e-3) / 1e12,
+                        "kernel_frac_of_peak": flops / (call64 * 1e-3) / 1e12 / PEAK["fp64"],
+                        "accuracy_vs_fp64": None}
     ms_step, launch_ms = per_form[args.precision]["ms_per_step"], per_form[args.precision]["call_us"] * 1e-3
     kernel_ms = per_form[args.precision]["kernel_us"] * 1e-3
     ms32 = per_form["fp32"]["ms_per_step"]
@@ -479,7 +508,7 @@ def main():
             "forward_adjoint": time_robust(args.precision, rs, objective=True)}
     if not args.no_cpu:
         line["cpu_baseline"], ref64 = cpu_leg(prob)
-        for f in forms:
+        for f in forms + ["fp64"]:
             per_form[f]["accuracy_vs_fp64"] = accuracy(ref64, doses[f])
         line["accuracy_vs_fp64"] = per_form[args.precision]["accuracy_vs_fp64"]
         line["fp32"]["accuracy_vs_fp64"] = per_form["fp32"]["accuracy_vs_fp64"]
