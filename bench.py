@@ -192,7 +192,8 @@ def adjoint_form(precision):
     """Arithmetic of the adjoint that goes with a forward form."""
     if precision == "fp32":
         return "fp32"
-    return "bf16" if precision in ("bf16", "fp16") else "bf16x3"
+    t = "fp16" if precision.startswith("fp16") else "bf16"
+    return t if precision in ("bf16", "fp16") else t + "x3"
 
 
 def time_adjoint(dp, D, steps, warmup, precision="fp32"):
@@ -426,6 +427,16 @@ def main():
         adj_err = float((gmix.double() - g32.double()).abs().max() / g32.double().abs().max())
     else:
         adj_ms, adj_ev, adj_err = adj32_ms, adj32_ev, 0.0
+    # both split-product adjoints side by side (the FP16 one scales R per wave)
+    adj_forms = {}
+    for af in ("bf16x3", "fp16x3"):
+        if af == aform:
+            ams, aerr = adj_ms, adj_err
+        else:
+            dp.autotune_split(af)
+            ams, _, ga = time_adjoint(dp, D32, max(5, args.steps // 4), 3, af)
+            aerr = float((ga.double() - g32.double()).abs().max() / g32.double().abs().max())
+        adj_forms[af] = {"ms": ams, "max_rel_diff_vs_fp32_adjoint": aerr}
     # forward + objective + adjoint as one planning-step call, for the headline form and FP32
     step_ms, _ = time_planning_step(dp, max(5, args.steps // 4), 3, args.precision, aform)
     step32_ms, _ = time_planning_step(dp, max(5, args.steps // 4), 3, "fp32", "fp32")
@@ -462,6 +473,7 @@ def main():
         "adjoint_launch_ms": adj_ev,
         "adjoint_launch_ms_fp32": adj32_ev,
         "adjoint_max_rel_diff_vs_fp32_adjoint": adj_err,
+        "adjoint_forms": adj_forms,
         "fwd_adj_ms": ms_step + adj_ms,
         "fwd_adj_ms_fp32": ms32 + adj32_ms,
         "fwd_adj_speedup_vs_fp32": (ms32 + adj32_ms) / (ms_step + adj_ms),

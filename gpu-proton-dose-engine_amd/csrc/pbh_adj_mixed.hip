@@ -43,11 +43,17 @@ struct AmTab {
     static constexpr int N = QC ? table_slots(2 * (T - 1) + QC * (K - 1) + 1) : FT_TAB;
 };
 
-template <int MODE, int QC>
+// T = __bf16 or _Float16. FP16 has five exponent bits, so each wave scales its rows of
+// a tile group's R by the power of two 2^-e_w that puts their largest |R| in [1, 2)
+// before the split (then |Q| <= 2 x 128 voxels). Stage 1 brings every wave's Q rows to
+// the group's common scale 2^-e (e the largest e_w) before Q's split, and the group's
+// c P partial is scaled back by 2^e. All scalings are exact powers of two, so the
+// gradient does not depend on the scale of R.
+template <typename T, int MODE, int QC>
 __device__ __forceinline__ void adj_mixed_body(const AdjArgs& A, int blk)
 {
-    typedef __bf16 T;
-    typedef v8bf V8;
+    typedef typename Vec8<T>::type V8;
+    constexpr bool F16 = !__is_same(T, __bf16);
     constexpr bool SPLIT = MODE != MODE_PLAIN;
     constexpr int TXN = AmTab<QC, AM_TX, AM_SC>::N, TYN = AmTab<QC, AM_TY, AM_SC>::N;
     const FwdArgs& a = A.f;
@@ -67,6 +73,7 @@ __device__ __forceinline__ void adj_mixed_body(const AdjArgs& A, int blk)
     // rows); each lo | hi << 4, steps lo .. hi - 1
     __shared__ unsigned char awin[2][AM_MAXK][9];
     __shared__ int wcnt[FT_THREADS / 64];
+    __shared__ int rexp[F16 ? FT_THREADS / 64 : 1];      // FP16: e_w of each wave's R rows
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, c16 = lane & 15;
@@ -167,10 +174,9 @@ __device__ __forceinline__ void adj_mixed_body(const AdjArgs& A, int blk)
                         awin[par][q][w] = first > last ? 0 : (unsigned char)((first >> 5) | ((last / 32 + 1) << 4));
                     }
                     V8 rh[AM_NT][4], rl[AM_NT][4];
-#pragma unroll
-                    for (int u = 0; u < AM_NT; ++u) {
+                    // R of tile u, this wave's row: columns 32 ks + 8 g + e
+                    auto load_r = [&](int u, float (&rv)[4][8]) {
                         const int y = (tg + u) * AM_TY + 16 * wave + c16;
-                        float rv[4][8];
                         if (u < nt && interior_x && y < a.ny) {
                             // whole 128-column row segment inside the grid, 16-byte aligned
                             const float4* src = (const float4*)(Rz + (size_t)y * a.nx + x0 + 8 * g);
@@ -189,6 +195,8 @@ __device__ __forceinline__ void adj_mixed_body(const AdjArgs& A, int blk)
                                     rv[ks][e] = (u < nt && y < a.ny && x < a.nx) ? Rz[(size_t)y * a.nx + x] : 0.0f;
                                 }
                         }
+                    };
+                    auto split_r = [&](int u, const float (&rv)[4][8]) {
 #pragma unroll
                         for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
@@ -198,6 +206,38 @@ __device__ __forceinline__ void adj_mixed_body(const AdjArgs& A, int blk)
                                 rh[u][ks][e] = h;
                                 rl[u][ks][e] = l;
                             }
+                    };
+                    int eR = 0;     // FP16: this wave's e_w
+                    if (!F16) {
+#pragma unroll
+                        for (int u = 0; u < AM_NT; ++u) {
+                            float rv[4][8];
+                            load_r(u, rv);
+                            split_r(u, rv);
+                        }
+                    } else {
+                        float rv[AM_NT][4][8];
+                        float m = 0.0f;
+#pragma unroll
+                        for (int u = 0; u < AM_NT; ++u) {
+                            load_r(u, rv[u]);
+#pragma unroll
+                            for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+                                for (int e = 0; e < 8; ++e) m = fmaxf(m, fabsf(rv[u][ks][e]));
+                        }
+                        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+                        eR = (m > 0.0f && m <= 3.0e38f) ? ilogbf(m) : -1000;
+                        if (lane == 0) rexp[wave] = eR;     // read after the tables' barrier below
+                        const float sc = eR > -1000 ? ldexpf(1.0f, -eR) : 0.0f;
+#pragma unroll
+                        for (int u = 0; u < AM_NT; ++u) {
+#pragma unroll
+                            for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+                                for (int e = 0; e < 8; ++e) rv[u][ks][e] *= sc;
+                            split_r(u, rv[u]);
+                        }
                     }
                     const TableRow rowx = table_row(gx, Cx, AM_TX);
                     TableRow rowy[AM_NT];
@@ -240,6 +280,14 @@ __device__ __forceinline__ void adj_mixed_body(const AdjArgs& A, int blk)
                     __syncthreads();            // the previous group's readers of buffer 0 are done
                     build(0, 0);
                     __syncthreads();
+                    // FP16: the group's scale e and this wave's factor 2^(e_w - e) for its Q rows
+                    int eG = 0;
+                    float qsc = 1.0f;
+                    if (F16) {
+                        eG = max(max(rexp[0], rexp[1]), max(rexp[2], rexp[3]));
+                        qsc = eR > -1000 ? ldexpf(1.0f, eR - eG) : 0.0f;
+                        if (eG == -1000) eG = 0;             // the whole group's R is zero
+                    }
                     // per layer q: stage 1 (Tx[b] -> Q), barrier, stage 2 (Ty[b], Q) with the
                     // tables of layer q + 1 built into the other buffer, barrier
                     for (int q = 0; q < nmine; ++q) {
@@ -280,8 +328,8 @@ __device__ __forceinline__ void adj_mixed_body(const AdjArgs& A, int blk)
 #pragma unroll
                                 for (int h2 = 0; h2 < 2; ++h2) {
                                     T a0, b0, a1, b1;
-                                    split<T>(acc1[u][2 * h2], a0, b0);
-                                    split<T>(acc1[u][2 * h2 + 1], a1, b1);
+                                    split<T>(F16 ? acc1[u][2 * h2] * qsc : acc1[u][2 * h2], a0, b0);
+                                    split<T>(F16 ? acc1[u][2 * h2 + 1] * qsc : acc1[u][2 * h2 + 1], a1, b1);
                                     hv[h2] = (unsigned)bits16(a0) | ((unsigned)bits16(a1) << 16);
                                     lv[h2] = (unsigned)bits16(b0) | ((unsigned)bits16(b1) << 16);
                                 }
@@ -294,7 +342,7 @@ __device__ __forceinline__ void adj_mixed_body(const AdjArgs& A, int blk)
                         // stage 2: P[j][i] blocks (mb, nb) = wave + 4 qq of the 3 x 3, summed over
                         // the group's tiles, joined to the slice's partial once
                         float* pz = A.part + ((size_t)z * a.L + lay[q]) * a.nsy * a.nsx;
-                        const double cn = lcn[q];
+                        const double cn = F16 ? ldexp(lcn[q], eG) : lcn[q];     // FP16: R's scale undone
 #pragma unroll 1
                         for (int qq = 0; qq < 3; ++qq) {
                             const int b2 = wave + 4 * qq;
@@ -341,18 +389,20 @@ __device__ __forceinline__ void adj_mixed_body(const AdjArgs& A, int blk)
     }
 }
 
-template <int MODE, int QC>
-__global__ __launch_bounds__(FT_THREADS, 2)
+// FP16 split products on the tabulated geometry: held to the registers of 3 waves per
+// SIMD, which the BF16 kernel has by itself
+template <typename T, int MODE, int QC>
+__global__ __launch_bounds__(FT_THREADS, (!__is_same(T, __bf16) && MODE == MODE_X3 && QC > 0 ? 3 : 2))
 void adj_mixed_kernel(AdjArgs A)
 {
     int z, sh;
     xcd_unit((int)blockIdx.x, A.split, z, sh);
     if (z >= A.f.nz) return;
-    adj_mixed_body<MODE, QC>(A, z * A.split + sh);
+    adj_mixed_body<T, MODE, QC>(A, z * A.split + sh);
 }
 
-template <int MODE, int QC>
-__global__ __launch_bounds__(FT_THREADS, 2)
+template <typename T, int MODE, int QC>
+__global__ __launch_bounds__(FT_THREADS, (!__is_same(T, __bf16) && MODE == MODE_X3 && QC > 0 ? 3 : 2))
 void adj_mixed_batch_kernel(const AdjArgs* __restrict__ items, const int* __restrict__ wg0, int n)
 {
     const int k = batch_item(wg0, n, blockIdx.x);
@@ -360,7 +410,7 @@ void adj_mixed_batch_kernel(const AdjArgs* __restrict__ items, const int* __rest
     int z, sh;     // item ranges start at multiples of 8: launch index b and b - wg0[k] share an XCD
     xcd_unit((int)blockIdx.x - wg0[k], A.split, z, sh);
     if (z >= A.f.nz) return;
-    adj_mixed_body<MODE, QC>(A, z * A.split + sh);
+    adj_mixed_body<T, MODE, QC>(A, z * A.split + sh);
 }
 
 }  // namespace pbh
@@ -386,14 +436,26 @@ int pbh_launch_adj_mixed(const AdjArgs& A, int precision, int nz, hipStream_t s)
 {
     if (!split_fits(A)) return PBH_ERR_UNSUPPORTED;
     const dim3 grid((unsigned)xcd_grid(nz, A.split)), block(FT_THREADS);
-    const bool x3 = precision != PBH_PREC_BF16;
     const bool q5 = pbh_adj_mixed_q5(A);
-    if (x3) {
-        if (q5) hipLaunchKernelGGL((adj_mixed_kernel<MODE_X3, 5>), grid, block, 0, s, A);
-        else hipLaunchKernelGGL((adj_mixed_kernel<MODE_X3, 0>), grid, block, 0, s, A);
-    } else {
-        if (q5) hipLaunchKernelGGL((adj_mixed_kernel<MODE_PLAIN, 5>), grid, block, 0, s, A);
-        else hipLaunchKernelGGL((adj_mixed_kernel<MODE_PLAIN, 0>), grid, block, 0, s, A);
+    switch (precision) {
+    case PBH_PREC_BF16X3:
+        if (q5) hipLaunchKernelGGL((adj_mixed_kernel<__bf16, MODE_X3, 5>), grid, block, 0, s, A);
+        else hipLaunchKernelGGL((adj_mixed_kernel<__bf16, MODE_X3, 0>), grid, block, 0, s, A);
+        break;
+    case PBH_PREC_BF16:
+        if (q5) hipLaunchKernelGGL((adj_mixed_kernel<__bf16, MODE_PLAIN, 5>), grid, block, 0, s, A);
+        else hipLaunchKernelGGL((adj_mixed_kernel<__bf16, MODE_PLAIN, 0>), grid, block, 0, s, A);
+        break;
+    case PBH_PREC_FP16X3:
+        if (q5) hipLaunchKernelGGL((adj_mixed_kernel<_Float16, MODE_X3, 5>), grid, block, 0, s, A);
+        else hipLaunchKernelGGL((adj_mixed_kernel<_Float16, MODE_X3, 0>), grid, block, 0, s, A);
+        break;
+    case PBH_PREC_FP16:
+        if (q5) hipLaunchKernelGGL((adj_mixed_kernel<_Float16, MODE_PLAIN, 5>), grid, block, 0, s, A);
+        else hipLaunchKernelGGL((adj_mixed_kernel<_Float16, MODE_PLAIN, 0>), grid, block, 0, s, A);
+        break;
+    default:
+        return PBH_ERR_UNSUPPORTED;
     }
     return launch_rc("adj_mixed_kernel");
 }
@@ -407,13 +469,25 @@ int pbh_launch_adj_mixed_batch(const AdjArgs* items, const int* wg0, int n, int 
                                hipStream_t s)
 {
     const dim3 grid((unsigned)total), block(FT_THREADS);
-    const bool x3 = precision != PBH_PREC_BF16;
-    if (x3) {
-        if (q5) hipLaunchKernelGGL((adj_mixed_batch_kernel<MODE_X3, 5>), grid, block, 0, s, items, wg0, n);
-        else hipLaunchKernelGGL((adj_mixed_batch_kernel<MODE_X3, 0>), grid, block, 0, s, items, wg0, n);
-    } else {
-        if (q5) hipLaunchKernelGGL((adj_mixed_batch_kernel<MODE_PLAIN, 5>), grid, block, 0, s, items, wg0, n);
-        else hipLaunchKernelGGL((adj_mixed_batch_kernel<MODE_PLAIN, 0>), grid, block, 0, s, items, wg0, n);
+    switch (precision) {
+    case PBH_PREC_BF16X3:
+        if (q5) hipLaunchKernelGGL((adj_mixed_batch_kernel<__bf16, MODE_X3, 5>), grid, block, 0, s, items, wg0, n);
+        else hipLaunchKernelGGL((adj_mixed_batch_kernel<__bf16, MODE_X3, 0>), grid, block, 0, s, items, wg0, n);
+        break;
+    case PBH_PREC_BF16:
+        if (q5) hipLaunchKernelGGL((adj_mixed_batch_kernel<__bf16, MODE_PLAIN, 5>), grid, block, 0, s, items, wg0, n);
+        else hipLaunchKernelGGL((adj_mixed_batch_kernel<__bf16, MODE_PLAIN, 0>), grid, block, 0, s, items, wg0, n);
+        break;
+    case PBH_PREC_FP16X3:
+        if (q5) hipLaunchKernelGGL((adj_mixed_batch_kernel<_Float16, MODE_X3, 5>), grid, block, 0, s, items, wg0, n);
+        else hipLaunchKernelGGL((adj_mixed_batch_kernel<_Float16, MODE_X3, 0>), grid, block, 0, s, items, wg0, n);
+        break;
+    case PBH_PREC_FP16:
+        if (q5) hipLaunchKernelGGL((adj_mixed_batch_kernel<_Float16, MODE_PLAIN, 5>), grid, block, 0, s, items, wg0, n);
+        else hipLaunchKernelGGL((adj_mixed_batch_kernel<_Float16, MODE_PLAIN, 0>), grid, block, 0, s, items, wg0, n);
+        break;
+    default:
+        return PBH_ERR_UNSUPPORTED;
     }
     return launch_rc("adj_mixed_batch_kernel");
 }

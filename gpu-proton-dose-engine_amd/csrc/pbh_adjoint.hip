@@ -550,21 +550,30 @@ static int model_split(int nz, int L, const int32_t* active, long long slots, in
     return best;
 }
 
+// The adjoints with 16-bit operands (adj_mixed_kernel): BF16 or FP16, single or split
+// products.
+static bool adj_16bit(int precision)
+{
+    return precision == PBH_PREC_BF16 || precision == PBH_PREC_BF16X3 || precision == PBH_PREC_FP16 ||
+           precision == PBH_PREC_FP16X3;
+}
+
 extern "C" int pbh_adjoint_choose_split(const PbhGrid* g, const PbhLattice* lat, int precision,
                                         const int32_t* active_per_z, int compute_units)
 {
     int rc = pbh_validate(g, lat);
     if (rc != PBH_OK) return rc;
     if (!active_per_z) return PBH_ERR_NULL_ARG;
-    const bool mixed = precision == PBH_PREC_BF16 || precision == PBH_PREC_BF16X3;
+    const bool mixed = adj_16bit(precision);
     if (precision != PBH_PREC_FP32 && !mixed) return PBH_ERR_UNSUPPORTED;
-    // resident workgroups per CU: the FP32 kernel 2; the mixed kernels on the P = 2, Q = 5
-    // table geometry 3 (split products, 152 VGPRs) or 4 (single products, 112 VGPRs), else 2
+    // resident workgroups per CU: the FP32 kernel 2; the 16-bit kernels on the P = 2, Q = 5
+    // table geometry 3 (split products: 156 VGPRs BF16, 168 FP16), 4 (BF16 single products,
+    // 124 VGPRs) or 2 (FP16 single products, 185 VGPRs), else 2
     int per_cu = 2;
     AxisTable tx, ty;
     if (mixed && adj_axis(g->dx, lat->dx_spot, AJ_TX, AJ_SC, &tx) && adj_axis(g->dy, lat->dy_spot, AJ_TY, AJ_SC, &ty) &&
         tx.P == 2 && tx.Q == 5 && ty.P == 2 && ty.Q == 5)
-        per_cu = precision == PBH_PREC_BF16X3 ? 3 : 4;
+        per_cu = precision == PBH_PREC_BF16 ? 4 : precision == PBH_PREC_FP16 ? 2 : 3;
     // the mixed kernel keeps at most AM_MAXK (128) layers of a slice per workgroup
     const int min_split = mixed ? (lat->n_layers + 127) / 128 : 1;
     if (min_split > AJ_MAX_SPLIT) return PBH_ERR_UNSUPPORTED;
@@ -635,7 +644,7 @@ static int prepare_adjoint(const PbhGrid* g, const PbhLattice* lat, const float*
 {
     int rc = pbh_make_adj_args(g, lat, w, idd, sigma, R, part, grad_idd, grad_sigma, split, A);
     if (rc != PBH_OK) return rc;
-    const bool mixed = precision == PBH_PREC_BF16 || precision == PBH_PREC_BF16X3;
+    const bool mixed = adj_16bit(precision);
     // the BF16 adjoint gathers every factor from tables and keeps at most 128 layers per workgroup
     if (mixed && (!A->tab || !pbh_adj_mixed_fits(*A))) return PBH_ERR_UNSUPPORTED;
     return PBH_OK;
@@ -648,7 +657,7 @@ static int adjoint_launch(const PbhGrid* g, const PbhLattice* lat, const AdjArgs
                           float* grad_w, double* grad_sigma, int precision, const double* jpart, int njpart,
                           double* j_out, hipStream_t s)
 {
-    const bool mixed = precision == PBH_PREC_BF16 || precision == PBH_PREC_BF16X3;
+    const bool mixed = adj_16bit(precision);
     int rc;
     const size_t n = (size_t)lat->n_layers * lat->n_sy * lat->n_sx;
     if (mixed) {
@@ -683,7 +692,7 @@ extern "C" int pbh_adjoint_tuned(const PbhGrid* g, const PbhLattice* lat, const 
     int rc = pbh_validate(g, lat);
     if (rc != PBH_OK) return rc;
     if (!w || !idd || !sigma || !R || !grad_w || !workspace) return PBH_ERR_NULL_ARG;
-    const bool mixed = precision == PBH_PREC_BF16 || precision == PBH_PREC_BF16X3;
+    const bool mixed = adj_16bit(precision);
     if (precision != PBH_PREC_FP32 && !mixed) return PBH_ERR_UNSUPPORTED;
     if (mixed && (grad_idd || grad_sigma)) return PBH_ERR_UNSUPPORTED;   // table gradients: FP32 only
     if (split < 0) return PBH_ERR_BAD_DIM;
@@ -747,7 +756,7 @@ extern "C" int pbh_objective_adjoint(const PbhGrid* g, const PbhLattice* lat, co
     int rc = pbh_validate(g, lat);
     if (rc != PBH_OK) return rc;
     if (!w || !idd || !sigma || !dose || !presc || !grad_w || !j_out || !workspace) return PBH_ERR_NULL_ARG;
-    const bool mixed = precision == PBH_PREC_BF16 || precision == PBH_PREC_BF16X3;
+    const bool mixed = adj_16bit(precision);
     if (precision != PBH_PREC_FP32 && !mixed) return PBH_ERR_UNSUPPORTED;
     if (split < 0) return PBH_ERR_BAD_DIM;
     if (workspace_bytes < pbh_objective_adjoint_workspace_bytes(g, lat, precision)) return PBH_ERR_WORKSPACE;
@@ -789,7 +798,7 @@ extern "C" int pbh_forward_objective_adjoint(const PbhGrid* g, const PbhLattice*
     int rc = pbh_validate(g, lat);
     if (rc != PBH_OK) return rc;
     if (!w || !idd || !sigma || !dose || !presc || !grad_w || !j_out || !workspace) return PBH_ERR_NULL_ARG;
-    const bool mixed = adj_precision == PBH_PREC_BF16 || adj_precision == PBH_PREC_BF16X3;
+    const bool mixed = adj_16bit(adj_precision);
     if (adj_precision != PBH_PREC_FP32 && !mixed) return PBH_ERR_UNSUPPORTED;
     if (split < 0 || layer_chunk < 0) return PBH_ERR_BAD_DIM;
     const size_t need = pbh_forward_objective_adjoint_workspace_bytes(g, lat, fwd_precision, layer_chunk,

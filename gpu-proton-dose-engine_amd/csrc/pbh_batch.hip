@@ -278,9 +278,13 @@ extern "C" int pbh_batch_create(const PbhBatchItem* items, int n, int precision,
     b->n = n;
     b->precision = precision;
     b->peaks = is_peak(precision);
-    // the adjoint follows the forward's arithmetic: FP32, BF16 (plain forms) or BF16 x3
+    // the adjoint follows the forward's arithmetic: FP32, or the forward's 16-bit type with
+    // single products (plain forms) or split products (x3 and near-peak forms)
     b->adj_prec = precision == PBH_PREC_FP32 ? PBH_PREC_FP32
-                  : (precision == PBH_PREC_BF16 || precision == PBH_PREC_FP16) ? PBH_PREC_BF16 : PBH_PREC_BF16X3;
+                  : precision == PBH_PREC_BF16 ? PBH_PREC_BF16
+                  : precision == PBH_PREC_FP16 ? PBH_PREC_FP16
+                  : (precision == PBH_PREC_FP16X3 || precision == PBH_PREC_FP16X3_PEAK) ? PBH_PREC_FP16X3
+                                                                                          : PBH_PREC_BF16X3;
     b->fwd = make_groups(keys, tabs);
     for (int k = 0; k < n; ++k)
         b->max_wchunks = std::max(b->max_wchunks, items[k].lattice.n_layers * chunk_count(items[k].lattice.n_sy) *
@@ -335,7 +339,7 @@ extern "C" int pbh_batch_create(const PbhBatchItem* items, int n, int precision,
         ha.push_back(A);
         b->sig = b->sig || it.grad_sigma != nullptr;
         akeys.push_back(0);
-        // adjoint mode: bit 0 table factors, bit 1 BF16 kernel, bit 2 P = 2, Q = 5 slots
+        // adjoint mode: bit 0 table factors, bit 1 16-bit kernel, bit 2 P = 2, Q = 5 slots
         const bool mix = b->adj_prec != PBH_PREC_FP32 && A.tab && !it.grad_idd && !it.grad_sigma &&
                          pbh_adj_mixed_fits(A);
         atabs.push_back((char)((mix ? 3 : (A.tab ? 1 : 0)) | (pbh_adj_mixed_q5(A) ? 4 : 0)));

@@ -178,8 +178,8 @@ class DeviceProblem:
         return self._tuned[p]
 
     def autotune_split(self, precision: str, reps: int = 20, use_cache: bool = True) -> int:
-        """Measured layer split of the weight adjoint (``precision`` 'fp32', 'bf16' or
-        'bf16x3'): the dispatch model's split and its neighbours (splits 2 to 16) are
+        """Measured layer split of the weight adjoint (``precision`` 'fp32', 'bf16',
+        'bf16x3', 'fp16' or 'fp16x3'): the dispatch model's split and its neighbours (splits 2 to 16) are
         timed over `reps` objective + adjoint calls, and the fastest is used by later
         adjoint calls with split 0. Cached like ``autotune``. Every split gives the same
         dJ/dw bit for bit."""

@@ -117,16 +117,22 @@ def objective(dp: DeviceProblem, dose: torch.Tensor, presc: torch.Tensor,
     return j[0], R
 
 
+# Arithmetic of the weight adjoint: FP32, or 16-bit MFMA operands (BF16 or FP16) with
+# single or split (x3) products and FP32 accumulation
+ADJOINT_FORMS = ("fp32", "bf16", "bf16x3", "fp16", "fp16x3")
+
+
 def adjoint(dp: DeviceProblem, R: torch.Tensor, w: Optional[torch.Tensor] = None,
             lut_grads: bool = False, split: int = 0, precision: str = "fp32"):
     """dJ/dw (L, nsy, nsx) FP32 for the dose cotangent R; with ``lut_grads`` also
     dJ/dIDD and dJ/dsigma (L, nz) FP64 (FP32 arithmetic only). ``split``:
     workgroups per depth slice sharing its layers (0 = automatic).
-    ``precision``: 'fp32', or 'bf16' / 'bf16x3' (BF16 MFMA, dJ/dw only). Without table
+    ``precision``: 'fp32', or 'bf16' / 'bf16x3' / 'fp16' / 'fp16x3' (16-bit MFMA, dJ/dw
+    only; FP16 scales R by a power of two per tile group, exactly). Without table
     gradients, ``split`` 0 takes the split that pbh_adjoint_choose_split models fastest
     for this problem's active layers per slice."""
-    if precision not in ("fp32", "bf16", "bf16x3"):
-        raise ValueError(f"adjoint precision {precision!r}: 'fp32', 'bf16' or 'bf16x3'")
+    if precision not in ADJOINT_FORMS:
+        raise ValueError(f"adjoint precision {precision!r}: one of {', '.join(ADJOINT_FORMS)}")
     if lut_grads and precision != "fp32":
         raise ValueError("table gradients (lut_grads) are computed in FP32 only")
     L = _lib.lib()
@@ -152,8 +158,8 @@ def objective_gradient(dp: DeviceProblem, dose: torch.Tensor, presc: torch.Tenso
                        split: int = 0, precision: str = "fp32"):
     """(J, dJ/dw) of J = sum wv (D - Dp)^2 in one call (pbh_objective_adjoint): the same
     values as ``objective`` followed by ``adjoint``, with one kernel launch fewer."""
-    if precision not in ("fp32", "bf16", "bf16x3"):
-        raise ValueError(f"adjoint precision {precision!r}: 'fp32', 'bf16' or 'bf16x3'")
+    if precision not in ADJOINT_FORMS:
+        raise ValueError(f"adjoint precision {precision!r}: one of {', '.join(ADJOINT_FORMS)}")
     for name, t in (("dose", dose), ("presc", presc), ("vox_w", vox_w)):
         if t is not None and (t.dtype != torch.float32 or tuple(t.shape) != tuple(dp.shape) or
                               not t.is_contiguous()):
@@ -181,10 +187,10 @@ def forward_objective_gradient(dp: DeviceProblem, presc: torch.Tensor, vox_w: Op
     of w, J = sum wv (D - Dp)^2 and dJ/dw. Returns (D, J, dJ/dw); D and dJ/dw equal
     ``dp.forward`` followed by ``objective_gradient`` bit for bit (J up to the grouping
     of its FP64 sum), with the objective formed inside the forward's combine.
-    ``precision`` is the adjoint's ('fp32', 'bf16', 'bf16x3'); the forward's defaults
+    ``precision`` is the adjoint's (one of ADJOINT_FORMS); the forward's defaults
     to it ('fp32' forward for an 'fp32' adjoint)."""
-    if precision not in ("fp32", "bf16", "bf16x3"):
-        raise ValueError(f"adjoint precision {precision!r}: 'fp32', 'bf16' or 'bf16x3'")
+    if precision not in ADJOINT_FORMS:
+        raise ValueError(f"adjoint precision {precision!r}: one of {', '.join(ADJOINT_FORMS)}")
     fwd = forward_precision or precision
     for name, t in (("presc", presc), ("vox_w", vox_w), ("out", out)):
         if t is not None and (t.dtype != torch.float32 or tuple(t.shape) != tuple(dp.shape) or

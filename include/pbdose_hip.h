@@ -300,8 +300,10 @@ size_t pbh_adjoint_workspace_bytes(const PbhGrid* g, const PbhLattice* lat, int 
 
 /* grad_w (L, nsy, nsx) FP32 := sum_z IDD_l(z) Gy_l(z)^T R_z Gx_l(z) for a dose
  * cotangent R (nz, ny, nx). Deterministic (no atomics). precision:
- * PBH_PREC_FP32, or PBH_PREC_BF16 / PBH_PREC_BF16X3 (BF16 MFMA, plain or split
- * products, FP32 accumulation) when no table gradient is requested and the
+ * PBH_PREC_FP32, or PBH_PREC_BF16 / PBH_PREC_BF16X3 / PBH_PREC_FP16 /
+ * PBH_PREC_FP16X3 (16-bit MFMA, plain or split products, FP32 accumulation; FP16
+ * with a power-of-two scale of R per tile group, so any scale of R is exact)
+ * when no table gradient is requested and the
  * voxel and spot pitches have a small integer ratio (table geometry); other
  * combinations return PBH_ERR_UNSUPPORTED.
  * If grad_sigma != NULL it also writes grad_sigma (L, nz) FP64:
@@ -324,8 +326,8 @@ int pbh_adjoint_tuned(const PbhGrid* g, const PbhLattice* lat,
  * of every depth slice, active_per_z[nz] (host memory): simulates the dispatch
  * of the adjoint's workgroups onto the resident slots (compute_units CUs, <= 0:
  * the current device's) and returns the split (1 to 16) with the shortest
- * makespan, or a negative PBH_ERR_* code. precision: PBH_PREC_FP32, PBH_PREC_BF16
- * or PBH_PREC_BF16X3. Host only; no counterpart in the reference (its engines are
+ * makespan, or a negative PBH_ERR_* code. precision: PBH_PREC_FP32, PBH_PREC_BF16,
+ * PBH_PREC_BF16X3, PBH_PREC_FP16 or PBH_PREC_FP16X3. Host only; no counterpart in the reference (its engines are
  * forward-only). */
 int pbh_adjoint_choose_split(const PbhGrid* g, const PbhLattice* lat, int precision,
                              const int32_t* active_per_z, int compute_units);
@@ -433,9 +435,10 @@ int pbh_batch_create(const PbhBatchItem* items, int n_items, int precision,
                      const PbhBatchTuning* tuning, void* workspace, size_t workspace_bytes,
                      void* stream, PbhBatch** out);
 
-/* The adjoint of a batch follows `precision`: FP32 for PBH_PREC_FP32, BF16 for
- * the plain 16-bit forms, BF16 split products for the x3 forms; items that
- * request table gradients or have no table geometry use the FP32 adjoint. */
+/* The adjoint of a batch follows `precision`: FP32 for PBH_PREC_FP32, otherwise
+ * the forward's 16-bit type (BF16 or FP16) with single products for the plain
+ * forms and split products for the x3 and near-peak forms; items that request
+ * table gradients or have no table geometry use the FP32 adjoint. */
 
 /* Runs the whole batch on `stream`: kernel launches only (no copies, memsets
  * or allocations), so it can be captured into a HIP graph and replayed.

@@ -301,8 +301,9 @@ def test_objective_and_adjoint_reject_bad_arguments_without_launching():
     args = lambda prec, gi, split, nbytes: (g, lat, one, one, one, one, one, gi, nul, prec, split, one, nbytes, nul)
     assert L.pbh_adjoint_tuned(g, lat, nul, one, one, one, one, nul, nul, 0, 0, one, ab, nul) == -1   # no weights
     assert L.pbh_adjoint_tuned(*args(99, nul, 0, ab)) == -5                        # unknown precision
-    assert L.pbh_adjoint_tuned(*args(P["fp16"], nul, 0, ab)) == -5                 # no FP16 adjoint
+    assert L.pbh_adjoint_tuned(*args(P["fp16x3_peak"], nul, 0, ab)) == -5          # no near-peak adjoint
     assert L.pbh_adjoint_tuned(*args(P["bf16x3"], one, 0, ab)) == -5               # table gradients: FP32 only
+    assert L.pbh_adjoint_tuned(*args(P["fp16x3"], one, 0, ab)) == -5
     assert L.pbh_adjoint_tuned(*args(P["bf16x3"], nul, -1, ab)) == -2              # negative split
     assert L.pbh_adjoint_tuned(*args(P["bf16x3"], nul, 0, ab - 1)) == -7           # workspace too small
 
@@ -333,8 +334,11 @@ def test_adjoint_split_chooser():
     act = np.ascontiguousarray((idd != 0).sum(0).astype(np.int32))
     P = _lib.PREC
     assert L.pbh_adjoint_choose_split(gs, ls, P["bf16x3"], None, 256) == -1
-    assert L.pbh_adjoint_choose_split(gs, ls, P["fp16"], act.ctypes.data, 256) == -5
+    assert L.pbh_adjoint_choose_split(gs, ls, P["fp16x3_peak"], act.ctypes.data, 256) == -5
     assert L.pbh_adjoint_choose_split(gs, ls, P["bf16x3"], act.ctypes.data, 256) == 7
+    # the FP16 split kernel has the BF16 one's three workgroups per CU
+    assert L.pbh_adjoint_choose_split(gs, ls, P["fp16x3"], act.ctypes.data, 256) == 7
+    assert 1 <= L.pbh_adjoint_choose_split(gs, ls, P["fp16"], act.ctypes.data, 256) <= 16
     assert L.pbh_adjoint_choose_split(gs, ls, P["bf16"], act.ctypes.data, 256) == 10
     assert L.pbh_adjoint_choose_split(gs, ls, P["fp32"], act.ctypes.data, 256) == 5
     # every slice with all layers active on 8 CUs: more workgroups per slice only add
@@ -355,7 +359,7 @@ def test_objective_adjoint_rejects_bad_arguments_without_launching():
     args = lambda jout, prec, split, nbytes: (g, lat, one, one, one, one, one, nul, one, jout, prec, split,
                                               one, nbytes, nul)
     assert L.pbh_objective_adjoint(*args(nul, P["bf16x3"], 0, nb)) == -1          # no J output
-    assert L.pbh_objective_adjoint(*args(one, P["fp16"], 0, nb)) == -5            # no FP16 adjoint
+    assert L.pbh_objective_adjoint(*args(one, P["fp16x3_peak"], 0, nb)) == -5     # no near-peak adjoint
     assert L.pbh_objective_adjoint(*args(one, P["bf16x3"], -1, nb)) == -2         # negative split
     assert L.pbh_objective_adjoint(*args(one, P["bf16x3"], 0, nb - 1)) == -7      # workspace too small
     assert L.pbh_objective_adjoint(*args(one, P["bf16x3"], 17, nb)) == -5         # split above 16

@@ -217,14 +217,14 @@ def test_autograd_matches_torch_restatement_of_reference_engine():
     assert e <= 2e-5
 
 
-MIXED_ADJ_TOL = {"bf16x3": 3e-5, "bf16": 1e-2}
+MIXED_ADJ_TOL = {"bf16x3": 3e-5, "bf16": 1e-2, "fp16x3": 5e-6, "fp16": 3e-3}
 
 
-@pytest.mark.parametrize("precision", ["bf16x3", "bf16"])
+@pytest.mark.parametrize("precision", ["bf16x3", "bf16", "fp16x3", "fp16"])
 @pytest.mark.parametrize("geom", [(40, 36, 24, 11, 9, 0.2, 0.5), (70, 150, 6, 60, 50, 0.25, 0.3),
                                   (130, 65, 5, 7, 13, 0.2, 0.5)])
 def test_mixed_adjoint_matches_oracle(precision, geom):
-    """BF16 MFMA adjoint (dJ/dw): 2 mm / 5 mm pitch (compile-time table slots),
+    """16-bit MFMA adjoint (dJ/dw): 2 mm / 5 mm pitch (compile-time table slots),
     2.5 mm / 3 mm (run-time slots), ragged tiles and multi-chunk lattices."""
     from pbdose import engines, planning
     nx, ny, nz, nsx, nsy, d, ds = geom
@@ -243,6 +243,36 @@ def test_mixed_adjoint_matches_oracle(precision, geom):
     assert e <= MIXED_ADJ_TOL[precision]
 
 
+@pytest.mark.parametrize("precision", ["fp16x3", "fp16", "bf16x3"])
+def test_16bit_adjoint_at_any_residual_scale(precision):
+    """R scaled by 2^k gives dJ/dw scaled by 2^k, bit for bit: FP16 scales each wave's
+    rows of R by a power of two and undoes it exactly. Against the FP32 adjoint: R
+    whose rows 16-31 of every 64-row tile (one wave's rows) are 32 times larger (waves
+    with different scales, brought to one common scale in stage 1), and R whose first
+    or last tile group is 2^30 times larger (a scale per group)."""
+    from pbdose import engines, planning
+    print(f"case 16-bit adjoint at any residual scale {precision}", flush=True)
+    p = small_problem(nx=64, ny=150, nz=12, d=0.2, nsx=21, nsy=17, ds=0.5,
+                      energies=tuple(np.linspace(70.0, 150.0, 8)))
+    dp = engines.DeviceProblem(p)
+    R = torch.as_tensor(np.random.default_rng(9).standard_normal(p.grid.shape), dtype=torch.float32,
+                        device=dp.dev)
+    g0 = planning.adjoint(dp, R, precision=precision)
+    for k in (-40, -12, 17, 40):
+        gk = planning.adjoint(dp, R * 2.0 ** k, precision=precision)
+        assert torch.equal(gk, g0 * 2.0 ** k), k
+    wave1 = [y for y in range(p.grid.ny) if 16 <= y % 64 < 32]
+    for name, rows, f in (("one wave's rows x 32", wave1, 32.0),
+                          ("tiles 0-1 x 2^30", list(range(0, 128)), 2.0 ** 30),
+                          ("tile 2 x 2^30", list(range(128, p.grid.ny)), 2.0 ** 30)):
+        Rm = R.clone()
+        Rm[:, rows, :] *= f
+        g32 = planning.adjoint(dp, Rm)
+        e = rel(planning.adjoint(dp, Rm, precision=precision).cpu(), g32.cpu())
+        print(f"  {name}: vs FP32 adjoint {e:.2e}", flush=True)
+        assert e <= MIXED_ADJ_TOL[precision], name
+
+
 def test_mixed_adjoint_clinical_field_and_unsupported_cases():
     """128^3 / 41x41 / 40 layers, R of the weighted-quadratic objective: the BF16x3
     adjoint against the FP32 adjoint (itself within 2e-7 of the oracle); table
@@ -258,7 +288,7 @@ def test_mixed_adjoint_clinical_field_and_unsupported_cases():
     Dp, wv = planning.central_cube(grid)
     _, R = planning.objective(dp, dp.forward("bf16x3"), Dp, wv)
     g32 = planning.adjoint(dp, R)
-    for prec in ("bf16x3", "bf16"):
+    for prec in ("bf16x3", "bf16", "fp16x3", "fp16"):
         e = rel(planning.adjoint(dp, R, precision=prec).cpu(), g32.cpu())
         print(f"  {prec} vs FP32 adjoint: {e:.2e}", flush=True)
         assert e <= MIXED_ADJ_TOL[prec]
@@ -270,7 +300,7 @@ def test_mixed_adjoint_clinical_field_and_unsupported_cases():
         planning.adjoint(dq, torch.zeros(dq.shape, dtype=torch.float32, device=dq.dev), precision="bf16x3")
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16", "fp16x3", "fp16"])
 def test_adjoint_does_not_depend_on_the_split(precision):
     """dJ/dw is the same, bit for bit, for every layer split 1-16 (powers of two or
     not: each (slice, layer) partial is computed by one workgroup in a fixed order,
@@ -296,7 +326,7 @@ def test_adjoint_does_not_depend_on_the_split(precision):
         assert torch.equal(gw5, gw1) and torch.equal(gi5, gi1) and torch.equal(gs5, gs1)
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "fp16x3"])
 def test_objective_gradient_equals_objective_then_adjoint(precision):
     """pbh_objective_adjoint gives J and dJ/dw bit for bit as pbh_objective followed
     by pbh_adjoint_tuned, on repeated calls (the workspace is reused), with and
@@ -318,6 +348,7 @@ def test_objective_gradient_equals_objective_then_adjoint(precision):
 
 
 @pytest.mark.parametrize("fwd,adj,rows", [("fp32", "fp32", 32), ("bf16x3", "bf16x3", 32), ("bf16x3", "bf16x3", 64),
+                                           ("fp16x3", "fp16x3", 64),
                                            ("fp16x3", "bf16x3", 64), ("bf16", "bf16", 64)])
 def test_planning_step_in_one_call(fwd, adj, rows):
     """pbh_forward_objective_adjoint (the objective formed in the forward's combine over
@@ -352,8 +383,8 @@ def test_planning_step_rejects_bad_arguments():
     p = small_problem()
     dp = engines.DeviceProblem(p)
     Dp, wv = planning.central_cube(p.grid)
-    with pytest.raises(ValueError):
-        planning.forward_objective_gradient(dp, Dp, wv, precision="fp16x3")
+    with pytest.raises(ValueError):         # no near-peak adjoint
+        planning.forward_objective_gradient(dp, Dp, wv, precision="fp16x3_peak")
     with pytest.raises(ValueError):
         planning.forward_objective_gradient(dp, Dp[:1].contiguous(), wv)
     with pytest.raises(RuntimeError):       # 64-row tiles need the 2 : 5 pitch geometry

@@ -66,10 +66,12 @@ def oracle_item(field, sc, z_idx=None):
 
 
 def adjoint_form(precision):
-    """The adjoint arithmetic a batch pairs with a forward form."""
+    """The adjoint arithmetic a batch pairs with a forward form: the forward's 16-bit
+    type, single products for the plain forms, split products otherwise."""
     if precision == "fp32":
         return "fp32"
-    return "bf16" if precision in ("bf16", "fp16") else "bf16x3"
+    t = "fp16" if precision.startswith("fp16") else "bf16"
+    return t if precision in ("bf16", "fp16") else t + "x3"
 
 
 def single_path(b, k, precision):
@@ -125,7 +127,7 @@ def check_items(b, fields, scen, precision):
         e_sg = rel(b.grad_w[k].cpu(), g1.cpu())
         worst = [max(x, y) for x, y in zip(worst, (e_d, e_g, e_j, e_s, e_sg))]
         assert e_d <= tol, (k, e_d)
-        assert e_g <= (2e-5 if precision != "bf16" else 1e-2), (k, e_g)
+        assert e_g <= {"bf16": 1e-2, "fp16": 3e-3}.get(adjoint_form(precision), 2e-5), (k, e_g)
         assert e_j <= 1e-9, (k, e_j)
         assert e_s <= 1e-6 and e_sg <= 1e-6, (k, e_s, e_sg)
         assert abs(float(b.J[k]) - float(J1)) <= 1e-6 * abs(float(J1))
