@@ -332,6 +332,14 @@ def profile_kernels(forms, reps=5):
     prob, dp = build_field(dev)
     out = torch.empty(dp.shape, dtype=torch.float32, device=dev)
     for f in forms:
+        if f == "fp64":
+            out64 = torch.empty(dp.shape, dtype=torch.float64, device=dev)
+            w64 = dp.w.double().contiguous()
+            for _ in range(reps):
+                dp.forward_fp64(out=out64, w=w64)
+            torch.cuda.synchronize()
+            print(f"forward fp64: {reps} launches", flush=True)
+            continue
         # the shape bench.py times (measured, or read from $PBDOSE_AUTOTUNE_CACHE when a
         # previous run of this mode filled it, so that a profiled run times nothing else)
         chunk, rows = dp.autotune(f)
@@ -341,7 +349,7 @@ def profile_kernels(forms, reps=5):
         print(f"forward {f}: {reps} launches ({rows}-row tiles, layer chunk {chunk})", flush=True)
     Dp, wv = planning.central_cube(prob.grid, device=dev)
     D = dp.forward("fp32")
-    for aform in sorted({adjoint_form(f) for f in forms} | {"fp32"}):
+    for aform in sorted({adjoint_form(f) for f in forms if f != "fp64"} | {"fp32"}):
         dp.autotune_split(aform)
         for _ in range(reps):
             planning.objective_gradient(dp, D, Dp, wv, precision=aform)
