@@ -415,7 +415,7 @@ def main():
     D64, ms64, call64 = time_forward_fp64(dp, steps64, args.warmup)
     doses["fp64"] = D64
     per_form["fp64"] = {"ms_per_step": ms64, "fields_per_s": 1e3 / ms64, "call_us": call64 * 1e3,
-                        "tile": "32 x 128 voxels, all layers (FP64 FMA on the vector ALUs)",
+                        "tile": "16 x 128 voxels, all layers (FP64 FMA on the vector ALUs)",
                         "kernel_us": call64 * 1e3, "steps": steps64,
                         "kernel_achieved_tflop_s": flops / (call64 * 1e-3) / 1e12,
                         "kernel_frac_of_peak": flops / (call64 * 1e-3) / 1e12 / PEAK["fp64"],

@@ -13,23 +13,26 @@
 //   - the y factors of the band and chunk go to LDS (FP_TY x FP_SC);
 //   - thread t forms the FP_TY * FP_SC / 256 entries U[t / 64 + 4 q][t % 64] of U,
 //     summing W over the y chunks: each W value is read from global memory once
-//     (consecutive threads, consecutive spots i) and used for all 8 rows;
+//     (consecutive threads, consecutive spots i) and used for all its rows;
 //   - c U goes to LDS transposed (spot-major, rows contiguous), and thread t (column
-//     x = t % FP_TX, rows 16 (t / FP_TX) + r) evaluates Gx[x][i] once per spot and
-//     adds it times U[y][i] to its 16 voxels, reading the 16 U values as 8 pairs.
+//     x = t % FP_TX, rows FP_RPT (t / FP_TX) + r) evaluates Gx[x][i] once per spot and
+//     adds it times U[y][i] to its FP_RPT voxels, reading the U values in pairs.
+// Sixteen rows per workgroup: an exp serves 8 voxels, and the 128^3 grid gets 1,024
+// workgroups, four waves per SIMD (32 rows: 16 voxels per exp but two waves per SIMD,
+// 1.36x slower; 8 rows: 1.5x slower).
 // Every voxel sums its layers, chunks and spots in a fixed order, so results are
 // deterministic. No workspace, no layer chunks.
 #include "pbh_fwd_core.h"
 
 namespace pbh {
 
-constexpr int FP_TY = 32;          // rows per workgroup
+constexpr int FP_TY = 16;          // rows per workgroup (16: four waves per SIMD at 128^3)
 constexpr int FP_TX = 128;         // columns per workgroup
 constexpr int FP_SC = 64;          // spots per chunk and axis
 constexpr int FP_THREADS = 256;
-constexpr int FP_RPT = FP_TY * FP_TX / FP_THREADS;       // rows per thread in the accumulation (16)
-constexpr int FP_UPT = FP_TY * FP_SC / FP_THREADS;       // U entries per thread (8)
-static_assert(FP_RPT == 16 && FP_UPT == 8, "tile shape");
+constexpr int FP_RPT = FP_TY * FP_TX / FP_THREADS;       // rows per thread in the accumulation (8)
+constexpr int FP_UPT = FP_TY * FP_SC / FP_THREADS;       // U entries per thread (4)
+static_assert(FP_RPT == 8 && FP_UPT == 4, "tile shape");
 
 struct Fp64Args {
     int nx, ny, nz, nsx, nsy, L;
