@@ -226,7 +226,7 @@ def lut_chain(grad_idd, grad_sigma, didd, dsigma) -> torch.Tensor:
 
 class _Dose(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, w, delta, model: FieldModel, precision: str):
+    def forward(ctx, w, delta, model: FieldModel, precision: str, adjoint_precision: str):
         dp = model.dp
         ctx.dtab = None
         if delta is not None:
@@ -241,28 +241,36 @@ class _Dose(torch.autograd.Function):
         ctx.model, ctx.dp = model, dp
         ctx.save_for_backward(w, delta if delta is not None else torch.empty(0))
         ctx.has_delta = delta is not None
+        ctx.adjoint_precision = adjoint_precision
         return D
 
     @staticmethod
     def backward(ctx, gD):
         w, delta = ctx.saved_tensors
         need_delta = ctx.has_delta and ctx.needs_input_grad[1]
-        out = adjoint(ctx.dp, gD.contiguous().float(), w=w.detach(), lut_grads=need_delta)
+        # table gradients (for the range shifts) are computed in FP32 only
+        prec = "fp32" if need_delta else ctx.adjoint_precision
+        out = adjoint(ctx.dp, gD.contiguous().float(), w=w.detach(), lut_grads=need_delta, precision=prec)
         if need_delta:
             gw, gi, gs = out
             didd, dsig = ctx.dtab if ctx.dtab is not None else ctx.model.lut_derivatives(delta.detach())
             gdelta = lut_chain(gi, gs, didd, dsig).to(delta.dtype)
         else:
             gw, gdelta = out, None
-        return (gw if ctx.needs_input_grad[0] else None), gdelta, None, None
+        return (gw if ctx.needs_input_grad[0] else None), gdelta, None, None, None
 
 
 def dose(w: torch.Tensor, model: FieldModel, delta: Optional[torch.Tensor] = None,
-         precision: str = "fp32") -> torch.Tensor:
-    """Differentiable forward dose D(w, delta) (nz, ny, nx) float32 on the device."""
+         precision: str = "fp32", adjoint_precision: str = "fp32") -> torch.Tensor:
+    """Differentiable forward dose D(w, delta) (nz, ny, nx) float32 on the device.
+    ``precision``: the forward's arithmetic; ``adjoint_precision``: the backward's for
+    dJ/dw (one of ADJOINT_FORMS; the range-shift gradient, which needs the table
+    gradients, always runs the FP32 adjoint)."""
     if w.dtype != torch.float32:
         raise ValueError("spot weights must be float32")
-    return _Dose.apply(w, delta, model, precision)
+    if adjoint_precision not in ADJOINT_FORMS:
+        raise ValueError(f"adjoint precision {adjoint_precision!r}: one of {', '.join(ADJOINT_FORMS)}")
+    return _Dose.apply(w, delta, model, precision, adjoint_precision)
 
 
 def central_cube(grid: DoseGrid, half: float = 3.0, dose_value: float = 2.0, w_in: float = 1.0,

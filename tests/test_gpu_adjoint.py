@@ -217,6 +217,45 @@ def test_autograd_matches_torch_restatement_of_reference_engine():
     assert e <= 2e-5
 
 
+@pytest.mark.parametrize("adj", ["fp16x3", "bf16x3"])
+def test_dose_autograd_with_16bit_adjoint(adj):
+    """planning.dose(..., adjoint_precision) backward: dJ/dw from the 16-bit adjoint
+    vs torch autograd through the batched-matmul engine (FP64); with a range-shift
+    leaf the backward runs the FP32 adjoint (table gradients are FP32 only), so
+    dJ/dw then equals the FP32 backward's bit for bit."""
+    from oracle.torch_cpu import torch_separable_cpu
+    from pbdose import planning
+    print(f"case dose autograd with the {adj} adjoint", flush=True)
+    fm = planning.FieldModel(small_problem().grid, small_problem().lattice)
+    p = fm.problem   # the field's own device-built tables
+    Dp_np = np.zeros(p.grid.shape)
+    Dp_np[8:16, 12:24, 14:26] = 30.0
+    wv_np = np.where(Dp_np > 0, 1.0, 0.1)
+    wl = torch.tensor(p.lattice.weights.astype(np.float32).astype(np.float64), requires_grad=True)
+    Dt = torch_separable_cpu(p.grid.x, p.grid.y, p.lattice.spot_x, p.lattice.spot_y, wl, p.idd, p.sigma)
+    (torch.as_tensor(wv_np) * (Dt - torch.as_tensor(Dp_np)) ** 2).sum().backward()
+    Dp_t = torch.as_tensor(Dp_np, dtype=torch.float32, device=fm.dev)
+    wv_t = torch.as_tensor(wv_np, dtype=torch.float32, device=fm.dev)
+
+    def grads(adjoint_precision, with_delta):
+        wt = torch.as_tensor(p.lattice.weights, dtype=torch.float32, device=fm.dev).requires_grad_(True)
+        delta = (torch.zeros(p.lattice.n_layers, dtype=torch.float64, device=fm.dev, requires_grad=True)
+                 if with_delta else None)
+        D = planning.dose(wt, fm, delta, adjoint_precision=adjoint_precision)
+        (wv_t * (D - Dp_t) ** 2).sum().backward()
+        return wt.grad.clone(), (delta.grad.clone() if with_delta else None)
+
+    g16, _ = grads(adj, False)
+    e = rel(g16.cpu(), wl.grad.numpy())
+    print(f"  dose backward with the {adj} adjoint vs torch autograd (FP64): {e:.2e}", flush=True)
+    assert e <= {"fp16x3": 5e-6, "bf16x3": 3e-5}[adj]
+    g32, d32 = grads("fp32", True)
+    g16d, d16 = grads(adj, True)
+    assert torch.equal(g16d, g32) and torch.equal(d16, d32)
+    with pytest.raises(ValueError):
+        planning.dose(torch.zeros(1, dtype=torch.float32, device=fm.dev), fm, adjoint_precision="fp64")
+
+
 MIXED_ADJ_TOL = {"bf16x3": 3e-5, "bf16": 1e-2, "fp16x3": 5e-6, "fp16": 3e-3}
 
 
