@@ -410,12 +410,12 @@ def main():
                        "kernel_achieved_tflop_s": flops / (kern_ms * 1e-3) / 1e12,
                        "kernel_frac_of_peak": flops / (kern_ms * 1e-3) / 1e12 / PEAK[base_type(f)],
                        "accuracy_vs_fp64": None}
-    # the FP64 form: FP64 weights, tables, arithmetic and dose on the vector ALUs (one kernel)
+    # the FP64 form: FP64 weights, tables, arithmetic and dose (one kernel: FP64 MFMA on factor tables)
     steps64 = max(20, args.steps // 4)
     D64, ms64, call64 = time_forward_fp64(dp, steps64, args.warmup)
     doses["fp64"] = D64
     per_form["fp64"] = {"ms_per_step": ms64, "fields_per_s": 1e3 / ms64, "call_us": call64 * 1e3,
-                        "tile": "16 x 128 voxels, all layers (FP64 FMA on the vector ALUs)",
+                        "tile": "16 x 128 voxels, all layers (FP64 MFMA 16x16x4 on LDS factor tables)",
                         "kernel_us": call64 * 1e3, "steps": steps64,
                         "kernel_achieved_tflop_s": flops / (call64 * 1e-3) / 1e12,
                         "kernel_frac_of_peak": flops / (call64 * 1e-3) / 1e12 / PEAK["fp64"],

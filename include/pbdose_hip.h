@@ -197,7 +197,9 @@ int pbh_forward(const PbhGrid* g, const PbhLattice* lat,
                 void* workspace, size_t workspace_bytes, void* stream);
 
 /* The forward dose in FP64: weights w (L, nsy, nsx), tables idd, sigma (L, nz)
- * and dose (nz, ny, nx) all FP64, FP64 FMA on the vector ALUs, every voxel's
+ * and dose (nz, ny, nx) all FP64 (FP64 MFMA over per-tile factor tables where
+ * the voxel and spot pitches are small-integer multiples of one length, FP64 FMA
+ * on the vector ALUs otherwise), every voxel's
  * layers and spots summed in a fixed order (deterministic). No workspace; the
  * dose is overwritten. engines.py:208-283 torch_separable(dtype=torch.float64)
  * on a GPU (per_layer=True); model of engines.py:172-202. */

@@ -63,6 +63,16 @@ def test_spot_chunks_fp64(spots):
     check64(make_problem(64, 48, 4, 0.25, nsx, nsy, 0.15, [100.0, 160.0], seed=5, x0=0.37, y0=-0.21))
 
 
+@pytest.mark.parametrize("pitch", [(0.2, 0.5), (0.3, 0.45), (0.2, 0.2), (0.25, 0.15), (0.25, 0.37), (0.2, 0.123)])
+def test_pitch_ratios_fp64(pitch):
+    # voxel / spot pitches that are small-integer multiples of one length (factor tables:
+    # 2:5, 2:3, 1:1, 5:3) and ratios that are not (0.25 / 0.37, 0.2 / 0.123: every factor
+    # evaluated where it is used)
+    d, ds = pitch
+    print(f"case fp64 pitches {pitch}", flush=True)
+    check64(make_problem(72, 40, 6, d, 19, 13, ds, [95.0, 140.0, 175.0], seed=8, x0=0.11, y0=-0.07))
+
+
 def test_zero_idd_layers_and_zero_weights_fp64():
     print("case fp64 zero IDD beyond range, zero weights", flush=True)
     w = np.random.default_rng(2).random((3, 9, 9))
