@@ -52,24 +52,26 @@ __global__ void expand_kernel(int nz, double* idd_out, double* sigma_out)
     if (k > 0) sigma_out[i] = sigma_out[z];
 }
 
-// A(z)[e] = sum_l IDD_l(z) w_l[e] in FP64, stored as FP32 (nz x nspot).
-__global__ void amplitude_kernel(const double* w, const double* idd, int L, int nz, int nspot, float* amp)
+// A(z)[e] = sum_l IDD_l(z) w_l[e] in FP64, stored as T (nz x nspot).
+template <typename T>
+__global__ void amplitude_kernel(const double* w, const double* idd, int L, int nz, int nspot, T* amp)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)nz * nspot) return;
     const int z = (int)(i / nspot), e = (int)(i % nspot);
     double s = 0.0;
     for (int l = 0; l < L; ++l) s += idd[(size_t)l * nz + z] * w[(size_t)l * nspot + e];
-    amp[i] = (float)s;
+    amp[i] = (T)s;
 }
 
 }  // namespace pbh
 
 using namespace pbh;
 
-extern "C" int pbh_average_sigma(const double* w, const double* idd, const double* sigma, int n_layers, int nz,
-                                 int nsy, int nsx, float* w_avg, double* idd_avg, double* sigma_avg,
-                                 double* layer_sums, void* stream)
+template <typename T>
+static int average_sigma(const double* w, const double* idd, const double* sigma, int n_layers, int nz,
+                         int nsy, int nsx, T* w_avg, double* idd_avg, double* sigma_avg,
+                         double* layer_sums, void* stream)
 {
     if (!w || !idd || !sigma || !w_avg || !idd_avg || !sigma_avg || !layer_sums) return PBH_ERR_NULL_ARG;
     if (n_layers < 1 || n_layers > PBH_MAX_LAYERS || nz < 1 || nz > PBH_MAX_DIM || nsy < 1 || nsx < 1 ||
@@ -88,7 +90,21 @@ extern "C" int pbh_average_sigma(const double* w, const double* idd, const doubl
     hipLaunchKernelGGL(expand_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, nz, idd_avg, sigma_avg);
     if ((rc = launch_rc("expand_kernel")) != PBH_OK) return rc;
     const long long n = (long long)nz * nspot;
-    hipLaunchKernelGGL(amplitude_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, w, idd, n_layers, nz,
+    hipLaunchKernelGGL(amplitude_kernel<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, w, idd, n_layers, nz,
                        (int)nspot, w_avg);
     return launch_rc("amplitude_kernel");
+}
+
+extern "C" int pbh_average_sigma(const double* w, const double* idd, const double* sigma, int n_layers, int nz,
+                                 int nsy, int nsx, float* w_avg, double* idd_avg, double* sigma_avg,
+                                 double* layer_sums, void* stream)
+{
+    return average_sigma(w, idd, sigma, n_layers, nz, nsy, nsx, w_avg, idd_avg, sigma_avg, layer_sums, stream);
+}
+
+extern "C" int pbh_average_sigma_fp64(const double* w, const double* idd, const double* sigma, int n_layers, int nz,
+                                      int nsy, int nsx, double* w_avg, double* idd_avg, double* sigma_avg,
+                                      double* layer_sums, void* stream)
+{
+    return average_sigma(w, idd, sigma, n_layers, nz, nsy, nsx, w_avg, idd_avg, sigma_avg, layer_sums, stream);
 }

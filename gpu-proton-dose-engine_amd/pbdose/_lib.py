@@ -86,6 +86,7 @@ SIGNATURES = {
                                                 C.c_int, C.c_int, _P, C.c_size_t, _P]),
     "pbh_forward_fp64": (C.c_int, [_G, _L, _P, _P, _P, _P, _P]),
     "pbh_average_sigma": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P]),
+    "pbh_average_sigma_fp64": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P]),
     "pbh_forward_noalloc": (C.c_int, [_G, _L, _P, _P, _P, _P, C.c_int, C.c_int, _P]),
     "pbh_adjoint_workspace_bytes": (C.c_size_t, [_G, _L, C.c_int]),
     "pbh_adjoint": (C.c_int, [_G, _L, _P, _P, _P, _P, _P, _P, _P, C.c_int, _P, C.c_size_t, _P]),

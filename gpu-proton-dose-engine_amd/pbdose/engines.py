@@ -333,7 +333,7 @@ def torch_separable(problem: PBProblem, device: str = "cuda", dtype=None,
     device        must be a CUDA (ROCm) device; there is no CPU path.
     dtype         None or torch.float32: the dose in FP32, computed in FP32 or in
                   the mixed form ``precision`` selects. torch.float64: FP64
-                  weights, tables, arithmetic and dose (per_layer=True only).
+                  weights, tables, arithmetic and dose.
     allow_tf32    accepted for compatibility; gfx950 has no TF32 mode, so it
                   has no effect (the reference measured none either) and True warns.
     per_layer     True (default): sigma exact per layer. False: the reference's
@@ -348,8 +348,6 @@ def torch_separable(problem: PBProblem, device: str = "cuda", dtype=None,
     """
     if dtype not in (None, torch.float32, torch.float64):
         raise ValueError(f"dtype {dtype} not supported: float32 (default) or float64")
-    if dtype == torch.float64 and not per_layer:
-        raise ValueError("dtype=float64 computes the per-layer-exact dose; per_layer=False is float32 only")
     if dtype == torch.float64 and precision != "fp32":
         raise ValueError(f"precision {precision!r} selects a float32 form; dtype=float64 computes in FP64")
     if return_intermediates:
@@ -361,20 +359,22 @@ def torch_separable(problem: PBProblem, device: str = "cuda", dtype=None,
         warnings.warn("allow_tf32 has no effect: gfx950 has no TF32 mode; the dose is computed in the "
                       "form `precision` names", stacklevel=2)
     t0 = time.perf_counter()
-    dp = DeviceProblem(problem, device) if per_layer else averaged_problem(problem, device)
-    D = dp.forward_fp64() if dtype == torch.float64 else dp.forward(precision)
+    fp64 = dtype == torch.float64
+    dp = DeviceProblem(problem, device) if per_layer else averaged_problem(problem, device, fp64=fp64)
+    D = dp.forward_fp64() if fp64 else dp.forward(precision)
     torch.cuda.synchronize(dp.dev)
     dt = time.perf_counter() - t0
     return (D, dt) if return_timing else D
 
 
-def averaged_problem(problem: PBProblem, device="cuda") -> DeviceProblem:
+def averaged_problem(problem: PBProblem, device="cuda", fp64: bool = False) -> DeviceProblem:
     """The single-sigma approximation (per_layer=False, engines.py:184-191,
     :243-255) as a DeviceProblem of nz layers, written on the device by
     pbh_average_sigma: layer k carries the amplitude map A(z_k) = sum_l IDD_l(z_k)
     w_l, an IDD of 1 at slice k only, and sigma_bar on every slice, so that the
     forward gives slice z the dose Gy A(z) Gx^T with width sigma_bar(z). The
-    sigma_bar(z) of the reference (engines.py:75-80) is ``.sigma[0]``."""
+    sigma_bar(z) of the reference (engines.py:75-80) is ``.sigma[0]``. ``fp64``: the
+    amplitude maps are kept in FP64 (pbh_average_sigma_fp64) for ``forward_fp64``."""
     dev = _lib.require_cuda(device)
     g, lat = problem.grid, problem.lattice
     w = torch.as_tensor(lat.ensure_weights(), dtype=torch.float64, device=dev).contiguous()
@@ -383,13 +383,14 @@ def averaged_problem(problem: PBProblem, device="cuda") -> DeviceProblem:
     L, nz = lat.n_layers, g.nz
     if tuple(w.shape) != (L, lat.n_sy, lat.n_sx) or tuple(idd.shape) != (L, nz) or tuple(sig.shape) != (L, nz):
         raise ValueError("weights must be (L, nsy, nsx) and idd/sigma (L, nz)")
-    wa = torch.empty((nz, lat.n_sy, lat.n_sx), dtype=torch.float32, device=dev)
+    wa = torch.empty((nz, lat.n_sy, lat.n_sx), dtype=torch.float64 if fp64 else torch.float32, device=dev)
     ia = torch.empty((nz, nz), dtype=torch.float64, device=dev)
     sa = torch.empty((nz, nz), dtype=torch.float64, device=dev)
     wl = torch.empty((L,), dtype=torch.float64, device=dev)
-    _lib.check(_lib.lib().pbh_average_sigma(_lib.ptr(w), _lib.ptr(idd), _lib.ptr(sig), L, nz, lat.n_sy, lat.n_sx,
-                                            _lib.ptr(wa), _lib.ptr(ia), _lib.ptr(sa), _lib.ptr(wl),
-                                            _lib.stream_of(dev)), "pbh_average_sigma")
+    name = "pbh_average_sigma_fp64" if fp64 else "pbh_average_sigma"
+    _lib.check(getattr(_lib.lib(), name)(_lib.ptr(w), _lib.ptr(idd), _lib.ptr(sig), L, nz, lat.n_sy, lat.n_sx,
+                                         _lib.ptr(wa), _lib.ptr(ia), _lib.ptr(sa), _lib.ptr(wl),
+                                         _lib.stream_of(dev)), name)
     lat_a = dataclasses.replace(lat, n_layers=nz, energies=None, weights=None)
     pa = PBProblem(g, lat_a, problem.idd, problem.sigma, dict(problem.meta), problem.fluence_normalized)
     dp = DeviceProblem(pa, dev, weights=wa, idd=ia, sigma=sa)

@@ -295,6 +295,13 @@ int pbh_average_sigma(const double* w, const double* idd, const double* sigma, i
                       int nsy, int nsx, float* w_avg, double* idd_avg, double* sigma_avg,
                       double* layer_sums, void* stream);
 
+/* pbh_average_sigma with the amplitude maps w_avg (nz, nsy, nsx) kept in FP64, for
+ * pbh_forward_fp64: torch_separable(dtype=torch.float64, per_layer=False)
+ * (engines.py:243-255 in FP64). */
+int pbh_average_sigma_fp64(const double* w, const double* idd, const double* sigma, int n_layers, int nz,
+                           int nsy, int nsx, double* w_avg, double* idd_avg, double* sigma_avg,
+                           double* layer_sums, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Adjoint of the forward map (inverse planning).
  * ------------------------------------------------------------------------- */

@@ -151,9 +151,8 @@ def test_product_refuses_cpu_device():
                      ({"return_intermediates": True}, "return_intermediates")):
         with pytest.warns(UserWarning, match=text), pytest.raises(ValueError, match="GPU only"):
             engines.torch_separable(prob, "cpu", **kw)
-    # float64 is the per-layer-exact FP64 form; other dtypes and FP32-only options refuse
+    # float64 is the FP64 form; other dtypes and the float32 arithmetic forms refuse
     for kw, text in (({"dtype": torch.float16}, "not supported"),
-                     ({"dtype": torch.float64, "per_layer": False}, "per_layer=False"),
                      ({"dtype": torch.float64, "precision": "bf16x3"}, "FP64")):
         with pytest.raises(ValueError, match=text):
             engines.torch_separable(prob, "cuda", **kw)
