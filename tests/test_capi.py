@@ -80,15 +80,18 @@ def test_average_sigma_rejects_bad_arguments_without_launching():
     nul, one = C.c_void_p(0), C.c_void_p(16)
     # stand-in pointers, never dereferenced: each call below fails a check first
     args = [one, one, one, 3, 16, 9, 9, one, one, one, one, nul]
-    bad = list(args); bad[0] = nul
-    assert L.pbh_average_sigma(*bad) == -1
-    bad = list(args); bad[10] = nul
-    assert L.pbh_average_sigma(*bad) == -1
-    for k, v in ((3, 0), (4, 0), (5, -1), (6, 0), (4, 100000), (6, 5000)):
-        bad = list(args); bad[k] = v
-        assert L.pbh_average_sigma(*bad) == -2, (k, v)
-    bad = list(args); bad[4], bad[5], bad[6] = 4096, 1024, 1024      # nz x spots > 2^31
-    assert L.pbh_average_sigma(*bad) == -5
+    for f in (L.pbh_average_sigma, L.pbh_average_sigma_fp64):
+        bad = list(args); bad[0] = nul
+        assert f(*bad) == -1
+        bad = list(args); bad[7] = nul
+        assert f(*bad) == -1
+        bad = list(args); bad[10] = nul
+        assert f(*bad) == -1
+        for k, v in ((3, 0), (4, 0), (5, -1), (6, 0), (4, 100000), (6, 5000)):
+            bad = list(args); bad[k] = v
+            assert f(*bad) == -2, (k, v)
+        bad = list(args); bad[4], bad[5], bad[6] = 4096, 1024, 1024      # nz x spots > 2^31
+        assert f(*bad) == -5
 
 
 def test_forward_workspace_query():
